@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""RK4 steps/s at 256x256x129 on one MI355X: mixed precision (headline) and FP64.
+
+Workload: constant stratification, Lx = Ly = 200 km, Lz = 1300 m,
+N0 = 5.2e-3 s^-1, latitude 33 deg, Nj = 85 (2/3 rule), nonhydrostatic,
+antialiased; seeded synthetic flow (|k|^-2 amplitudes, j <= 64, half wave /
+half geostrophic energy, rms horizontal speed 5 cm/s); fixed dt = 0.5 CFL.
+A "step" is one classic RK4 step (four nonlinear-flux evaluations plus the
+stage combinations) with the state resident in HBM.
+
+Prints one JSON line.  value = mixed-precision RK4 steps/s; the FP64 rate,
+the mixed-vs-FP64 relative total-energy error after the timed steps, the
+dominant kernel's roofline (HIP events, live) and the CPU oracle baseline
+ride along.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "wave-vortex-model_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = "RK4 steps/s at 256×256×129: mixed vs FP64 GPU at ≤1e-6 rel energy error"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_config():
+    import wvamd
+    return wvamd.make_config(256, 256, 129, 85, Lx=200e3, Ly=200e3, Lz=1300.0, N0=5.2e-3, latitude=33.0)
+
+
+def initial_state(k, seed=4001, jmax=64, urms=0.05):
+    from wvamd import flows
+    md = k.modes()
+    Ap, Am, A0 = flows.random_flow_shape(md["Kh"], k.Nj, k.coefficient_table("ApmTE"), k.coefficient_table("A0TE"),
+                                         seed=seed, jmax=jmax)
+    f = k.transform_wave_vortex_to_uvw(*[k.to_device(a) for a in (Ap, Am, A0)]).cpu().numpy()
+    s = flows.scale_factor_for_rms(f, k.config["Lz"], urms)
+    Ap, Am, A0 = Ap * s, Am * s, A0 * s
+    f = k.transform_wave_vortex_to_uvw(*[k.to_device(a) for a in (Ap, Am, A0)]).cpu().numpy()
+    dt = flows.cfl_time_step(f, md["k"], md["l"], k.coefficient_table("omega"), k.config["Lz"], 0.5)
+    return (Ap, Am, A0), dt
+
+
+def timed_steps(k, state, dt, steps, warmup):
+    import torch
+    dev = [k.to_device(a) for a in state]
+    t = k.rk4_advance(*dev, t=0.0, h=dt, nsteps=warmup) if warmup else 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k.rk4_advance(*dev, t=t, h=dt, nsteps=steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    return elapsed, dev
+
+
+def stage_profile(k, state, dt, steps=2):
+    """Per-kernel device time from HIP events recorded on the launch stream."""
+    import torch
+    dev = [k.to_device(a) for a in state]
+    k.rk4_advance(*dev, t=0.0, h=dt, nsteps=1)
+    k.reset_stage_timings()
+    k.set_stage_instrumentation(True)
+    k.rk4_advance(*dev, t=0.0, h=dt, nsteps=steps)
+    torch.cuda.synchronize()
+    rec = k.stage_timings()
+    k.set_stage_instrumentation(False)
+    return rec
+
+
+def cpu_baseline(cfg, state, dt, threads):
+    """Oracle FP64 RK4 step at the full configuration on the host cores."""
+    from oracle import Oracle
+    Oracle.set_threads(threads)
+    o = Oracle(cfg)
+    t0 = time.perf_counter()
+    o.rk4(*state, t=0.0, dt=dt, nsteps=1)
+    sec = time.perf_counter() - t0
+    return {"value": 1.0 / sec, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"1 FP64 RK4 step (4 fluxes) of the oracle at 256x256x129, {sec:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this path runs on a single MI355X "
+              "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import wvamd
+
+    cfg = workload_config()
+    k64 = wvamd.WVTransformConstantStratificationKernel(cfg, "fp64")
+    k32 = wvamd.WVTransformConstantStratificationKernel(cfg, "mixed")
+    state, dt = initial_state(k64)
+
+    sec32, d32 = timed_steps(k32, state, dt, args.steps, args.warmup)
+    sec64, d64 = timed_steps(k64, state, dt, args.steps, args.warmup)
+    e32 = k64.total_energy(*d32)
+    e64 = k64.total_energy(*d64)
+    energy_err = abs(e32 - e64) / abs(e64)
+    sps32 = args.steps / sec32
+    sps64 = args.steps / sec64
+
+    rec = stage_profile(k32, state, dt)
+    dom = max(rec, key=lambda r: r["total_ms"])
+    avg_ms = dom["total_ms"] / max(dom["launches"], 1)
+    achieved = dom["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+                "avg_launch_ms": round(avg_ms, 4)}
+    stages = {r["name"]: round(r["total_ms"] / max(r["launches"], 1), 4) for r in rec}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(cfg, state, dt, threads)
+
+    out = {
+        "metric": METRIC, "value": round(sps32, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * sec32 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 fields/FFTs/MFMA, f64 coefficients+RK4",
+        "data": "synthetic (seeded |k|^-2 flow, rms u 5 cm/s, dt = 0.5 CFL)",
+        "config": {"workload": "wave-vortex nonlinear flux + fixed RK4, 256x256x129 nonhydrostatic constant N",
+                   "Lx_m": 200e3, "Lz_m": 1300.0, "Nj": 85, "Nkl": k64.Nkl, "dt_s": dt},
+        "fp64_steps_per_s": round(sps64, 3), "fp64_ms_per_step": round(1e3 * sec64 / args.steps, 4),
+        "mixed_over_fp64": round(sps32 / sps64, 3), "energy_rel_err_mixed_vs_fp64": energy_err,
+        "roofline": roofline, "stage_ms_per_launch_mixed": stages, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,171 @@
+/*
+ * wv_hip.h -- C-ABI of the MI355X (gfx950) wave-vortex nonlinear-flux / RK4
+ * library (libwvamd.so).
+ *
+ * The reference exposes this path as the C++ class contract
+ * wavevortex::WVTransformConstantStratificationKernel (contract version 4,
+ * CompiledKernel/include/WaveVortexKernel/WVTransformConstantStratificationKernel.hpp:47-110)
+ * and the fixed-step integrator wavevortex::runtime::WVFixedStepRK4
+ * (PortableRuntime/include/WaveVortexRuntime/WVRungeKutta.hpp).  Every entry
+ * point below names the reference member it replaces.  All entry points use
+ * plain pointers and sizes, return a wv_status code (never throw) and leave a
+ * message readable with wv_last_error().
+ *
+ * Layouts are the reference's (WVKernelTypes.hpp:66-127):
+ *   coefficients  [Nj, Nkl] column-major complex (offset j + Nj*mode), stored as
+ *                 interleaved (re, im) doubles (WVComplex64);
+ *   spatial fields [Nx, Ny, Nz, Nfield] column-major doubles (x fastest).
+ *
+ * "_device" entry points take device pointers (hipMalloc'd, resident in HBM)
+ * and enqueue on `stream` (a hipStream_t; NULL = the default stream); they
+ * return once the work is enqueued, except where a result is returned on
+ * the host (energy, rk4 time), which synchronizes the stream.  The entry
+ * points without the suffix take host pointers, copy in and out, and
+ * synchronize (the reference's host-memory semantics).
+ *
+ * Like the reference kernel, a wv_kernel is not reentrant: concurrent calls on
+ * one handle return WV_REENTRANT_EXECUTION.  Distinct handles are independent.
+ */
+#ifndef WV_HIP_H
+#define WV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WV_CONTRACT_VERSION 4u /* WVKernelTypes.hpp:11 WVKernelContractVersion */
+
+/* WVKernelStatusCode, WVKernelTypes.hpp:22-35 (same values and order). */
+typedef enum wv_status {
+    WV_SUCCESS = 0,
+    WV_INVALID_CONFIGURATION,
+    WV_INVALID_SHAPE,
+    WV_INVALID_POINTER,
+    WV_OVERLAPPING_ARRAYS,
+    WV_SIZE_OVERFLOW,
+    WV_ALLOCATION_FAILURE,
+    WV_FFT_PLAN_FAILURE,
+    WV_FFT_EXECUTION_FAILURE,
+    WV_NUMERICAL_FAILURE,
+    WV_UNSUPPORTED_OPERATION,
+    WV_REENTRANT_EXECUTION
+} wv_status;
+
+/* Arithmetic used inside the flux.  FP64 everywhere, or FP32 spatial fields,
+ * spectra, FFTs and vertical transforms with FP64 coefficients, RK4 state and
+ * reductions. */
+typedef enum wv_precision { WV_PRECISION_FP64 = 0, WV_PRECISION_MIXED = 1 } wv_precision;
+
+/* WVTransformConstantStratificationConfiguration, WVKernelTypes.hpp:129-146. */
+typedef struct wv_config {
+    uint32_t contractVersion;
+    size_t Nx, Ny, Nz, Nj;
+    double Lx, Ly, Lz, N0, rho0, g, planetaryRadius, rotationRate, latitude;
+    int32_t isHydrostatic;
+    int32_t shouldAntialias;
+} wv_config;
+
+typedef struct wv_kernel wv_kernel;
+
+/* Per-stage device time of the last calls (WVKernelMetrics, ...Kernel.hpp:12-45),
+ * filled from HIP events when stage instrumentation is on. */
+typedef struct wv_stage_timing {
+    char name[32];
+    double total_ms;
+    uint64_t launches;
+    double bytes_per_launch; /* algorithmic HBM bytes one launch moves */
+} wv_stage_timing;
+
+/* --- lifetime ----------------------------------------------------------- */
+/* WVTransformConstantStratificationKernel::create (...Kernel.cpp:442-491). The
+ * FFT engine argument of the reference has no counterpart: the transforms are
+ * this library's own HIP kernels.  `device` selects the HIP device. */
+wv_status wv_create(const wv_config* config, wv_precision precision, int device, wv_kernel** kernel);
+void wv_destroy(wv_kernel* kernel);
+/* Message of the last failing call on this thread (or of the last create). */
+const char* wv_last_error(void);
+uint32_t wv_contract_version(void);
+
+/* --- descriptor (WVTransformConstantStratificationDescriptor, WVKernelTypes.hpp:214-233) */
+size_t wv_nkl(const wv_kernel* kernel);
+double wv_coriolis_frequency(const wv_kernel* kernel);
+/* fourierModes() + halfSpectrumMappings() per WV mode; any pointer may be NULL. */
+wv_status wv_descriptor_modes(const wv_kernel* kernel, int64_t* kMode, int64_t* lMode, double* k, double* l,
+                              double* Kh, double* cosAlpha, double* sinAlpha, int64_t* storageRow,
+                              uint8_t* conjugated);
+/* verticalModes() tables by name ("omega", "UApField", ..., plus the total-energy
+ * factors "ApmTE"/"A0TE"); complex tables return 2*count doubles.  Returns the
+ * number of doubles written, -1 if unknown.  out may be NULL to query the size. */
+long wv_descriptor_table(const wv_kernel* kernel, const char* name, double* out);
+/* bytes held on the device by this kernel (descriptor tables + workspaces);
+ * WVTransformConstantStratificationKernel::persistentBytes (...Kernel.cpp:514-519). */
+size_t wv_persistent_bytes(const wv_kernel* kernel);
+
+/* --- the flux ---------------------------------------------------------------
+ * WVTransformConstantStratificationKernel::nonlinearFlux (...Kernel.cpp:1140-1146,
+ * 1173-1244): F = the wave-vortex projection of -(u . grad)(u,v,w,eta), evaluated
+ * at time t and rotated back to t0.  Outputs may not overlap each other or the
+ * inputs (validateStateAndFlux, WVKernelTypes.cpp:343-362). */
+wv_status wv_nonlinear_flux_device(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                                   const double* A0, double* Fp, double* Fm, double* F0, void* stream);
+wv_status wv_nonlinear_flux(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                            const double* A0, double* Fp, double* Fm, double* F0);
+
+/* nonlinearFluxWithAdvectionFields (...Kernel.cpp:1148-1158): also returns the
+ * advecting (u,v,w) as a [Nx,Ny,Nz,3] bundle. */
+wv_status wv_nonlinear_flux_with_advection_fields_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                         const double* Am, const double* A0, double* Fp,
+                                                         double* Fm, double* F0, double* uvw, void* stream);
+
+/* --- transforms ------------------------------------------------------------- */
+/* transformWaveVortexToUVWEta (...Kernel.cpp:686-694,745-776): fields [Nx,Ny,Nz,4]. */
+wv_status wv_transform_wave_vortex_to_uvweta_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                    const double* Am, const double* A0, double* fields,
+                                                    void* stream);
+/* transformWaveVortexToUVW (...Kernel.cpp:696-717): fields [Nx,Ny,Nz,3]. */
+wv_status wv_transform_wave_vortex_to_uvw_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                 const double* Am, const double* A0, double* fields,
+                                                 void* stream);
+/* transformUVWEtaToWaveVortex (nonhydrostatic, fields [..,4]) and
+ * transformUVEtaToWaveVortex (hydrostatic, fields [..,3]) (...Kernel.cpp:583-684). */
+wv_status wv_transform_uvweta_to_wave_vortex_device(wv_kernel* kernel, const double* fields, double t, double t0,
+                                                    double* Ap, double* Am, double* A0, void* stream);
+wv_status wv_transform_uveta_to_wave_vortex_device(wv_kernel* kernel, const double* fields, double t, double t0,
+                                                   double* Ap, double* Am, double* A0, void* stream);
+
+/* transformToSpatialDomainWithFAllDerivatives / ...GAllDerivatives (...Kernel.cpp:819-863):
+ * the F-family (cosine) or G-family (sine) field Apm*waveScale + A0*geostrophicScale
+ * and its x, y, z derivatives, fields [Nx,Ny,Nz,4] = (value, d/dx, d/dy, d/dz). */
+wv_status wv_transform_to_spatial_domain_with_f_all_derivatives_device(wv_kernel* kernel, const double* Apm,
+                                                                       const double* A0, double* fields,
+                                                                       void* stream);
+wv_status wv_transform_to_spatial_domain_with_g_all_derivatives_device(wv_kernel* kernel, const double* Apm,
+                                                                       const double* A0, double* fields,
+                                                                       void* stream);
+
+/* --- fixed-step RK4 ----------------------------------------------------------
+ * WVFixedStepRK4::step / advanceToTime (PortableRuntime/src/WVRungeKutta.cpp:335-480)
+ * for the nonlinear-advection system: nsteps classic RK4 steps of size h, state
+ * (Ap, Am, A0) updated in place on the device, *t advanced on the host.  The
+ * stage combinations are fused into the flux projection. */
+wv_status wv_rk4_advance_device(wv_kernel* kernel, double* Ap, double* Am, double* A0, double* t, double t0,
+                                double h, int nsteps, void* stream);
+
+/* Total energy sum(ApmTE*(|Ap|^2+|Am|^2) + A0TE*|A0|^2), FP64 reduction
+ * (@WVTransformConstantStratification/WVTransformConstantStratification.m:374-376). */
+wv_status wv_total_energy_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0,
+                                 double* energy, void* stream);
+
+/* --- instrumentation (setStageInstrumentation, ...Kernel.hpp:68) ----------- */
+wv_status wv_set_stage_instrumentation(wv_kernel* kernel, int enabled);
+/* Copies up to `capacity` stage records; returns the number of stages. */
+int wv_stage_timings(wv_kernel* kernel, wv_stage_timing* out, int capacity);
+void wv_reset_stage_timings(wv_kernel* kernel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WV_HIP_H */

@@ -33,8 +33,8 @@ class Config(C.Structure):
 def make_config(Nx, Ny, Nz, Nj=None, Lx=50e3, Ly=None, Lz=1300.0, N0=5.2e-3, rho0=1025.0,
                 g=9.81, planetaryRadius=6.371e6, rotationRate=7.2921e-5, latitude=33.0,
                 isHydrostatic=False, shouldAntialias=True):
-    if Nj is None:
-        Nj = max(1, (2 * ((Nz - 1) // 2)) // 3) if Nz > 3 else 1
+    if Nj is None:  # WVGeometryDoublyPeriodicStratifiedConstant.m:61-68
+        Nj = (2 * (Nz - 1)) // 3 if shouldAntialias and Nz - 1 > 3 else Nz - 1
     if Ly is None:
         Ly = Lx
     return dict(Nx=Nx, Ny=Ny, Nz=Nz, Nj=Nj, Lx=Lx, Ly=Ly, Lz=Lz, N0=N0, rho0=rho0, g=g,

@@ -1,0 +1,1482 @@
+// MI355X (gfx950) implementation of the wave-vortex nonlinear flux and the
+// fixed-step RK4 that drives it.  Host entry points are the C-ABI declared in
+// include/wv_hip.h.
+//
+// One right-hand side (reference: WVTransformConstantStratificationKernel::
+// nonlinearFluxImpl, CompiledKernel/src/WVTransformConstantStratificationKernel.cpp:1173-1244)
+// runs as six kernels, all resident in HBM:
+//
+//   k_fields   per coefficient: phase exp(i w (t-t0)) and the evolved field
+//              spectra u, v, w, eta (WVCoefficientFormulas.hpp:15-33)
+//   k_synth    per retained half-spectrum column: spectral assembly of the 15
+//              (12 hydrostatic) channels -- U, V, W and d/dx, d/dy, d/dz of
+//              every target field (...Kernel.cpp:246-291) -- and the vertical
+//              DCT-I / DST-I as dense MFMA products over the Nj retained modes
+//   k_yfft     inverse complex FFT along y on [Ny, kx-tile] LDS tiles
+//   k_xfused   per (y, z) row: c2r of all channels along x, the advection
+//              products F_T = -(U dT/dx + V dT/dy + W dT/dz) (...Kernel.cpp:1230)
+//              and r2c of the products, all in LDS; spatial fields never
+//              reach HBM
+//   k_yfft     forward complex FFT along y
+//   k_proj     per coefficient tile: vertical forward DCT-I / DST-I with its
+//              normalization as MFMA products over z, the wave-vortex
+//              projection of every target (...Kernel.cpp:396-431) and either
+//              the flux store or the fused RK4 stage update
+//              (PortableRuntime/src/WVRungeKutta.cpp:335-462)
+//
+// Half-spectrum data are kept only for kx < nkx (the largest retained kx)
+// in [channel][z][iy][kx] order, so y-FFT tiles read and write contiguous kx
+// runs and the x transforms read contiguous rows.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wv_descriptor.hpp"
+#include "wv_device.hpp"
+#include "wv_hip.h"
+
+namespace wvamd {
+
+// ------------------------------------------------------------ device data --
+struct ColumnDev {
+    int iy, kx, wv;
+    unsigned flags;
+    double k, l, Kh, cosA, sinA;
+};
+
+// One synthesis group: up to four output channels derived from the evolved
+// spectra of the fields in `field` (0=u, 1=v, 2=w, 3=eta) by `op`
+// (0 = the field itself, 1 = d/dx, 2 = d/dy, 3 = d/dz), each followed by the
+// DCT-I (fam 0) or DST-I (fam 1).  Every channel carries the factor 1/2 of
+// the reference assembly and is zeroed at the vertical endpoints when its
+// output family is the sine family (...Kernel.cpp:240,286-288 and
+// WVCoefficientFormulas.hpp:42-58).
+struct SynthGroup {
+    int nch;
+    int field[4];
+    int op[4];
+    int fam[4];
+    int outch[4];
+};
+constexpr int kMaxGroups = 8;
+struct SynthPlan {
+    int ngroups;
+    SynthGroup g[kMaxGroups];
+};
+
+enum EpilogueMode : int { kFluxOut = 0, kRk1 = 1, kRk2 = 2, kRk3 = 3, kRk4 = 4 };
+
+struct ProjTables {
+    const double* omega;
+    const double* A0Vort;
+    const double* A0Buoy;
+    const double* NA0;
+    const double* ApmN;
+    const double* ApmDScaled;
+    const cx<double>* ApmD;
+    const double* apmW;
+    const double* inertialScale;
+};
+
+struct Epilogue {
+    int mode;
+    double scale;  // 0.5h (stages 1,2), h (stage 3), h/6 (stage 4)
+    cx<double>* out0;
+    cx<double>* out1;
+    cx<double>* out2;
+    cx<double>* y0;
+    cx<double>* y1;
+    cx<double>* y2;
+    cx<double>* st0;
+    cx<double>* st1;
+    cx<double>* st2;
+    cx<double>* w0;
+    cx<double>* w1;
+    cx<double>* w2;
+};
+
+// ----------------------------------------------------------------- kernels -
+
+// Evolved field spectra per coefficient (WVCoefficientFormulas.hpp:15-33).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fields(const cx<double>* __restrict__ Ap, const cx<double>* __restrict__ Am,
+                                                const cx<double>* __restrict__ A0, const double* __restrict__ omega,
+                                                const cx<double>* __restrict__ UAp, const cx<double>* __restrict__ VAp,
+                                                const cx<double>* __restrict__ WAp, const double* __restrict__ NAp,
+                                                const cx<double>* __restrict__ UA0, const cx<double>* __restrict__ VA0,
+                                                const double* __restrict__ NA0, double elapsed, int M,
+                                                cx<T>* __restrict__ FV) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M) return;
+    double sn, cs;
+    sincos(omega[idx] * elapsed, &sn, &cs);
+    const cx<double> p{cs, sn};
+    const cx<double> ap = cmul(Ap[idx], p), am = cmul(Am[idx], cconj(p)), a0 = A0[idx];
+    const cx<double> u = cadd(cadd(cmul(UAp[idx], ap), cmul(cconj(UAp[idx]), am)), cmul(UA0[idx], a0));
+    const cx<double> v = cadd(cadd(cmul(VAp[idx], ap), cmul(cconj(VAp[idx]), am)), cmul(VA0[idx], a0));
+    const cx<double> w = cadd(cmul(WAp[idx], ap), cmul(WAp[idx], am));
+    const cx<double> e = cadd(cadd(cscale(ap, NAp[idx]), cscale(am, -NAp[idx])), cscale(a0, NA0[idx]));
+    FV[idx] = {T(u.re), T(u.im)};
+    FV[M + idx] = {T(v.re), T(v.im)};
+    FV[2 * M + idx] = {T(w.re), T(w.im)};
+    FV[3 * M + idx] = {T(e.re), T(e.im)};
+}
+
+// F- or G-family field spectrum Apm*waveScale + A0*geostrophicScale per
+// coefficient (fieldFamilyDerivativeSpectrum, WVCoefficientFormulas.hpp:60-73).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fgfields(const cx<double>* __restrict__ Apm, const cx<double>* __restrict__ A0,
+                                                  const double* __restrict__ fWaveScale,
+                                                  const double* __restrict__ gWaveScale, const double* __restrict__ Fg,
+                                                  const double* __restrict__ Gg, int Nj, int M, int fFamily,
+                                                  cx<T>* __restrict__ FV) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M) return;
+    const int j = idx % Nj;
+    const double ws = fFamily ? fWaveScale[idx] : gWaveScale[j];
+    const double gs = fFamily ? Fg[j] : Gg[j];
+    const cx<double> v = cadd(cscale(Apm[idx], ws), cscale(A0[idx], gs));
+    FV[idx] = {T(v.re), T(v.im)};
+}
+
+// Spectral assembly + vertical synthesis (MFMA).  One wave computes, for 8
+// consecutive columns and one group, ZT tiles of 16 z levels:
+//   H[ch][z][iy][kx] = sum_j syn_fam(ch)[z][j] * value_ch(j, column).
+// Complex values are two real GEMM columns; lane l owns GEMM column l%16
+// (column l%32/2, real or imaginary part) and k-row l/16.
+template <typename T, int ZT>
+__global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
+                                               int ncol, int M, const T* __restrict__ synC,
+                                               const T* __restrict__ synS, int Zs, int Js, int Nz, int Nj, int Ny,
+                                               int nkxP, const T* __restrict__ mj, SynthPlan plan,
+                                               cx<T>* __restrict__ H) {
+    using MF = Mfma<T>;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int ncg = (ncol + 7) / 8;
+    const int ztiles = Zs / 16;
+    const int nzc = (ztiles + ZT - 1) / ZT;
+    if (wave >= ncg * plan.ngroups * nzc) return;
+    const int cg = wave % ncg;
+    const int rest = wave / ncg;
+    const int gi = rest % plan.ngroups;
+    const int zc = rest / plan.ngroups;
+    const SynthGroup& G = plan.g[gi];
+    const int n = lane & 15, kq = lane >> 4;
+    const int col = cg * 8 + (n >> 1);
+    const int part = n & 1;
+    const bool colValid = col < ncol;
+    ColumnDev cd{};
+    if (colValid) cd = cols[col];
+    const int zt0 = zc * ZT;
+
+    typename MF::acc acc[4][ZT];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int q = 0; q < ZT; ++q) acc[c][q] = MF::zero();
+
+    const int nsteps = Js / 4;
+    for (int s = 0; s < nsteps; ++s) {
+        const int j = 4 * s + kq;
+        T b[4] = {T(0), T(0), T(0), T(0)};
+        if (colValid && j < Nj) {
+            const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
+            const bool endpoint = (j == 0 || j + 1 == Nz);
+            const T half = T(0.5);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (c < G.nch) {
+                    const cx<T> f = FV[static_cast<size_t>(G.field[c]) * M + idx];
+                    const bool sineOut = G.fam[c] == 1;
+                    cx<T> val;
+                    switch (G.op[c]) {
+                        case 0: val = f; break;
+                        case 1: val = cmuli(f, T(cd.k)); break;
+                        case 2: val = cmuli(f, T(cd.l)); break;
+                        default: val = cscale(f, sineOut ? -mj[j] : mj[j]); break;
+                    }
+                    val = (sineOut && endpoint) ? cx<T>{T(0), T(0)} : cscale(val, half);
+                    if (cd.flags & kColConj) val.im = -val.im;
+                    if (cd.flags & kColSelfConj) val.im = T(0);
+                    b[c] = part ? val.im : val.re;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < ZT; ++q) {
+            const int zt = zt0 + q;
+            if (zt < ztiles) {
+                const int a_off = (16 * zt + n) * Js + 4 * s + kq;
+                const T aC = synC[a_off];
+                const T aS = synS[a_off];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (c < G.nch) acc[c][q] = MF::run(G.fam[c] ? aS : aC, b[c], acc[c][q]);
+            }
+        }
+    }
+    if (!colValid) return;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (c >= G.nch) continue;
+#pragma unroll
+        for (int q = 0; q < ZT; ++q) {
+            const int zt = zt0 + q;
+            if (zt >= ztiles) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int z = 16 * zt + 4 * kq + i;
+                if (z < Nz) {
+                    T* dst = reinterpret_cast<T*>(
+                        &H[((static_cast<size_t>(G.outch[c]) * Nz + z) * Ny + cd.iy) * nkxP + cd.kx]);
+                    dst[part] = acc[c][q][i];
+                }
+            }
+        }
+    }
+}
+
+// Complex FFT along y of [Ny, KT] tiles: in/out layout [ch][z][iy][nkxP].
+template <typename T>
+__global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int Ny,
+                                              int log2Ny, int nkxP, int KT, int Nz, const cx<T>* __restrict__ twY,
+                                              int sign) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+    const int kx0 = blockIdx.x * KT;
+    const size_t plane = (static_cast<size_t>(blockIdx.z) * Nz + blockIdx.y) * static_cast<size_t>(Ny) * nkxP;
+    const int nel = Ny * KT;
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        const int kk = e % KT, iy = e / KT;
+        tile[e] = in[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk];
+    }
+    __syncthreads();
+    fftLds(tile, Ny, log2Ny, KT, 1, KT, true, twY, sign);
+    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        const int kk = e % KT, iy = e / KT;
+        out[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk] = tile[e];
+    }
+}
+
+// Hermitian-extended x spectrum of channel `ch` at row (z, y): the c2r input
+// of WVReferenceFFTEngine.cpp:65-86 (imaginary part of kx = 0 ignored).
+template <typename T>
+__device__ __forceinline__ cx<T> hermitianX(const cx<T>* __restrict__ G, size_t rowBase, size_t chStride, int ch,
+                                            int k, int Nx, int nkx) {
+    const cx<T>* row = G + ch * chStride + rowBase;
+    if (k == 0) return {row[0].re, T(0)};
+    if (k < nkx) return row[k];
+    if (Nx - k < nkx) return cconj(row[Nx - k]);
+    return {T(0), T(0)};
+}
+
+// Per (y, z) row: c2r of the NCH channels along x (two real transforms per
+// complex FFT), the NT advection products and r2c of the products.
+template <typename T, int NCH, int NT>
+__global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P, int Nx,
+                                                int log2Nx, int Ny, int Nz, int nkx, int nkxP,
+                                                const cx<T>* __restrict__ twX) {
+    constexpr int PAIRS = (NCH + 1) / 2;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int y = blockIdx.x, z = blockIdx.y;
+    const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
+    const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
+    for (int e = threadIdx.x; e < PAIRS * Nx; e += blockDim.x) {
+        const int p = e / Nx, k = e % Nx;
+        const cx<T> A = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
+        const cx<T> B = (2 * p + 1 < NCH) ? hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx) : cx<T>{T(0), T(0)};
+        buf[e] = {A.re - B.im, A.im + B.re};
+    }
+    __syncthreads();
+    fftLds(buf, Nx, log2Nx, 1, Nx, PAIRS, false, twX, +1);
+    for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+        T ch[2 * PAIRS];
+#pragma unroll
+        for (int p = 0; p < PAIRS; ++p) {
+            const cx<T> v = buf[p * Nx + x];
+            ch[2 * p] = v.re;
+            ch[2 * p + 1] = v.im;
+        }
+        const T U = ch[0], V = ch[1], W = ch[2];
+        T F[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int t = 0; t < NT; ++t) F[t] = -(U * ch[3 + 3 * t] + V * ch[4 + 3 * t] + W * ch[5 + 3 * t]);
+        buf[x] = {F[0], F[1]};
+        buf[Nx + x] = {F[2], F[3]};
+    }
+    __syncthreads();
+    fftLds(buf, Nx, log2Nx, 1, Nx, 2, false, twX, -1);
+    const size_t tStride = chStride;
+    for (int e = threadIdx.x; e < 2 * nkx; e += blockDim.x) {
+        const int q = e / nkx, k = e % nkx;
+        const cx<T> Za = buf[q * Nx + k];
+        const cx<T> Zb = buf[q * Nx + (k == 0 ? 0 : Nx - k)];
+        const T h = T(0.5);
+        const cx<T> Fa{(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
+        const cx<T> Fb{(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
+        if (2 * q < NT) P[(2 * q) * tStride + rowBase + k] = Fa;
+        if (2 * q + 1 < NT) P[(2 * q + 1) * tStride + rowBase + k] = Fb;
+    }
+}
+
+// c2r along x of nch channels to FP64 real fields [Nx,Ny,Nz,nch] (inverse transforms).
+template <typename T>
+__global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, double* __restrict__ out, int nch, int Nx,
+                                              int log2Nx, int Ny, int Nz, int nkx, int nkxP,
+                                              const cx<T>* __restrict__ twX) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int pairs = (nch + 1) / 2;
+    const int y = blockIdx.x, z = blockIdx.y;
+    const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
+    const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
+    for (int e = threadIdx.x; e < pairs * Nx; e += blockDim.x) {
+        const int p = e / Nx, k = e % Nx;
+        const cx<T> A = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
+        const cx<T> B = (2 * p + 1 < nch) ? hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx) : cx<T>{T(0), T(0)};
+        buf[e] = {A.re - B.im, A.im + B.re};
+    }
+    __syncthreads();
+    fftLds(buf, Nx, log2Nx, 1, Nx, pairs, false, twX, +1);
+    const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
+    for (int e = threadIdx.x; e < nch * Nx; e += blockDim.x) {
+        const int c = e / Nx, x = e % Nx;
+        const cx<T> v = buf[(c / 2) * Nx + x];
+        out[c * R + (static_cast<size_t>(z) * Ny + y) * Nx + x] = static_cast<double>((c & 1) ? v.im : v.re);
+    }
+}
+
+// Vertical analysis (MFMA) + wave-vortex projection + flux store or RK4 stage.
+// One wave: 8 consecutive columns x 16 vertical modes j, all targets.
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_proj(const cx<T>* __restrict__ Q, const ColumnDev* __restrict__ cols,
+                                              int ncol, const T* __restrict__ anaC, const T* __restrict__ anaS,
+                                              int Ja, int Za, int Nz, int Nj, int Ny, int nkxP,
+                                              int targetCode, int hydro, ProjTables tb, double elapsed,
+                                              double hscale, Epilogue ep) {
+    using MF = Mfma<T>;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int ncg = (ncol + 7) / 8;
+    const int njt = Ja / 16;
+    if (wave >= ncg * njt) return;
+    const int cg = wave % ncg;
+    const int jt = wave / ncg;
+    const int n = lane & 15, kq = lane >> 4;
+    const int col = cg * 8 + (n >> 1);
+    const int part = n & 1;
+    const bool colValid = col < ncol;
+    ColumnDev cd{};
+    if (colValid) cd = cols[col];
+    const size_t tStride = static_cast<size_t>(Nz) * Ny * nkxP;
+    const size_t colOff = static_cast<size_t>(cd.iy) * nkxP + cd.kx;
+
+    typename MF::acc acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = MF::zero();
+    const int nsteps = Za / 4;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int target = (targetCode >> (4 * t)) & 0xF;
+        const T* A = target < 2 ? anaC : anaS;
+        typename MF::acc a4 = MF::zero();
+        for (int s = 0; s < nsteps; ++s) {
+            const int z = 4 * s + kq;
+            T b = T(0);
+            if (colValid && z < Nz) {
+                const cx<T> q = Q[t * tStride + static_cast<size_t>(z) * Ny * nkxP + colOff];
+                b = part ? q.im : q.re;
+            }
+            const T a = A[(16 * jt + n) * Za + 4 * s + kq];
+            a4 = MF::run(a, b, a4);
+        }
+        acc[t] = a4;
+    }
+    // complex coefficient c_t(j, column) for rows j = 16 jt + 4 kq + i
+    double cre[4][4], cim[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double mine = static_cast<double>(acc[t][i]);
+            const double other = __shfl_xor(mine, 1);
+            cre[t][i] = part ? other : mine;
+            cim[t][i] = part ? mine : other;
+        }
+    }
+    if (!colValid || (cd.flags & kColCompletion)) return;
+    const bool conj = (cd.flags & kColConj) != 0;
+    const bool inertial = cd.wv == 0;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+        const int i = 2 * part + ii;
+        const int j = 16 * jt + 4 * kq + i;
+        if (j >= Nj) continue;
+        const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
+        double sn, cs;
+        sincos(tb.omega[idx] * elapsed, &sn, &cs);
+        const cx<double> p{cs, sn};
+        const double a0v = tb.A0Vort[idx], a0b = tb.A0Buoy[idx], na0 = tb.NA0[idx], apmn = tb.ApmN[idx];
+        cx<double> Fp{0.0, 0.0}, Fm{0.0, 0.0}, F0{0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int target = (targetCode >> (4 * t)) & 0xF;
+            cx<double> value{cre[t][i], conj ? -cim[t][i] : cim[t][i]};
+            value = cscale(value, hscale);
+            cx<double> a0{0.0, 0.0};
+            if (target == 0) a0 = cscale(cmuli(value, -cd.l), a0v);
+            if (target == 1) a0 = cscale(cmuli(value, cd.k), a0v);
+            if (target == 3) a0 = cscale(value, a0b);
+            const cx<double> buoyValue = csub(target == 3 ? value : cx<double>{0.0, 0.0}, cscale(a0, na0));
+            const cx<double> buoy = cscale(buoyValue, apmn);
+            cx<double> wave{0.0, 0.0};
+            if (hydro) {
+                if (target == 0) wave = cmul(cmuli(value, cd.k), tb.ApmD[idx]);
+                if (target == 1) wave = cmul(cmuli(value, cd.l), tb.ApmD[idx]);
+            } else {
+                if (target == 0) wave = cscale(value, cd.cosA * tb.ApmDScaled[idx]);
+                if (target == 1) wave = cscale(value, cd.sinA * tb.ApmDScaled[idx]);
+                if (target == 2) wave = cmuli(value, (cd.Kh / 2.0) * tb.apmW[j]);
+            }
+            cx<double> cAp = cadd(wave, buoy), cAm = csub(wave, buoy);
+            if (inertial) {
+                cx<double> a{0.0, 0.0};
+                if (target == 0) a = cscale(value, tb.inertialScale[j]);
+                if (target == 1) a = cscale(cmuli(value, -1.0), tb.inertialScale[j]);
+                cAp = a;
+                cAm = cconj(a);
+            }
+            Fp = cadd(Fp, cmul(cAp, cconj(p)));
+            Fm = cadd(Fm, cmul(cAm, p));
+            F0 = cadd(F0, a0);
+        }
+        const cx<double> k3[3] = {Fp, Fm, F0};
+        cx<double>* outs[3] = {ep.out0, ep.out1, ep.out2};
+        cx<double>* ys[3] = {ep.y0, ep.y1, ep.y2};
+        cx<double>* sts[3] = {ep.st0, ep.st1, ep.st2};
+        cx<double>* ws[3] = {ep.w0, ep.w1, ep.w2};
+        // RK4 stage arithmetic of WVFixedStepRK4::step with scaledSum
+        // (value + scale*increment, WVRungeKutta.cpp:15-19), rounded per
+        // operation as on the CPU.
+        auto axpy = [](cx<double> y, cx<double> x, double s) {
+            return cx<double>{__dadd_rn(y.re, __dmul_rn(s, x.re)), __dadd_rn(y.im, __dmul_rn(s, x.im))};
+        };
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const cx<double> kv = k3[a];
+            switch (ep.mode) {
+                case kFluxOut: outs[a][idx] = kv; break;
+                case kRk1:
+                    ws[a][idx] = kv;
+                    sts[a][idx] = axpy(ys[a][idx], kv, ep.scale);
+                    break;
+                case kRk2:
+                    ws[a][idx] = axpy(ws[a][idx], kv, 2.0);
+                    sts[a][idx] = axpy(ys[a][idx], kv, ep.scale);
+                    break;
+                case kRk3:
+                    ws[a][idx] = axpy(ws[a][idx], kv, 2.0);
+                    sts[a][idx] = axpy(ys[a][idx], kv, ep.scale);
+                    break;
+                default: {
+                    const cx<double> w = axpy(ws[a][idx], kv, 1.0);
+                    ys[a][idx] = axpy(ys[a][idx], w, ep.scale);
+                    break;
+                }
+            }
+        }
+    }
+}
+
+// r2c along x of nf real FP64 fields [Nx,Ny,Nz,nf] into P[c][z][y][kx < nkx]
+// (the horizontal forward plans of the forward transforms).
+template <typename T>
+__global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<T>* __restrict__ P, int nf, int Nx,
+                                              int log2Nx, int Ny, int Nz, int nkx, int nkxP,
+                                              const cx<T>* __restrict__ twX) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int pairs = (nf + 1) / 2;
+    const int y = blockIdx.x, z = blockIdx.y;
+    const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
+    const size_t row = (static_cast<size_t>(z) * Ny + y) * Nx;
+    for (int e = threadIdx.x; e < pairs * Nx; e += blockDim.x) {
+        const int p = e / Nx, x = e % Nx;
+        const T a = T(in[(2 * p) * R + row + x]);
+        const T b = (2 * p + 1 < nf) ? T(in[(2 * p + 1) * R + row + x]) : T(0);
+        buf[e] = {a, b};
+    }
+    __syncthreads();
+    fftLds(buf, Nx, log2Nx, 1, Nx, pairs, false, twX, -1);
+    const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
+    const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
+    for (int e = threadIdx.x; e < pairs * nkx; e += blockDim.x) {
+        const int q = e / nkx, k = e % nkx;
+        const cx<T> Za = buf[q * Nx + k];
+        const cx<T> Zb = buf[q * Nx + (k == 0 ? 0 : Nx - k)];
+        const T h = T(0.5);
+        P[(2 * q) * chStride + rowBase + k] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
+        if (2 * q + 1 < nf) P[(2 * q + 1) * chStride + rowBase + k] = {(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
+    }
+}
+
+// Vertical analysis of the forward transforms (MFMA, as k_proj) followed by
+// projectNonhydrostaticMode / projectHydrostaticMode (...Kernel.cpp:299-370):
+// channels (U, V, W, N) nonhydrostatic or (U, V, N) hydrostatic.
+template <typename T, int NF>
+__global__ __launch_bounds__(256) void k_projfwd(const cx<T>* __restrict__ Q, const ColumnDev* __restrict__ cols,
+                                                 int ncol, const T* __restrict__ anaC, const T* __restrict__ anaS,
+                                                 int Ja, int Za, int Nz, int Nj, int Ny, int nkxP, ProjTables tb,
+                                                 double elapsed, double hscale, cx<double>* __restrict__ outAp,
+                                                 cx<double>* __restrict__ outAm, cx<double>* __restrict__ outA0) {
+    using MF = Mfma<T>;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int ncg = (ncol + 7) / 8;
+    const int njt = Ja / 16;
+    if (wave >= ncg * njt) return;
+    const int cg = wave % ncg;
+    const int jt = wave / ncg;
+    const int n = lane & 15, kq = lane >> 4;
+    const int col = cg * 8 + (n >> 1);
+    const int part = n & 1;
+    const bool colValid = col < ncol;
+    ColumnDev cd{};
+    if (colValid) cd = cols[col];
+    const size_t tStride = static_cast<size_t>(Nz) * Ny * nkxP;
+    const size_t colOff = static_cast<size_t>(cd.iy) * nkxP + cd.kx;
+    typename MF::acc acc[NF];
+    const int nsteps = Za / 4;
+#pragma unroll
+    for (int c = 0; c < NF; ++c) {
+        const T* A = c < 2 ? anaC : anaS;  // U, V cosine family; W, N sine family
+        typename MF::acc a4 = MF::zero();
+        for (int s = 0; s < nsteps; ++s) {
+            const int z = 4 * s + kq;
+            T b = T(0);
+            if (colValid && z < Nz) {
+                const cx<T> q = Q[c * tStride + static_cast<size_t>(z) * Ny * nkxP + colOff];
+                b = part ? q.im : q.re;
+            }
+            a4 = MF::run(A[(16 * jt + n) * Za + 4 * s + kq], b, a4);
+        }
+        acc[c] = a4;
+    }
+    double cre[NF][4], cim[NF][4];
+#pragma unroll
+    for (int c = 0; c < NF; ++c)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const double mine = static_cast<double>(acc[c][i]);
+            const double other = __shfl_xor(mine, 1);
+            cre[c][i] = part ? other : mine;
+            cim[c][i] = part ? mine : other;
+        }
+    if (!colValid || (cd.flags & kColCompletion)) return;
+    const bool conj = (cd.flags & kColConj) != 0;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+        const int i = 2 * part + ii;
+        const int j = 16 * jt + 4 * kq + i;
+        if (j >= Nj) continue;
+        const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
+        cx<double> v[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) v[c] = cscale(cx<double>{cre[c][i], conj ? -cim[c][i] : cim[c][i]}, hscale);
+        const cx<double> U = v[0], V = v[1], N = v[NF - 1];
+        const cx<double> vort = csub(cmuli(V, cd.k), cmuli(U, cd.l));
+        const cx<double> A0 = cadd(cscale(vort, tb.A0Vort[idx]), cscale(N, tb.A0Buoy[idx]));
+        cx<double> cAp, cAm;
+        if (cd.wv == 0) {
+            const cx<double> a = cscale(csub(U, cmuli(V, 1.0)), tb.inertialScale[j]);
+            cAp = a;
+            cAm = cconj(a);
+        } else {
+            cx<double> wave;
+            if (NF == 3) {
+                wave = cmul(tb.ApmD[idx], cadd(cmuli(U, cd.k), cmuli(V, cd.l)));
+            } else {
+                const cx<double> delta = cscale(cadd(cscale(U, cd.cosA), cscale(V, cd.sinA)), tb.ApmDScaled[idx]);
+                wave = cadd(delta, cmuli(v[2], (cd.Kh / 2.0) * tb.apmW[j]));
+            }
+            const cx<double> buoy = cscale(csub(N, cscale(A0, tb.NA0[idx])), tb.ApmN[idx]);
+            cAp = cadd(wave, buoy);
+            cAm = csub(wave, buoy);
+        }
+        double sn, cs;
+        sincos(tb.omega[idx] * elapsed, &sn, &cs);
+        const cx<double> p{cs, sn};
+        outAp[idx] = cmul(cAp, cconj(p));
+        outAm[idx] = cmul(cAm, p);
+        outA0[idx] = A0;
+    }
+}
+
+// Deterministic two-pass FP64 energy reduction.
+__global__ __launch_bounds__(256) void k_energy_partial(const cx<double>* __restrict__ Ap,
+                                                        const cx<double>* __restrict__ Am,
+                                                        const cx<double>* __restrict__ A0,
+                                                        const double* __restrict__ ApmTE,
+                                                        const double* __restrict__ A0TE, int M,
+                                                        double* __restrict__ partial) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const cx<double> p = Ap[i], m = Am[i], z = A0[i];
+        acc += ApmTE[i] * ((p.re * p.re + p.im * p.im) + (m.re * m.re + m.im * m.im)) + A0TE[i] * (z.re * z.re + z.im * z.im);
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__ partial, int n,
+                                                      double* __restrict__ out) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// ------------------------------------------------------------------- host --
+namespace {
+
+thread_local std::string gLastError;
+
+struct Failure {
+    wv_status code;
+    std::string message;
+};
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Failure{WV_FFT_EXECUTION_FAILURE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+int log2Exact(size_t n) {
+    if (n == 0 || (n & (n - 1)) != 0) return -1;
+    int b = 0;
+    while ((size_t(1) << b) < n) ++b;
+    return b;
+}
+
+template <typename T>
+size_t csize() { return sizeof(cx<T>); }
+
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t n) {
+        bytes = n;
+        if (n == 0) n = 16;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            p = nullptr;
+            throw Failure{WV_ALLOCATION_FAILURE, "Unable to allocate device workspace."};
+        }
+    }
+    template <typename U>
+    U* as() const { return static_cast<U*>(p); }
+};
+
+struct StageRecord {
+    std::string name;
+    double totalMs = 0.0;
+    uint64_t launches = 0;
+    double bytesPerLaunch = 0.0;
+};
+
+}  // namespace
+
+struct KernelImpl {
+    Descriptor d;
+    wv_precision precision = WV_PRECISION_FP64;
+    int device = 0;
+    int NT = 4, NCH = 15, targetCode = 0, KT = 16;
+    size_t ncol = 0;
+    size_t tsize = 16;  // bytes of one field complex value
+    std::atomic<bool> executing{false};
+
+    DeviceBuffer cols, omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled, ApmD, ApmTE, A0TE,
+        apmW, inertialScale, fWaveScale, gWaveScale, Fg, Gg;
+    DeviceBuffer synC, synS, anaC, anaS, twX, twY, mj;
+    DeviceBuffer FV, H, G, P, Q, stage, weighted, partial, scalar;
+    DeviceBuffer fieldsScratch;
+
+    bool instrument = false;
+    std::vector<StageRecord> records;
+    struct Pending {
+        int record;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+
+    size_t persistentBytes() const {
+        size_t s = 0;
+        for (const DeviceBuffer* b : {&cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
+                                      &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
+                                      &anaS, &twX, &twY, &mj, &FV, &H, &G, &P, &Q, &stage, &weighted, &partial, &scalar})
+            s += b->bytes;
+        return s;
+    }
+
+    int recordIndex(const char* name, double bytes) {
+        for (size_t i = 0; i < records.size(); ++i)
+            if (records[i].name == name) return static_cast<int>(i);
+        records.push_back({name, 0.0, 0, bytes});
+        return static_cast<int>(records.size() - 1);
+    }
+
+    template <typename Launch>
+    void timed(const char* name, double bytes, hipStream_t s, Launch&& launch) {
+        if (!instrument) {
+            launch();
+            hipCheck(hipGetLastError(), name);
+            return;
+        }
+        Pending pd{recordIndex(name, bytes), nullptr, nullptr};
+        hipCheck(hipEventCreate(&pd.a), "hipEventCreate");
+        hipCheck(hipEventCreate(&pd.b), "hipEventCreate");
+        hipCheck(hipEventRecord(pd.a, s), "hipEventRecord");
+        launch();
+        hipCheck(hipGetLastError(), name);
+        hipCheck(hipEventRecord(pd.b, s), "hipEventRecord");
+        pending.push_back(pd);
+    }
+
+    void drainTimings() {
+        for (auto& pd : pending) {
+            hipCheck(hipEventSynchronize(pd.b), "hipEventSynchronize");
+            float ms = 0.f;
+            hipCheck(hipEventElapsedTime(&ms, pd.a, pd.b), "hipEventElapsedTime");
+            records[pd.record].totalMs += ms;
+            records[pd.record].launches += 1;
+            (void)hipEventDestroy(pd.a);
+            (void)hipEventDestroy(pd.b);
+        }
+        pending.clear();
+    }
+
+    // ---- setup ----------------------------------------------------------
+    template <typename U>
+    void upload(DeviceBuffer& b, const std::vector<U>& v) {
+        b.alloc(v.size() * sizeof(U));
+        if (!v.empty()) hipCheck(hipMemcpy(b.p, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice), "upload");
+    }
+    template <typename T>
+    void uploadAs(DeviceBuffer& b, const std::vector<double>& v) {
+        std::vector<T> tmp(v.begin(), v.end());
+        upload(b, tmp);
+    }
+
+    template <typename T>
+    void setupTyped() {
+        uploadAs<T>(synC, d.synC);
+        uploadAs<T>(synS, d.synS);
+        uploadAs<T>(anaC, d.anaC);
+        uploadAs<T>(anaS, d.anaS);
+        uploadAs<T>(twX, d.twX);
+        uploadAs<T>(twY, d.twY);
+        uploadAs<T>(mj, d.m);
+        tsize = csize<T>();
+    }
+
+    void setup(const wv_config& cfg, wv_precision prec, int dev) {
+        precision = prec;
+        device = dev;
+        hipCheck(hipSetDevice(dev), "hipSetDevice");
+        KT = static_cast<int>(std::max<size_t>(1, std::min<size_t>(16, 4096 / std::max<size_t>(cfg.Ny, 1))));
+        while (KT & (KT - 1)) --KT;  // power of two
+        std::string err;
+        try {
+            err = d.build(cfg, static_cast<size_t>(KT));
+        } catch (const std::overflow_error& e) {
+            throw Failure{WV_SIZE_OVERFLOW, e.what()};
+        } catch (const std::bad_alloc&) {
+            throw Failure{WV_ALLOCATION_FAILURE, "Unable to allocate immutable kernel descriptor storage."};
+        }
+        if (!err.empty()) throw Failure{WV_INVALID_CONFIGURATION, err};
+        const bool hydro = cfg.isHydrostatic != 0;
+        NT = hydro ? 3 : 4;
+        NCH = 3 + 3 * NT;
+        targetCode = hydro ? (0 | (1 << 4) | (3 << 8)) : (0 | (1 << 4) | (2 << 8) | (3 << 12));
+        const size_t pairs = (static_cast<size_t>(NCH) + 1) / 2;
+        if (pairs * cfg.Nx > static_cast<size_t>(kFftRegs) * 256 || static_cast<size_t>(KT) * cfg.Ny > static_cast<size_t>(kFftRegs) * 256)
+            throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx <= 512, Ny <= 4096)."};
+        // columns
+        ncol = d.columns.size();
+        std::vector<ColumnDev> hc(ncol);
+        for (size_t i = 0; i < ncol; ++i) {
+            const Column& c = d.columns[i];
+            const FourierMode& fm = d.modes[static_cast<size_t>(c.wv)];
+            hc[i] = {c.iy, c.kx, c.wv, c.flags, fm.k, fm.l, fm.Kh, fm.cosAlpha, fm.sinAlpha};
+        }
+        upload(cols, hc);
+        upload(omega, d.omega);
+        upload(UAp, d.UAp);
+        upload(VAp, d.VAp);
+        upload(WAp, d.WAp);
+        upload(NAp, d.NAp);
+        upload(UA0, d.UA0);
+        upload(VA0, d.VA0);
+        upload(NA0, d.NA0);
+        upload(A0Vort, d.A0Vort);
+        upload(A0Buoy, d.A0Buoy);
+        upload(ApmN, d.ApmN);
+        upload(ApmDScaled, d.ApmDScaled);
+        upload(ApmD, d.ApmD);
+        upload(ApmTE, d.ApmTE);
+        upload(A0TE, d.A0TE);
+        upload(apmW, d.apmW);
+        upload(inertialScale, d.inertialScale);
+        upload(fWaveScale, d.fWaveScale);
+        upload(gWaveScale, d.gWaveScale);
+        upload(Fg, d.Fg);
+        upload(Gg, d.Gg);
+        if (precision == WV_PRECISION_FP64) setupTyped<double>();
+        else setupTyped<float>();
+        const size_t plane = cfg.Ny * d.nkxP;
+        const size_t chField = cfg.Nz * plane;
+        FV.alloc(4 * d.M * tsize);
+        H.alloc(static_cast<size_t>(NCH) * chField * tsize);
+        G.alloc(static_cast<size_t>(NCH) * chField * tsize);
+        P.alloc(static_cast<size_t>(NT) * chField * tsize);
+        Q.alloc(static_cast<size_t>(NT) * chField * tsize);
+        hipCheck(hipMemset(H.p, 0, H.bytes), "hipMemset");
+        hipCheck(hipMemset(P.p, 0, P.bytes), "hipMemset");
+        stage.alloc(3 * d.M * sizeof(cx<double>));
+        weighted.alloc(3 * d.M * sizeof(cx<double>));
+        partial.alloc(1024 * sizeof(double));
+        scalar.alloc(16 * sizeof(double));
+        hipCheck(hipDeviceSynchronize(), "setup");
+    }
+
+    // ---- pipeline -----------------------------------------------------------
+    SynthPlan fluxPlan() const {
+        SynthPlan sp{};
+        sp.ngroups = 1 + NT;
+        sp.g[0] = {3, {0, 1, 2, 0}, {0, 0, 0, 0}, {0, 0, 1, 0}, {0, 1, 2, 0}};
+        for (int t = 0; t < NT; ++t) {
+            const int target = (targetCode >> (4 * t)) & 0xF;
+            const int famXY = target < 2 ? 0 : 1;
+            const int famZ = 1 - famXY;
+            sp.g[1 + t] = {3, {target, target, target, 0}, {1, 2, 3, 0}, {famXY, famXY, famZ, 0},
+                           {3 + 3 * t, 4 + 3 * t, 5 + 3 * t, 0}};
+        }
+        return sp;
+    }
+
+    template <typename T>
+    void launchFields(const double* Ap, const double* Am, const double* A0, double elapsed, hipStream_t s) {
+        const int M = static_cast<int>(d.M);
+        timed("fields", 48.0 * M + 136.0 * M + 4.0 * M * sizeof(cx<T>), s, [&] {
+            hipLaunchKernelGGL(k_fields<T>, dim3((M + 255) / 256), dim3(256), 0, s,
+                               reinterpret_cast<const cx<double>*>(Ap), reinterpret_cast<const cx<double>*>(Am),
+                               reinterpret_cast<const cx<double>*>(A0), omega.as<double>(), UAp.as<cx<double>>(),
+                               VAp.as<cx<double>>(), WAp.as<cx<double>>(), NAp.as<double>(), UA0.as<cx<double>>(),
+                               VA0.as<cx<double>>(), NA0.as<double>(), elapsed, M, FV.as<cx<T>>());
+        });
+    }
+
+    template <typename T>
+    void launchSynth(const SynthPlan& sp, hipStream_t s) {
+        constexpr int ZT = 3;
+        const int ncg = static_cast<int>((ncol + 7) / 8);
+        const int ztiles = static_cast<int>(d.Zs / 16);
+        const int nzc = (ztiles + ZT - 1) / ZT;
+        const int waves = ncg * sp.ngroups * nzc;
+        int nch = 0;
+        for (int g = 0; g < sp.ngroups; ++g) nch += sp.g[g].nch;
+        const double bytes = static_cast<double>(nch) * d.c.Nz * ncol * sizeof(cx<T>) + 4.0 * d.M * sizeof(cx<T>);
+        timed("synth", bytes, s, [&] {
+            hipLaunchKernelGGL((k_synth<T, ZT>), dim3((waves + 3) / 4), dim3(256), 0, s, FV.as<cx<T>>(),
+                               cols.as<ColumnDev>(), static_cast<int>(ncol), static_cast<int>(d.M), synC.as<T>(),
+                               synS.as<T>(), static_cast<int>(d.Zs), static_cast<int>(d.Js),
+                               static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
+                               static_cast<int>(d.nkxP), mj.as<T>(), sp, H.as<cx<T>>());
+        });
+    }
+
+    template <typename T>
+    void launchYfft(const DeviceBuffer& in, DeviceBuffer& out, int nch, int sign, hipStream_t s, const char* name) {
+        const size_t lds = static_cast<size_t>(KT) * d.c.Ny * sizeof(cx<T>);
+        const double bytes = 2.0 * nch * d.c.Nz * d.c.Ny * d.nkxP * sizeof(cx<T>);
+        timed(name, bytes, s, [&] {
+            hipLaunchKernelGGL(k_yfft<T>, dim3(static_cast<unsigned>(d.nkxP / KT), static_cast<unsigned>(d.c.Nz), nch),
+                               dim3(256), lds, s, in.as<cx<T>>(), out.as<cx<T>>(), static_cast<int>(d.c.Ny),
+                               log2Exact(d.c.Ny), static_cast<int>(d.nkxP), KT, static_cast<int>(d.c.Nz),
+                               twY.as<cx<T>>(), sign);
+        });
+    }
+
+    template <typename T>
+    void launchXfused(hipStream_t s) {
+        const int pairs = (NCH + 1) / 2;
+        const size_t lds = static_cast<size_t>(pairs) * d.c.Nx * sizeof(cx<T>);
+        const double bytes = (static_cast<double>(NCH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>);
+        const dim3 grid(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz));
+        const int Nx = static_cast<int>(d.c.Nx), l2 = log2Exact(d.c.Nx), Ny = static_cast<int>(d.c.Ny),
+                  Nz = static_cast<int>(d.c.Nz), nkx = static_cast<int>(d.nkx), nkxP = static_cast<int>(d.nkxP);
+        timed("xfused", bytes, s, [&] {
+            if (NT == 4)
+                hipLaunchKernelGGL((k_xfused<T, 15, 4>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, l2,
+                                   Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+            else
+                hipLaunchKernelGGL((k_xfused<T, 12, 3>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, l2,
+                                   Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+        });
+    }
+
+    template <typename T>
+    void launchProj(double elapsed, const Epilogue& ep, hipStream_t s) {
+        const int ncg = static_cast<int>((ncol + 7) / 8);
+        const int waves = ncg * static_cast<int>(d.Ja / 16);
+        ProjTables tb{omega.as<double>(), A0Vort.as<double>(), A0Buoy.as<double>(), NA0.as<double>(),
+                      ApmN.as<double>(), ApmDScaled.as<double>(), ApmD.as<cx<double>>(), apmW.as<double>(),
+                      inertialScale.as<double>()};
+        const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
+        const double bytes = static_cast<double>(NT) * d.c.Nz * ncol * sizeof(cx<T>) + d.M * (48.0 + 48.0 + 48.0);
+        timed("proj", bytes, s, [&] {
+            const dim3 grid((waves + 3) / 4), block(256);
+            const int ncolI = static_cast<int>(ncol), Ja = static_cast<int>(d.Ja), Za = static_cast<int>(d.Za),
+                      Nz = static_cast<int>(d.c.Nz), Nj = static_cast<int>(d.c.Nj), Ny = static_cast<int>(d.c.Ny),
+                      nkxP = static_cast<int>(d.nkxP), hydro = d.c.isHydrostatic ? 1 : 0;
+            if (NT == 4)
+                hipLaunchKernelGGL((k_proj<T, 4>), grid, block, 0, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
+                                   anaC.as<T>(), anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, targetCode, hydro, tb,
+                                   elapsed, hscale, ep);
+            else
+                hipLaunchKernelGGL((k_proj<T, 3>), grid, block, 0, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
+                                   anaC.as<T>(), anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, targetCode, hydro, tb,
+                                   elapsed, hscale, ep);
+        });
+    }
+
+    template <typename T>
+    void rhs(const double* Ap, const double* Am, const double* A0, double t, double t0, const Epilogue& ep,
+             hipStream_t s) {
+        const double elapsed = t - t0;
+        launchFields<T>(Ap, Am, A0, elapsed, s);
+        launchSynth<T>(fluxPlan(), s);
+        launchYfft<T>(H, G, NCH, +1, s, "yfft_inv");
+        launchXfused<T>(s);
+        launchYfft<T>(P, Q, NT, -1, s, "yfft_fwd");
+        launchProj<T>(elapsed, ep, s);
+    }
+
+    void rhsAny(const double* Ap, const double* Am, const double* A0, double t, double t0, const Epilogue& ep,
+                hipStream_t s) {
+        if (precision == WV_PRECISION_FP64) rhs<double>(Ap, Am, A0, t, t0, ep, s);
+        else rhs<float>(Ap, Am, A0, t, t0, ep, s);
+    }
+
+    template <typename T>
+    void inverse(const double* Ap, const double* Am, const double* A0, double t, double t0, int nf, double* out,
+                 hipStream_t s) {
+        launchFields<T>(Ap, Am, A0, t - t0, s);
+        SynthPlan sp{};
+        sp.ngroups = 1;
+        if (nf == 4) sp.g[0] = {4, {0, 1, 2, 3}, {0, 0, 0, 0}, {0, 0, 1, 1}, {0, 1, 2, 3}};
+        else sp.g[0] = {3, {0, 1, 2, 0}, {0, 0, 0, 0}, {0, 0, 1, 0}, {0, 1, 2, 0}};
+        launchSynth<T>(sp, s);
+        launchYfft<T>(H, G, nf, +1, s, "yfft_inv");
+        const int pairs = (nf + 1) / 2;
+        const size_t lds = static_cast<size_t>(pairs) * d.c.Nx * sizeof(cx<T>);
+        timed("xinv", static_cast<double>(nf) * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
+            hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
+                               lds, s, G.as<cx<T>>(), out, nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
+                               static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
+                               static_cast<int>(d.nkxP), twX.as<cx<T>>());
+        });
+    }
+
+    // Forward transform (...Kernel.cpp:583-684): r2c along x, FFT along y,
+    // vertical analysis and the wave-vortex projection formulas.
+    template <typename T>
+    void forward(const double* fields, double t, double t0, int nf, double* Ap, double* Am, double* A0,
+                 hipStream_t s) {
+        const int pairs = (nf + 1) / 2;
+        const size_t lds = static_cast<size_t>(pairs) * d.c.Nx * sizeof(cx<T>);
+        timed("xfwd", static_cast<double>(nf) * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
+            hipLaunchKernelGGL(k_xfwd<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
+                               lds, s, fields, P.as<cx<T>>(), nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
+                               static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
+                               static_cast<int>(d.nkxP), twX.as<cx<T>>());
+        });
+        launchYfft<T>(P, Q, nf, -1, s, "yfft_fwd");
+        const int ncg = static_cast<int>((ncol + 7) / 8);
+        const int waves = ncg * static_cast<int>(d.Ja / 16);
+        ProjTables tb{omega.as<double>(), A0Vort.as<double>(), A0Buoy.as<double>(), NA0.as<double>(),
+                      ApmN.as<double>(), ApmDScaled.as<double>(), ApmD.as<cx<double>>(), apmW.as<double>(),
+                      inertialScale.as<double>()};
+        const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
+        const dim3 grid((waves + 3) / 4), block(256);
+        const int ncolI = static_cast<int>(ncol), Ja = static_cast<int>(d.Ja), Za = static_cast<int>(d.Za),
+                  Nz = static_cast<int>(d.c.Nz), Nj = static_cast<int>(d.c.Nj), Ny = static_cast<int>(d.c.Ny),
+                  nkxP = static_cast<int>(d.nkxP);
+        auto* oAp = reinterpret_cast<cx<double>*>(Ap);
+        auto* oAm = reinterpret_cast<cx<double>*>(Am);
+        auto* oA0 = reinterpret_cast<cx<double>*>(A0);
+        timed("projfwd", static_cast<double>(nf) * d.c.Nz * ncol * sizeof(cx<T>) + 48.0 * d.M, s, [&] {
+            if (nf == 4)
+                hipLaunchKernelGGL((k_projfwd<T, 4>), grid, block, 0, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
+                                   anaC.as<T>(), anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, tb, t - t0, hscale, oAp, oAm,
+                                   oA0);
+            else
+                hipLaunchKernelGGL((k_projfwd<T, 3>), grid, block, 0, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
+                                   anaC.as<T>(), anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, tb, t - t0, hscale, oAp, oAm,
+                                   oA0);
+        });
+    }
+
+    // transformToSpatialDomainWith{F,G}AllDerivatives (...Kernel.cpp:819-863):
+    // value, d/dx, d/dy, d/dz of an F- (cosine) or G- (sine) family field.
+    template <typename T>
+    void fgAll(const double* Apm, const double* A0, bool fFamily, double* out, hipStream_t s) {
+        const int M = static_cast<int>(d.M);
+        timed("fgfields", 48.0 * M + M * sizeof(cx<T>), s, [&] {
+            hipLaunchKernelGGL(k_fgfields<T>, dim3((M + 255) / 256), dim3(256), 0, s,
+                               reinterpret_cast<const cx<double>*>(Apm), reinterpret_cast<const cx<double>*>(A0),
+                               fWaveScale.as<double>(), gWaveScale.as<double>(), Fg.as<double>(), Gg.as<double>(),
+                               static_cast<int>(d.c.Nj), M, fFamily ? 1 : 0, FV.as<cx<T>>());
+        });
+        SynthPlan sp{};
+        sp.ngroups = 1;
+        const int a = fFamily ? 0 : 1, b = 1 - a;
+        sp.g[0] = {4, {0, 0, 0, 0}, {0, 1, 2, 3}, {a, a, a, b}, {0, 1, 2, 3}};
+        launchSynth<T>(sp, s);
+        launchYfft<T>(H, G, 4, +1, s, "yfft_inv");
+        const size_t lds = static_cast<size_t>(2) * d.c.Nx * sizeof(cx<T>);
+        timed("xinv", 4.0 * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
+            hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
+                               lds, s, G.as<cx<T>>(), out, 4, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
+                               static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
+                               static_cast<int>(d.nkxP), twX.as<cx<T>>());
+        });
+    }
+
+    double energy(const double* Ap, const double* Am, const double* A0, hipStream_t s) {
+        const int M = static_cast<int>(d.M);
+        const int blocks = std::max(1, std::min(1024, (M + 255) / 256));
+        hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const cx<double>*>(Ap),
+                           reinterpret_cast<const cx<double>*>(Am), reinterpret_cast<const cx<double>*>(A0),
+                           ApmTE.as<double>(), A0TE.as<double>(), M, partial.as<double>());
+        hipCheck(hipGetLastError(), "energy");
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, partial.as<double>(), blocks, scalar.as<double>());
+        hipCheck(hipGetLastError(), "energy");
+        double e = 0.0;
+        hipCheck(hipMemcpyAsync(&e, scalar.p, sizeof(double), hipMemcpyDeviceToHost, s), "energy copy");
+        hipCheck(hipStreamSynchronize(s), "energy sync");
+        return e;
+    }
+};
+
+}  // namespace wvamd
+
+// ==================================================================== C-ABI ==
+using wvamd::Failure;
+using wvamd::KernelImpl;
+using wvamd::hipCheck;
+
+struct wv_kernel {
+    KernelImpl impl;
+};
+
+namespace {
+
+wv_status fail(wv_status code, const std::string& msg) {
+    wvamd::gLastError = msg;
+    return code;
+}
+
+template <typename F>
+wv_status guarded(F&& f) {
+    try {
+        return f();
+    } catch (const Failure& e) {
+        return fail(e.code, e.message);
+    } catch (const std::bad_alloc&) {
+        return fail(WV_ALLOCATION_FAILURE, "Host allocation failed.");
+    } catch (const std::overflow_error& e) {
+        return fail(WV_SIZE_OVERFLOW, e.what());
+    } catch (const std::exception& e) {
+        return fail(WV_NUMERICAL_FAILURE, e.what());
+    }
+}
+
+struct ExecGuard {
+    std::atomic<bool>& flag;
+    bool entered;
+    explicit ExecGuard(std::atomic<bool>& f) : flag(f) {
+        bool expected = false;
+        entered = flag.compare_exchange_strong(expected, true);
+    }
+    ~ExecGuard() {
+        if (entered) flag.store(false);
+    }
+};
+
+bool overlaps(const void* a, size_t an, const void* b, size_t bn) {
+    if (!a || !b || an == 0 || bn == 0) return false;
+    const auto x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
+    return x < y + bn && y < x + an;
+}
+
+// validateStateAndFlux (WVKernelTypes.cpp:343-362) for device pointers.
+wv_status validateFlux(const KernelImpl& k, double t, double t0, const void* Ap, const void* Am, const void* A0,
+                       const void* Fp, const void* Fm, const void* F0) {
+    if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "State times must be finite.");
+    const char* names[6] = {"Ap", "Am", "A0", "Fp", "Fm", "F0"};
+    const void* ptrs[6] = {Ap, Am, A0, Fp, Fm, F0};
+    for (int i = 0; i < 6; ++i)
+        if (!ptrs[i] && k.d.M) return fail(WV_INVALID_POINTER, std::string(names[i]) + " has a null data pointer.");
+    const size_t bytes = k.d.M * 16;
+    for (int i = 3; i < 6; ++i) {
+        for (int j = i + 1; j < 6; ++j)
+            if (overlaps(ptrs[i], bytes, ptrs[j], bytes)) return fail(WV_OVERLAPPING_ARRAYS, "Flux outputs must not overlap.");
+        for (int j = 0; j < 3; ++j)
+            if (overlaps(ptrs[i], bytes, ptrs[j], bytes))
+                return fail(WV_OVERLAPPING_ARRAYS, "Flux outputs must not overlap state inputs.");
+    }
+    return WV_SUCCESS;
+}
+
+hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+const char* wv_last_error(void) { return wvamd::gLastError.c_str(); }
+uint32_t wv_contract_version(void) { return WV_CONTRACT_VERSION; }
+
+wv_status wv_create(const wv_config* config, wv_precision precision, int device, wv_kernel** kernel) {
+    if (!config || !kernel) return fail(WV_INVALID_POINTER, "wv_create requires a configuration and an output handle.");
+    *kernel = nullptr;
+    if (precision != WV_PRECISION_FP64 && precision != WV_PRECISION_MIXED)
+        return fail(WV_INVALID_CONFIGURATION, "Unknown precision.");
+    return guarded([&] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+            return fail(WV_UNSUPPORTED_OPERATION, "No HIP device is available.");
+        if (device < 0 || device >= count) return fail(WV_INVALID_CONFIGURATION, "Invalid HIP device index.");
+        std::unique_ptr<wv_kernel> k(new wv_kernel());
+        k->impl.setup(*config, precision, device);
+        *kernel = k.release();
+        return WV_SUCCESS;
+    });
+}
+
+void wv_destroy(wv_kernel* kernel) {
+    if (!kernel) return;
+    (void)hipSetDevice(kernel->impl.device);
+    (void)hipDeviceSynchronize();
+    for (auto& pd : kernel->impl.pending) {
+        (void)hipEventDestroy(pd.a);
+        (void)hipEventDestroy(pd.b);
+    }
+    delete kernel;
+}
+
+size_t wv_nkl(const wv_kernel* kernel) { return kernel ? kernel->impl.d.Nkl : 0; }
+double wv_coriolis_frequency(const wv_kernel* kernel) { return kernel ? kernel->impl.d.f : 0.0; }
+size_t wv_persistent_bytes(const wv_kernel* kernel) { return kernel ? kernel->impl.persistentBytes() : 0; }
+
+wv_status wv_descriptor_modes(const wv_kernel* kernel, int64_t* kMode, int64_t* lMode, double* k, double* l,
+                              double* Kh, double* cosAlpha, double* sinAlpha, int64_t* storageRow,
+                              uint8_t* conjugated) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    const auto& md = kernel->impl.d.modes;
+    for (size_t i = 0; i < md.size(); ++i) {
+        if (kMode) kMode[i] = md[i].kMode;
+        if (lMode) lMode[i] = md[i].lMode;
+        if (k) k[i] = md[i].k;
+        if (l) l[i] = md[i].l;
+        if (Kh) Kh[i] = md[i].Kh;
+        if (cosAlpha) cosAlpha[i] = md[i].cosAlpha;
+        if (sinAlpha) sinAlpha[i] = md[i].sinAlpha;
+        if (storageRow) storageRow[i] = static_cast<int64_t>(md[i].storageRow);
+        if (conjugated) conjugated[i] = md[i].conjugated ? 1 : 0;
+    }
+    return WV_SUCCESS;
+}
+
+long wv_descriptor_table(const wv_kernel* kernel, const char* name, double* out) {
+    if (!kernel || !name) return -1;
+    return kernel->impl.d.table(name, out);
+}
+
+wv_status wv_nonlinear_flux_device(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                                   const double* A0, double* Fp, double* Fm, double* F0, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (auto st = validateFlux(k, t, t0, Ap, Am, A0, Fp, Fm, F0); st != WV_SUCCESS) return st;
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        wvamd::Epilogue ep{};
+        ep.mode = wvamd::kFluxOut;
+        ep.out0 = reinterpret_cast<wvamd::cx<double>*>(Fp);
+        ep.out1 = reinterpret_cast<wvamd::cx<double>*>(Fm);
+        ep.out2 = reinterpret_cast<wvamd::cx<double>*>(F0);
+        k.rhsAny(Ap, Am, A0, t, t0, ep, S(stream));
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_nonlinear_flux(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                            const double* A0, double* Fp, double* Fm, double* F0) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (auto st = validateFlux(k, t, t0, Ap, Am, A0, Fp, Fm, F0); st != WV_SUCCESS) return st;
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        const size_t bytes = k.d.M * 16;
+        wvamd::DeviceBuffer buf;
+        buf.alloc(6 * bytes);
+        char* b = buf.as<char>();
+        hipCheck(hipMemcpy(b, Ap, bytes, hipMemcpyHostToDevice), "copy");
+        hipCheck(hipMemcpy(b + bytes, Am, bytes, hipMemcpyHostToDevice), "copy");
+        hipCheck(hipMemcpy(b + 2 * bytes, A0, bytes, hipMemcpyHostToDevice), "copy");
+        const double* d = reinterpret_cast<const double*>(b);
+        double* o = reinterpret_cast<double*>(b + 3 * bytes);
+        const size_t n = k.d.M * 2;
+        const wv_status st = wv_nonlinear_flux_device(kernel, t, t0, d, d + n, d + 2 * n, o, o + n, o + 2 * n, nullptr);
+        if (st != WV_SUCCESS) return st;
+        hipCheck(hipMemcpy(Fp, o, bytes, hipMemcpyDeviceToHost), "copy");
+        hipCheck(hipMemcpy(Fm, o + n, bytes, hipMemcpyDeviceToHost), "copy");
+        hipCheck(hipMemcpy(F0, o + 2 * n, bytes, hipMemcpyDeviceToHost), "copy");
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_nonlinear_flux_with_advection_fields_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                         const double* Am, const double* A0, double* Fp,
+                                                         double* Fm, double* F0, double* uvw, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    if (!uvw) return fail(WV_INVALID_POINTER, "Advection fields has a null data pointer.");
+    wv_status st = wv_transform_wave_vortex_to_uvw_device(kernel, t, t0, Ap, Am, A0, uvw, stream);
+    if (st != WV_SUCCESS) return st;
+    return wv_nonlinear_flux_device(kernel, t, t0, Ap, Am, A0, Fp, Fm, F0, stream);
+}
+
+static wv_status inverseCommon(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                               const double* A0, double* fields, int nf, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (!Ap || !Am || !A0 || !fields) return fail(WV_INVALID_POINTER, "Inverse transform has a null data pointer.");
+    const size_t cb = k.d.M * 16, fb = k.d.R * nf * sizeof(double);
+    for (const void* in : {static_cast<const void*>(Ap), static_cast<const void*>(Am), static_cast<const void*>(A0)})
+        if (overlaps(fields, fb, in, cb)) return fail(WV_OVERLAPPING_ARRAYS, "Inverse spatial output must not overlap coefficient inputs.");
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        if (k.precision == WV_PRECISION_FP64) k.inverse<double>(Ap, Am, A0, t, t0, nf, fields, S(stream));
+        else k.inverse<float>(Ap, Am, A0, t, t0, nf, fields, S(stream));
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_transform_wave_vortex_to_uvweta_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                    const double* Am, const double* A0, double* fields,
+                                                    void* stream) {
+    return inverseCommon(kernel, t, t0, Ap, Am, A0, fields, 4, stream);
+}
+
+wv_status wv_transform_wave_vortex_to_uvw_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                 const double* Am, const double* A0, double* fields, void* stream) {
+    return inverseCommon(kernel, t, t0, Ap, Am, A0, fields, 3, stream);
+}
+
+static wv_status forwardCommon(wv_kernel* kernel, const double* fields, double t, double t0, double* Ap,
+                               double* Am, double* A0, int nf, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    const bool hydro = k.d.c.isHydrostatic != 0;
+    if (nf == 4 && hydro) return fail(WV_INVALID_CONFIGURATION, "transformUVWEtaToWaveVortex requires a nonhydrostatic kernel.");
+    if (nf == 3 && !hydro) return fail(WV_INVALID_CONFIGURATION, "transformUVEtaToWaveVortex requires a hydrostatic kernel.");
+    if (!fields) return fail(WV_INVALID_POINTER, "Fields has a null data pointer.");
+    if (!Ap || !Am || !A0) return fail(WV_INVALID_POINTER, "Coefficient output has a null data pointer.");
+    const size_t cb = k.d.M * 16, fb = k.d.R * nf * sizeof(double);
+    void* outs[3] = {Ap, Am, A0};
+    for (int i = 0; i < 3; ++i) {
+        if (overlaps(outs[i], cb, fields, fb)) return fail(WV_OVERLAPPING_ARRAYS, "Forward coefficient outputs must not overlap the spatial input.");
+        for (int j = i + 1; j < 3; ++j)
+            if (overlaps(outs[i], cb, outs[j], cb)) return fail(WV_OVERLAPPING_ARRAYS, "Forward coefficient outputs must not overlap each other.");
+    }
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        if (k.precision == WV_PRECISION_FP64) k.forward<double>(fields, t, t0, nf, Ap, Am, A0, S(stream));
+        else k.forward<float>(fields, t, t0, nf, Ap, Am, A0, S(stream));
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_transform_uvweta_to_wave_vortex_device(wv_kernel* kernel, const double* fields, double t, double t0,
+                                                    double* Ap, double* Am, double* A0, void* stream) {
+    return forwardCommon(kernel, fields, t, t0, Ap, Am, A0, 4, stream);
+}
+
+wv_status wv_transform_uveta_to_wave_vortex_device(wv_kernel* kernel, const double* fields, double t, double t0,
+                                                   double* Ap, double* Am, double* A0, void* stream) {
+    return forwardCommon(kernel, fields, t, t0, Ap, Am, A0, 3, stream);
+}
+
+static wv_status fgCommon(wv_kernel* kernel, const double* Apm, const double* A0, double* fields, bool fFamily,
+                          void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (!Apm) return fail(WV_INVALID_POINTER, "Apm has a null data pointer.");
+    if (!A0) return fail(WV_INVALID_POINTER, "A0 has a null data pointer.");
+    if (!fields) return fail(WV_INVALID_POINTER, "Field derivatives has a null data pointer.");
+    const size_t cb = k.d.M * 16, fb = k.d.R * 4 * sizeof(double);
+    if (overlaps(fields, fb, Apm, cb) || overlaps(fields, fb, A0, cb))
+        return fail(WV_OVERLAPPING_ARRAYS, "Derivative output must not overlap coefficient inputs.");
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        if (k.precision == WV_PRECISION_FP64) k.fgAll<double>(Apm, A0, fFamily, fields, S(stream));
+        else k.fgAll<float>(Apm, A0, fFamily, fields, S(stream));
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_transform_to_spatial_domain_with_f_all_derivatives_device(wv_kernel* kernel, const double* Apm,
+                                                                       const double* A0, double* fields,
+                                                                       void* stream) {
+    return fgCommon(kernel, Apm, A0, fields, true, stream);
+}
+
+wv_status wv_transform_to_spatial_domain_with_g_all_derivatives_device(wv_kernel* kernel, const double* Apm,
+                                                                       const double* A0, double* fields,
+                                                                       void* stream) {
+    return fgCommon(kernel, Apm, A0, fields, false, stream);
+}
+
+wv_status wv_rk4_advance_device(wv_kernel* kernel, double* Ap, double* Am, double* A0, double* t, double t0,
+                                double h, int nsteps, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    if (!Ap || !Am || !A0 || !t) return fail(WV_INVALID_POINTER, "RK4 state has a null data pointer.");
+    if (!(h > 0.0) || !std::isfinite(h)) return fail(WV_INVALID_CONFIGURATION, "RK4 step size must be finite and positive.");
+    if (nsteps < 0) return fail(WV_INVALID_CONFIGURATION, "RK4 step count must be nonnegative.");
+    KernelImpl& k = kernel->impl;
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "RK4 stepping is not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        using wvamd::cx;
+        const size_t M = k.d.M;
+        cx<double>* y[3] = {reinterpret_cast<cx<double>*>(Ap), reinterpret_cast<cx<double>*>(Am),
+                            reinterpret_cast<cx<double>*>(A0)};
+        cx<double>* st = k.stage.as<cx<double>>();
+        cx<double>* w = k.weighted.as<cx<double>>();
+        wvamd::Epilogue ep{};
+        ep.y0 = y[0];
+        ep.y1 = y[1];
+        ep.y2 = y[2];
+        ep.st0 = st;
+        ep.st1 = st + M;
+        ep.st2 = st + 2 * M;
+        ep.w0 = w;
+        ep.w1 = w + M;
+        ep.w2 = w + 2 * M;
+        const double* sd = reinterpret_cast<const double*>(st);
+        for (int n = 0; n < nsteps; ++n) {
+            const double ti = *t;
+            ep.mode = wvamd::kRk1;
+            ep.scale = 0.5 * h;
+            k.rhsAny(Ap, Am, A0, ti, t0, ep, S(stream));
+            ep.mode = wvamd::kRk2;
+            k.rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + 0.5 * h, t0, ep, S(stream));
+            ep.mode = wvamd::kRk3;
+            ep.scale = h;
+            k.rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + 0.5 * h, t0, ep, S(stream));
+            ep.mode = wvamd::kRk4;
+            ep.scale = h / 6.0;
+            k.rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + h, t0, ep, S(stream));
+            *t = ti + h;
+        }
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_total_energy_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0,
+                                 double* energy, void* stream) {
+    if (!kernel || !energy) return fail(WV_INVALID_POINTER, "null pointer");
+    KernelImpl& k = kernel->impl;
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        *energy = k.energy(Ap, Am, A0, S(stream));
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_set_stage_instrumentation(wv_kernel* kernel, int enabled) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    kernel->impl.instrument = enabled != 0;
+    return WV_SUCCESS;
+}
+
+int wv_stage_timings(wv_kernel* kernel, wv_stage_timing* out, int capacity) {
+    if (!kernel) return -1;
+    try {
+        kernel->impl.drainTimings();
+    } catch (const Failure& e) {
+        fail(e.code, e.message);
+        return -1;
+    }
+    const auto& r = kernel->impl.records;
+    for (int i = 0; i < capacity && i < static_cast<int>(r.size()); ++i) {
+        std::snprintf(out[i].name, sizeof(out[i].name), "%s", r[static_cast<size_t>(i)].name.c_str());
+        out[i].total_ms = r[static_cast<size_t>(i)].totalMs;
+        out[i].launches = r[static_cast<size_t>(i)].launches;
+        out[i].bytes_per_launch = r[static_cast<size_t>(i)].bytesPerLaunch;
+    }
+    return static_cast<int>(r.size());
+}
+
+void wv_reset_stage_timings(wv_kernel* kernel) {
+    if (!kernel) return;
+    try {
+        kernel->impl.drainTimings();
+    } catch (...) {
+    }
+    for (auto& r : kernel->impl.records) {
+        r.totalMs = 0.0;
+        r.launches = 0;
+    }
+}
+
+}  // extern "C"
