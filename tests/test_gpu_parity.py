@@ -4,7 +4,9 @@ the reference kernel and against the CPU oracle, through the C-ABI.
 Tolerances (relative, max|diff| / max|ref| over Fp, Fm, F0 jointly):
   FP64 path     1e-12 per flux and per RK4 step (the reference's own
                 compiled-vs-MATLAB bar, wvCompiledBackendBuild.m:356-398)
-  mixed path    2e-5 per flux (FP32 fields, spectra, FFTs and vertical MFMA)
+  mixed path    1e-4 per flux on white-noise coefficients (FP32 fields, spectra,
+                FFTs and vertical MFMA; white noise maximizes cancellation in the
+                products -- realistic flows are checked through energy instead)
 """
 import os
 
@@ -20,7 +22,7 @@ TABLES = ["omega", "fWaveScale", "UApField", "VApField", "WApField", "NApField",
 GOLDEN_TINY = [p for p in golden_files("golden_c") if "flow" not in os.path.basename(p)]
 IDS = [os.path.basename(p)[:-4] for p in GOLDEN_TINY]
 FP64_TOL = 1e-12
-MIXED_TOL = 2e-5
+MIXED_TOL = 1e-4
 
 
 def _kernel(cfg, precision="fp64"):

@@ -28,7 +28,9 @@ __device__ __forceinline__ cx<T> cmuli(cx<T> a, T s) { return {-a.im * s, a.re *
 
 // ---------------------------------------------------------------- MFMA ----
 // 16x16x4 MFMA, one A and one B element per lane, four C elements per lane.
-// Lane l holds A[l%16][l/16], B[l/16][l%16] and C[4*(l/16)+i][l%16], i=0..3.
+// Lane l holds A[l%16][l/16] and B[l/16][l%16]; C element i of lane l is
+// C[row(l/16, i)][l%16] with row = 4*(l/16) + i for f32 and (l/16) + 4*i for
+// f64 (both measured on gfx950).
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
@@ -41,6 +43,7 @@ struct Mfma<float> {
     __device__ static __forceinline__ acc run(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static __forceinline__ int row(int kq, int i) { return 4 * kq + i; }
 };
 template <>
 struct Mfma<double> {
@@ -49,29 +52,73 @@ struct Mfma<double> {
     __device__ static __forceinline__ acc run(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static __forceinline__ int row(int kq, int i) { return kq + 4 * i; }
 };
 
 // ----------------------------------------------------------------- FFT ----
 // In-place batched complex FFT in LDS, all threads of the block participate.
-// Element n of transform b lives at buf[n*sn + b*sb] (nb transforms of length
-// N).  X_k = sum_n x_n exp(sign*2*pi*i*n*k/N); tw[q] = exp(-2*pi*i*q/N).
-// Power-of-two N uses Stockham autosort passes (one radix-2 pass when log2 N
-// is odd, then radix-4); other N use a direct DFT.  Each pass holds at most 16
-// complex values per thread, so the caller must launch at least
-// nb*N/16 threads.  The caller synchronizes before the call; the function
-// returns after a final barrier.
+// Element n of transform b lives at buf[lds(n)*sn + b*sb] where lds(n) = n for
+// strided layouts (sn > 1) and n + n/16 for unit-stride layouts (sn == 1, one
+// pad slot per 16 entries so that radix-16 passes are free of bank
+// conflicts).  nb transforms of length N.
+// X_k = sum_n x_n exp(sign*2*pi*i*n*k/N); tw[q] = exp(-2*pi*i*q/N).
+// Power-of-two N runs Stockham autosort passes of radix 16 (then 8, 4 or 2
+// for the remaining factor), each butterfly an in-register DFT; other N use a
+// direct DFT.  A pass holds at most 16 complex values per thread, so the
+// caller must launch at least nb*N/16 threads.  The caller synchronizes
+// before the call; the function returns after a final barrier.
 constexpr int kFftRegs = 16;
 
-template <typename T>
-__device__ __forceinline__ void butterfly4(cx<T>* v, int sign) {
-    const cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-    const cx<T> a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
-    // (+/- i) * a3
-    const cx<T> ia3 = sign < 0 ? cx<T>{a3.im, -a3.re} : cx<T>{-a3.im, a3.re};
-    v[0] = cadd(a0, a2);
-    v[2] = csub(a0, a2);
-    v[1] = cadd(a1, ia3);
-    v[3] = csub(a1, ia3);
+__device__ __forceinline__ int ldsIndex(int n, int sn) { return sn == 1 ? n + (n >> 4) : n * sn; }
+// padded length of a unit-stride transform
+__host__ __device__ constexpr int ldsLength(int N) { return N + (N >> 4); }
+
+// exp(-2 pi i q / 16), q = 0..15
+__device__ __forceinline__ double w16cos(int q) {
+    constexpr double c[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                              0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                              -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                              0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+    return c[q & 15];
+}
+__device__ __forceinline__ double w16sin(int q) { return -w16cos((q + 12) & 15); }  // -sin(2 pi q/16)
+
+// In-register DFT of R = 2, 4, 8 or 16 points, natural order in and out
+// (radix-2 decimation in frequency, then a compile-time bit-reversal).
+template <typename T, int R>
+__device__ __forceinline__ void dftReg(cx<T>* v, int sign) {
+#pragma unroll
+    for (int half = R / 2; half >= 1; half /= 2) {
+#pragma unroll
+        for (int s = 0; s < R; s += 2 * half) {
+#pragma unroll
+            for (int q = 0; q < half; ++q) {
+                const cx<T> a = v[s + q], b = v[s + q + half];
+                v[s + q] = cadd(a, b);
+                const cx<T> d = csub(a, b);
+                const int qi = q * (16 / (2 * half));  // exp(sign 2 pi i q/(2 half)) = W16^(qi)
+                if (qi == 0) {
+                    v[s + q + half] = d;
+                } else if (qi == 4) {  // -/+ i
+                    v[s + q + half] = sign < 0 ? cx<T>{d.im, -d.re} : cx<T>{-d.im, d.re};
+                } else {
+                    const T wr = T(w16cos(qi)), wi = T(sign < 0 ? w16sin(qi) : -w16sin(qi));
+                    v[s + q + half] = {d.re * wr - d.im * wi, d.re * wi + d.im * wr};
+                }
+            }
+        }
+    }
+    cx<T> tmp[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        int r = 0;
+#pragma unroll
+        for (int b = 1, m = R / 2; b < R; b <<= 1, m >>= 1)
+            if (k & b) r |= m;
+        tmp[k] = v[r];
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = tmp[k];
 }
 
 template <typename T, int R>
@@ -91,7 +138,7 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, 
             const int k = jj % Ns;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                cx<T> x = buf[(jj + r * NR) * sn + b * sb];
+                cx<T> x = buf[ldsIndex(jj + r * NR, sn) + b * sb];
                 if (r > 0 && k > 0) {
                     cx<T> w = tw[r * k * tstep];
                     if (sign > 0) w.im = -w.im;
@@ -99,13 +146,7 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, 
                 }
                 v[it][r] = x;
             }
-            if constexpr (R == 4) {
-                butterfly4(v[it], sign);
-            } else {
-                const cx<T> a = v[it][0], c = v[it][1];
-                v[it][0] = cadd(a, c);
-                v[it][1] = csub(a, c);
-            }
+            dftReg<T, R>(v[it], sign);
         }
     }
     __syncthreads();
@@ -117,7 +158,7 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, 
             const int jj = bFast ? item / nb : item % NR;
             const int d = (jj / Ns) * Ns * R + (jj % Ns);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[(d + r * Ns) * sn + b * sb] = v[it][r];
+            for (int r = 0; r < R; ++r) buf[ldsIndex(d + r * Ns, sn) + b * sb] = v[it][r];
         }
     }
     __syncthreads();
@@ -138,7 +179,7 @@ __device__ void directDft(cx<T>* buf, int N, int sn, int sb, int nb, bool bFast,
             for (int n = 0; n < N; ++n) {
                 cx<T> w = tw[(n * k) % N];
                 if (sign > 0) w.im = -w.im;
-                acc = cadd(acc, cmul(buf[n * sn + b * sb], w));
+                acc = cadd(acc, cmul(buf[ldsIndex(n, sn) + b * sb], w));
             }
             out[it] = acc;
         }
@@ -150,7 +191,7 @@ __device__ void directDft(cx<T>* buf, int N, int sn, int sb, int nb, bool bFast,
         if (item < items) {
             const int b = bFast ? item % nb : item / N;
             const int k = bFast ? item / nb : item % N;
-            buf[k * sn + b * sb] = out[it];
+            buf[ldsIndex(k, sn) + b * sb] = out[it];
         }
     }
     __syncthreads();
@@ -163,12 +204,15 @@ __device__ void fftLds(cx<T>* buf, int N, int log2N, int sn, int sb, int nb, boo
         directDft(buf, N, sn, sb, nb, bFast, tw, sign);
         return;
     }
+    // leading pass takes the factor 2^(log2N mod 4); the rest are radix 16
     int Ns = 1;
-    if (log2N & 1) {
-        stockhamPass<T, 2>(buf, N, Ns, sn, sb, nb, bFast, tw, sign);
-        Ns = 2;
+    switch (log2N & 3) {
+        case 1: stockhamPass<T, 2>(buf, N, Ns, sn, sb, nb, bFast, tw, sign); Ns = 2; break;
+        case 2: stockhamPass<T, 4>(buf, N, Ns, sn, sb, nb, bFast, tw, sign); Ns = 4; break;
+        case 3: stockhamPass<T, 8>(buf, N, Ns, sn, sb, nb, bFast, tw, sign); Ns = 8; break;
+        default: break;
     }
-    for (; Ns < N; Ns *= 4) stockhamPass<T, 4>(buf, N, Ns, sn, sb, nb, bFast, tw, sign);
+    for (; Ns < N; Ns *= 16) stockhamPass<T, 16>(buf, N, Ns, sn, sb, nb, bFast, tw, sign);
 }
 
 }  // namespace wvamd

@@ -216,7 +216,8 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
         for (int q = 0; q < ZT; ++q) {
             const int zt = zt0 + q;
             if (zt < ztiles) {
-                const int a_off = (16 * zt + n) * Js + 4 * s + kq;
+                // operands pre-arranged as [ztile][step][lane]: one coalesced load per wave
+                const size_t a_off = (static_cast<size_t>(zt) * nsteps + s) * 64 + lane;
                 const T aC = synC[a_off];
                 const T aS = synS[a_off];
 #pragma unroll
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
             if (zt >= ztiles) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int z = 16 * zt + 4 * kq + i;
+                const int z = 16 * zt + MF::row(kq, i);
                 if (z < Nz) {
                     T* dst = reinterpret_cast<T*>(
                         &H[((static_cast<size_t>(G.outch[c]) * Nz + z) * Ny + cd.iy) * nkxP + cd.kx]);
@@ -289,6 +290,7 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
     constexpr int PAIRS = (NCH + 1) / 2;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+    const int NxP = ldsLength(Nx);
     const int y = blockIdx.x, z = blockIdx.y;
     const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
@@ -296,15 +298,16 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
         const int p = e / Nx, k = e % Nx;
         const cx<T> A = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
         const cx<T> B = (2 * p + 1 < NCH) ? hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx) : cx<T>{T(0), T(0)};
-        buf[e] = {A.re - B.im, A.im + B.re};
+        buf[p * NxP + ldsIndex(k, 1)] = {A.re - B.im, A.im + B.re};
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, Nx, PAIRS, false, twX, +1);
+    fftLds(buf, Nx, log2Nx, 1, NxP, PAIRS, false, twX, +1);
     for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+        const int xl = ldsIndex(x, 1);
         T ch[2 * PAIRS];
 #pragma unroll
         for (int p = 0; p < PAIRS; ++p) {
-            const cx<T> v = buf[p * Nx + x];
+            const cx<T> v = buf[p * NxP + xl];
             ch[2 * p] = v.re;
             ch[2 * p + 1] = v.im;
         }
@@ -312,16 +315,16 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
         T F[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int t = 0; t < NT; ++t) F[t] = -(U * ch[3 + 3 * t] + V * ch[4 + 3 * t] + W * ch[5 + 3 * t]);
-        buf[x] = {F[0], F[1]};
-        buf[Nx + x] = {F[2], F[3]};
+        buf[xl] = {F[0], F[1]};
+        buf[NxP + xl] = {F[2], F[3]};
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, Nx, 2, false, twX, -1);
+    fftLds(buf, Nx, log2Nx, 1, NxP, 2, false, twX, -1);
     const size_t tStride = chStride;
     for (int e = threadIdx.x; e < 2 * nkx; e += blockDim.x) {
         const int q = e / nkx, k = e % nkx;
-        const cx<T> Za = buf[q * Nx + k];
-        const cx<T> Zb = buf[q * Nx + (k == 0 ? 0 : Nx - k)];
+        const cx<T> Za = buf[q * NxP + ldsIndex(k, 1)];
+        const cx<T> Zb = buf[q * NxP + ldsIndex(k == 0 ? 0 : Nx - k, 1)];
         const T h = T(0.5);
         const cx<T> Fa{(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
         const cx<T> Fb{(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
@@ -338,6 +341,7 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int pairs = (nch + 1) / 2;
+    const int NxP = ldsLength(Nx);
     const int y = blockIdx.x, z = blockIdx.y;
     const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
@@ -345,14 +349,14 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
         const int p = e / Nx, k = e % Nx;
         const cx<T> A = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
         const cx<T> B = (2 * p + 1 < nch) ? hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx) : cx<T>{T(0), T(0)};
-        buf[e] = {A.re - B.im, A.im + B.re};
+        buf[p * NxP + ldsIndex(k, 1)] = {A.re - B.im, A.im + B.re};
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, Nx, pairs, false, twX, +1);
+    fftLds(buf, Nx, log2Nx, 1, NxP, pairs, false, twX, +1);
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
     for (int e = threadIdx.x; e < nch * Nx; e += blockDim.x) {
         const int c = e / Nx, x = e % Nx;
-        const cx<T> v = buf[(c / 2) * Nx + x];
+        const cx<T> v = buf[(c / 2) * NxP + ldsIndex(x, 1)];
         out[c * R + (static_cast<size_t>(z) * Ny + y) * Nx + x] = static_cast<double>((c & 1) ? v.im : v.re);
     }
 }
@@ -398,12 +402,12 @@ __global__ __launch_bounds__(256) void k_proj(const cx<T>* __restrict__ Q, const
                 const cx<T> q = Q[t * tStride + static_cast<size_t>(z) * Ny * nkxP + colOff];
                 b = part ? q.im : q.re;
             }
-            const T a = A[(16 * jt + n) * Za + 4 * s + kq];
+            const T a = A[(static_cast<size_t>(jt) * nsteps + s) * 64 + lane];
             a4 = MF::run(a, b, a4);
         }
         acc[t] = a4;
     }
-    // complex coefficient c_t(j, column) for rows j = 16 jt + 4 kq + i
+    // complex coefficient c_t(j, column) for rows j = 16 jt + row(kq, i)
     double cre[4][4], cim[4][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -421,7 +425,7 @@ __global__ __launch_bounds__(256) void k_proj(const cx<T>* __restrict__ Q, const
 #pragma unroll
     for (int ii = 0; ii < 2; ++ii) {
         const int i = 2 * part + ii;
-        const int j = 16 * jt + 4 * kq + i;
+        const int j = 16 * jt + MF::row(kq, i);
         if (j >= Nj) continue;
         const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
         double sn, cs;
@@ -508,6 +512,7 @@ __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int pairs = (nf + 1) / 2;
+    const int NxP = ldsLength(Nx);
     const int y = blockIdx.x, z = blockIdx.y;
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
     const size_t row = (static_cast<size_t>(z) * Ny + y) * Nx;
@@ -515,16 +520,16 @@ __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<
         const int p = e / Nx, x = e % Nx;
         const T a = T(in[(2 * p) * R + row + x]);
         const T b = (2 * p + 1 < nf) ? T(in[(2 * p + 1) * R + row + x]) : T(0);
-        buf[e] = {a, b};
+        buf[p * NxP + ldsIndex(x, 1)] = {a, b};
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, Nx, pairs, false, twX, -1);
+    fftLds(buf, Nx, log2Nx, 1, NxP, pairs, false, twX, -1);
     const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     for (int e = threadIdx.x; e < pairs * nkx; e += blockDim.x) {
         const int q = e / nkx, k = e % nkx;
-        const cx<T> Za = buf[q * Nx + k];
-        const cx<T> Zb = buf[q * Nx + (k == 0 ? 0 : Nx - k)];
+        const cx<T> Za = buf[q * NxP + ldsIndex(k, 1)];
+        const cx<T> Zb = buf[q * NxP + ldsIndex(k == 0 ? 0 : Nx - k, 1)];
         const T h = T(0.5);
         P[(2 * q) * chStride + rowBase + k] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
         if (2 * q + 1 < nf) P[(2 * q + 1) * chStride + rowBase + k] = {(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
@@ -569,7 +574,7 @@ __global__ __launch_bounds__(256) void k_projfwd(const cx<T>* __restrict__ Q, co
                 const cx<T> q = Q[c * tStride + static_cast<size_t>(z) * Ny * nkxP + colOff];
                 b = part ? q.im : q.re;
             }
-            a4 = MF::run(A[(16 * jt + n) * Za + 4 * s + kq], b, a4);
+            a4 = MF::run(A[(static_cast<size_t>(jt) * nsteps + s) * 64 + lane], b, a4);
         }
         acc[c] = a4;
     }
@@ -588,7 +593,7 @@ __global__ __launch_bounds__(256) void k_projfwd(const cx<T>* __restrict__ Q, co
 #pragma unroll
     for (int ii = 0; ii < 2; ++ii) {
         const int i = 2 * part + ii;
-        const int j = 16 * jt + 4 * kq + i;
+        const int j = 16 * jt + MF::row(kq, i);
         if (j >= Nj) continue;
         const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
         cx<double> v[NF];
@@ -794,12 +799,23 @@ struct KernelImpl {
         upload(b, tmp);
     }
 
+    // MFMA A operands in [tile][step][lane] order: lane l of step s in tile r
+    // holds M[16 r + l%16][4 s + l/16] of a row-major [rows x cols] matrix.
+    static std::vector<double> mfmaOperand(const std::vector<double>& m, size_t rows, size_t cols) {
+        const size_t tiles = rows / 16, steps = cols / 4;
+        std::vector<double> out(tiles * steps * 64);
+        for (size_t r = 0; r < tiles; ++r)
+            for (size_t st = 0; st < steps; ++st)
+                for (size_t l = 0; l < 64; ++l) out[(r * steps + st) * 64 + l] = m[(16 * r + l % 16) * cols + 4 * st + l / 16];
+        return out;
+    }
+
     template <typename T>
     void setupTyped() {
-        uploadAs<T>(synC, d.synC);
-        uploadAs<T>(synS, d.synS);
-        uploadAs<T>(anaC, d.anaC);
-        uploadAs<T>(anaS, d.anaS);
+        uploadAs<T>(synC, mfmaOperand(d.synC, d.Zs, d.Js));
+        uploadAs<T>(synS, mfmaOperand(d.synS, d.Zs, d.Js));
+        uploadAs<T>(anaC, mfmaOperand(d.anaC, d.Ja, d.Za));
+        uploadAs<T>(anaS, mfmaOperand(d.anaS, d.Ja, d.Za));
         uploadAs<T>(twX, d.twX);
         uploadAs<T>(twY, d.twY);
         uploadAs<T>(mj, d.m);
@@ -828,6 +844,13 @@ struct KernelImpl {
         const size_t pairs = (static_cast<size_t>(NCH) + 1) / 2;
         if (pairs * cfg.Nx > static_cast<size_t>(kFftRegs) * 256 || static_cast<size_t>(KT) * cfg.Ny > static_cast<size_t>(kFftRegs) * 256)
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx <= 512, Ny <= 4096)."};
+        int maxLds = 0;
+        hipCheck(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+        const size_t elem = prec == WV_PRECISION_FP64 ? 16 : 8;
+        const size_t xLds = pairs * static_cast<size_t>(ldsLength(static_cast<int>(cfg.Nx))) * elem;
+        const size_t yLds = static_cast<size_t>(KT) * cfg.Ny * elem;
+        if (xLds > static_cast<size_t>(maxLds) || yLds > static_cast<size_t>(maxLds))
+            throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid needs more LDS per workgroup than the device provides."};
         // columns
         ncol = d.columns.size();
         std::vector<ColumnDev> hc(ncol);
@@ -937,7 +960,7 @@ struct KernelImpl {
     template <typename T>
     void launchXfused(hipStream_t s) {
         const int pairs = (NCH + 1) / 2;
-        const size_t lds = static_cast<size_t>(pairs) * d.c.Nx * sizeof(cx<T>);
+        const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         const double bytes = (static_cast<double>(NCH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>);
         const dim3 grid(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz));
         const int Nx = static_cast<int>(d.c.Nx), l2 = log2Exact(d.c.Nx), Ny = static_cast<int>(d.c.Ny),
@@ -1006,7 +1029,7 @@ struct KernelImpl {
         launchSynth<T>(sp, s);
         launchYfft<T>(H, G, nf, +1, s, "yfft_inv");
         const int pairs = (nf + 1) / 2;
-        const size_t lds = static_cast<size_t>(pairs) * d.c.Nx * sizeof(cx<T>);
+        const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xinv", static_cast<double>(nf) * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
             hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
                                lds, s, G.as<cx<T>>(), out, nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
@@ -1021,7 +1044,7 @@ struct KernelImpl {
     void forward(const double* fields, double t, double t0, int nf, double* Ap, double* Am, double* A0,
                  hipStream_t s) {
         const int pairs = (nf + 1) / 2;
-        const size_t lds = static_cast<size_t>(pairs) * d.c.Nx * sizeof(cx<T>);
+        const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xfwd", static_cast<double>(nf) * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
             hipLaunchKernelGGL(k_xfwd<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
                                lds, s, fields, P.as<cx<T>>(), nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
@@ -1071,7 +1094,7 @@ struct KernelImpl {
         sp.g[0] = {4, {0, 0, 0, 0}, {0, 1, 2, 3}, {a, a, a, b}, {0, 1, 2, 3}};
         launchSynth<T>(sp, s);
         launchYfft<T>(H, G, 4, +1, s, "yfft_inv");
-        const size_t lds = static_cast<size_t>(2) * d.c.Nx * sizeof(cx<T>);
+        const size_t lds = static_cast<size_t>(2) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xinv", 4.0 * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
             hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
                                lds, s, G.as<cx<T>>(), out, 4, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
