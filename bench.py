@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile", choices=["mixed", "fp64", "both"], default=None,
+                    help="run only `steps` RK4 steps of the workload plus a 512 MiB copy kernel "
+                         "(the byte-counter calibration), for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this path runs on a single MI355X "
@@ -102,6 +105,19 @@ def main():
     import wvamd
 
     cfg = workload_config()
+    if args.profile:
+        kk = wvamd.WVTransformConstantStratificationKernel(cfg, "fp64")
+        state, dt = initial_state(kk)
+        for prec in (("mixed", "fp64") if args.profile == "both" else (args.profile,)):
+            k = kk if prec == "fp64" else wvamd.WVTransformConstantStratificationKernel(cfg, prec)
+            sec, _ = timed_steps(k, state, dt, args.steps, args.warmup)
+            print(f"{prec}: {args.steps / sec:.2f} steps/s", flush=True)
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 2.0, out=b)  # calibration kernel: reads and writes exactly 512 MiB
+        torch.cuda.synchronize()
+        print("copy calibration: 536870912 bytes read, 536870912 bytes written", flush=True)
+        return
     k64 = wvamd.WVTransformConstantStratificationKernel(cfg, "fp64")
     k32 = wvamd.WVTransformConstantStratificationKernel(cfg, "mixed")
     state, dt = initial_state(k64)

@@ -121,11 +121,18 @@ __device__ __forceinline__ void dftReg(cx<T>* v, int sign) {
     for (int k = 0; k < R; ++k) v[k] = tmp[k];
 }
 
+// Transform b of a batch lives at offset (b % nb1)*sb + (b / nb1)*sb2.
+struct Batch {
+    int nb, nb1, sb, sb2;
+    __device__ __forceinline__ int offset(int b) const { return (b % nb1) * sb + (b / nb1) * sb2; }
+};
+
 template <typename T, int R>
-__device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, int sb, int nb, bool bFast,
+__device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, Batch bt, bool bFast,
                                              const cx<T>* __restrict__ tw, int sign) {
     constexpr int IPT = kFftRegs / R;
     const int NR = N / R;
+    const int nb = bt.nb;
     const int items = nb * NR;
     const int tstep = N / (Ns * R);
     cx<T> v[IPT][R];
@@ -136,9 +143,10 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, 
             const int b = bFast ? item % nb : item / NR;
             const int jj = bFast ? item / nb : item % NR;
             const int k = jj % Ns;
+            const int boff = bt.offset(b);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                cx<T> x = buf[ldsIndex(jj + r * NR, sn) + b * sb];
+                cx<T> x = buf[ldsIndex(jj + r * NR, sn) + boff];
                 if (r > 0 && k > 0) {
                     cx<T> w = tw[r * k * tstep];
                     if (sign > 0) w.im = -w.im;
@@ -157,17 +165,19 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, 
             const int b = bFast ? item % nb : item / NR;
             const int jj = bFast ? item / nb : item % NR;
             const int d = (jj / Ns) * Ns * R + (jj % Ns);
+            const int boff = bt.offset(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[ldsIndex(d + r * Ns, sn) + b * sb] = v[it][r];
+            for (int r = 0; r < R; ++r) buf[ldsIndex(d + r * Ns, sn) + boff] = v[it][r];
         }
     }
     __syncthreads();
 }
 
 template <typename T>
-__device__ void directDft(cx<T>* buf, int N, int sn, int sb, int nb, bool bFast, const cx<T>* __restrict__ tw,
+__device__ void directDft(cx<T>* buf, int N, int sn, Batch bt, bool bFast, const cx<T>* __restrict__ tw,
                           int sign) {
     cx<T> out[kFftRegs];
+    const int nb = bt.nb;
     const int items = nb * N;
 #pragma unroll
     for (int it = 0; it < kFftRegs; ++it) {
@@ -179,7 +189,7 @@ __device__ void directDft(cx<T>* buf, int N, int sn, int sb, int nb, bool bFast,
             for (int n = 0; n < N; ++n) {
                 cx<T> w = tw[(n * k) % N];
                 if (sign > 0) w.im = -w.im;
-                acc = cadd(acc, cmul(buf[ldsIndex(n, sn) + b * sb], w));
+                acc = cadd(acc, cmul(buf[ldsIndex(n, sn) + bt.offset(b)], w));
             }
             out[it] = acc;
         }
@@ -191,28 +201,41 @@ __device__ void directDft(cx<T>* buf, int N, int sn, int sb, int nb, bool bFast,
         if (item < items) {
             const int b = bFast ? item % nb : item / N;
             const int k = bFast ? item / nb : item % N;
-            buf[ldsIndex(k, sn) + b * sb] = out[it];
+            buf[ldsIndex(k, sn) + bt.offset(b)] = out[it];
         }
     }
     __syncthreads();
 }
 
 template <typename T>
-__device__ void fftLds(cx<T>* buf, int N, int log2N, int sn, int sb, int nb, bool bFast, const cx<T>* __restrict__ tw,
+__device__ void fftLds(cx<T>* buf, int N, int log2N, int sn, Batch bt, bool bFast, const cx<T>* __restrict__ tw,
                        int sign) {
     if (log2N < 0) {
-        directDft(buf, N, sn, sb, nb, bFast, tw, sign);
+        directDft(buf, N, sn, bt, bFast, tw, sign);
         return;
     }
     // leading pass takes the factor 2^(log2N mod 4); the rest are radix 16
     int Ns = 1;
     switch (log2N & 3) {
-        case 1: stockhamPass<T, 2>(buf, N, Ns, sn, sb, nb, bFast, tw, sign); Ns = 2; break;
-        case 2: stockhamPass<T, 4>(buf, N, Ns, sn, sb, nb, bFast, tw, sign); Ns = 4; break;
-        case 3: stockhamPass<T, 8>(buf, N, Ns, sn, sb, nb, bFast, tw, sign); Ns = 8; break;
+        case 1: stockhamPass<T, 2>(buf, N, Ns, sn, bt, bFast, tw, sign); Ns = 2; break;
+        case 2: stockhamPass<T, 4>(buf, N, Ns, sn, bt, bFast, tw, sign); Ns = 4; break;
+        case 3: stockhamPass<T, 8>(buf, N, Ns, sn, bt, bFast, tw, sign); Ns = 8; break;
         default: break;
     }
-    for (; Ns < N; Ns *= 16) stockhamPass<T, 16>(buf, N, Ns, sn, sb, nb, bFast, tw, sign);
+    for (; Ns < N; Ns *= 16) stockhamPass<T, 16>(buf, N, Ns, sn, bt, bFast, tw, sign);
+}
+
+// Single-level batch: transform b at offset b*sb.
+template <typename T>
+__device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, int sn, int sb, int nb, bool bFast,
+                                       const cx<T>* __restrict__ tw, int sign) {
+    fftLds(buf, N, log2N, sn, Batch{nb, nb, sb, 0}, bFast, tw, sign);
+}
+
+// Copies the N-entry twiddle table into LDS (block-wide; caller synchronizes).
+template <typename T>
+__device__ __forceinline__ void loadTwiddles(cx<T>* dst, const cx<T>* __restrict__ src, int N) {
+    for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
 }
 
 }  // namespace wvamd

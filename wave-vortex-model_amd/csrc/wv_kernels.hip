@@ -148,11 +148,13 @@ __global__ __launch_bounds__(256) void k_fgfields(const cx<double>* __restrict__
     FV[idx] = {T(v.re), T(v.im)};
 }
 
-// Spectral assembly + vertical synthesis (MFMA).  One wave computes, for 8
-// consecutive columns and one group, ZT tiles of 16 z levels:
+// Spectral assembly + vertical synthesis (MFMA).  One workgroup owns 8
+// consecutive columns: it stages their evolved field spectra (all j < Nj) in
+// LDS once, then its waves loop over (group, z-chunk) tasks, each computing ZT
+// tiles of 16 z levels for the group's channels:
 //   H[ch][z][iy][kx] = sum_j syn_fam(ch)[z][j] * value_ch(j, column).
 // Complex values are two real GEMM columns; lane l owns GEMM column l%16
-// (column l%32/2, real or imaginary part) and k-row l/16.
+// (column (l%16)/2, real or imaginary part) and k-row l/16.
 template <typename T, int ZT>
 __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
                                                int ncol, int M, const T* __restrict__ synC,
@@ -160,87 +162,100 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
                                                int nkxP, const T* __restrict__ mj, SynthPlan plan,
                                                cx<T>* __restrict__ H) {
     using MF = Mfma<T>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* fv = reinterpret_cast<cx<T>*>(smem);  // [4 fields][Js][8 columns]
+    __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int ncg = (ncol + 7) / 8;
+    const int waveInBlock = threadIdx.x >> 6;
+    const int nwaves = blockDim.x >> 6;
+    const int cg = blockIdx.x;
     const int ztiles = Zs / 16;
     const int nzc = (ztiles + ZT - 1) / ZT;
-    if (wave >= ncg * plan.ngroups * nzc) return;
-    const int cg = wave % ncg;
-    const int rest = wave / ncg;
-    const int gi = rest % plan.ngroups;
-    const int zc = rest / plan.ngroups;
-    const SynthGroup& G = plan.g[gi];
-    const int n = lane & 15, kq = lane >> 4;
-    const int col = cg * 8 + (n >> 1);
-    const int part = n & 1;
-    const bool colValid = col < ncol;
-    ColumnDev cd{};
-    if (colValid) cd = cols[col];
-    const int zt0 = zc * ZT;
-
-    typename MF::acc acc[4][ZT];
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int q = 0; q < ZT; ++q) acc[c][q] = MF::zero();
-
     const int nsteps = Js / 4;
-    for (int s = 0; s < nsteps; ++s) {
-        const int j = 4 * s + kq;
-        T b[4] = {T(0), T(0), T(0), T(0)};
-        if (colValid && j < Nj) {
-            const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
-            const bool endpoint = (j == 0 || j + 1 == Nz);
-            const T half = T(0.5);
+    if (threadIdx.x < 8) {
+        const int col = cg * 8 + threadIdx.x;
+        if (col < ncol) scol[threadIdx.x] = cols[col];
+        else scol[threadIdx.x] = ColumnDev{0, 0, -1, 0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    }
+    __syncthreads();
+    // stage spectra: fv[f][j][c], loads contiguous in j for each column
+    for (int e = threadIdx.x; e < 4 * Js * 8; e += blockDim.x) {
+        const int j = e % Js, fc = e / Js;
+        const int c = fc % 8, f = fc / 8;
+        const int wv = scol[c].wv;
+        cx<T> v{T(0), T(0)};
+        if (wv >= 0 && j < Nj) v = FV[static_cast<size_t>(f) * M + j + static_cast<size_t>(Nj) * wv];
+        fv[(f * Js + j) * 8 + c] = v;
+    }
+    __syncthreads();
+    const int n = lane & 15, kq = lane >> 4;
+    const int c = n >> 1, part = n & 1;
+    const ColumnDev cd = scol[c];
+    const bool colValid = cd.wv >= 0;
+    const T kx = T(cd.k), ly = T(cd.l);
+    for (int task = waveInBlock; task < plan.ngroups * nzc; task += nwaves) {
+        const int gi = task % plan.ngroups;
+        const int zc = task / plan.ngroups;
+        const SynthGroup& G = plan.g[gi];
+        const int zt0 = zc * ZT;
+        typename MF::acc acc[4][ZT];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if (c < G.nch) {
-                    const cx<T> f = FV[static_cast<size_t>(G.field[c]) * M + idx];
-                    const bool sineOut = G.fam[c] == 1;
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int r = 0; r < ZT; ++r) acc[q][r] = MF::zero();
+        for (int s = 0; s < nsteps; ++s) {
+            const int j = 4 * s + kq;
+            const bool endpoint = (j == 0 || j + 1 == Nz);
+            const T mjj = j < Nj ? mj[j] : T(0);
+            T b[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q < G.nch) {
+                    const cx<T> f = fv[(G.field[q] * Js + j) * 8 + c];
+                    const bool sineOut = G.fam[q] == 1;
                     cx<T> val;
-                    switch (G.op[c]) {
+                    switch (G.op[q]) {
                         case 0: val = f; break;
-                        case 1: val = cmuli(f, T(cd.k)); break;
-                        case 2: val = cmuli(f, T(cd.l)); break;
-                        default: val = cscale(f, sineOut ? -mj[j] : mj[j]); break;
+                        case 1: val = cmuli(f, kx); break;
+                        case 2: val = cmuli(f, ly); break;
+                        default: val = cscale(f, sineOut ? -mjj : mjj); break;
                     }
-                    val = (sineOut && endpoint) ? cx<T>{T(0), T(0)} : cscale(val, half);
+                    val = (sineOut && endpoint) ? cx<T>{T(0), T(0)} : cscale(val, T(0.5));
                     if (cd.flags & kColConj) val.im = -val.im;
                     if (cd.flags & kColSelfConj) val.im = T(0);
-                    b[c] = part ? val.im : val.re;
+                    b[q] = part ? val.im : val.re;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < ZT; ++r) {
+                const int zt = zt0 + r;
+                if (zt < ztiles) {
+                    // operands pre-arranged as [ztile][step][lane]: one coalesced load per wave
+                    const size_t a_off = (static_cast<size_t>(zt) * nsteps + s) * 64 + lane;
+                    const T aC = synC[a_off];
+                    const T aS = synS[a_off];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (q < G.nch) acc[q][r] = MF::run(G.fam[q] ? aS : aC, b[q], acc[q][r]);
                 }
             }
         }
+        if (!colValid) continue;
 #pragma unroll
-        for (int q = 0; q < ZT; ++q) {
-            const int zt = zt0 + q;
-            if (zt < ztiles) {
-                // operands pre-arranged as [ztile][step][lane]: one coalesced load per wave
-                const size_t a_off = (static_cast<size_t>(zt) * nsteps + s) * 64 + lane;
-                const T aC = synC[a_off];
-                const T aS = synS[a_off];
+        for (int q = 0; q < 4; ++q) {
+            if (q >= G.nch) continue;
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (c < G.nch) acc[c][q] = MF::run(G.fam[c] ? aS : aC, b[c], acc[c][q]);
-            }
-        }
-    }
-    if (!colValid) return;
+            for (int r = 0; r < ZT; ++r) {
+                const int zt = zt0 + r;
+                if (zt >= ztiles) continue;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        if (c >= G.nch) continue;
-#pragma unroll
-        for (int q = 0; q < ZT; ++q) {
-            const int zt = zt0 + q;
-            if (zt >= ztiles) continue;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int z = 16 * zt + MF::row(kq, i);
-                if (z < Nz) {
-                    T* dst = reinterpret_cast<T*>(
-                        &H[((static_cast<size_t>(G.outch[c]) * Nz + z) * Ny + cd.iy) * nkxP + cd.kx]);
-                    dst[part] = acc[c][q][i];
+                for (int i = 0; i < 4; ++i) {
+                    const int z = 16 * zt + MF::row(kq, i);
+                    if (z < Nz) {
+                        T* dst = reinterpret_cast<T*>(
+                            &H[((static_cast<size_t>(G.outch[q]) * Nz + z) * Ny + cd.iy) * nkxP + cd.kx]);
+                        dst[part] = acc[q][r][i];
+                    }
                 }
             }
         }
@@ -248,23 +263,31 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
 }
 
 // Complex FFT along y of [Ny, KT] tiles: in/out layout [ch][z][iy][nkxP].
+// rowLimit[tile] (if given) is the largest |ky| retained in the kx tile: input
+// rows r < iy < Ny - r are known zeros (skipIn) or output rows there are not
+// needed (skipOut).
 template <typename T>
 __global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int Ny,
                                               int log2Ny, int nkxP, int KT, int Nz, const cx<T>* __restrict__ twY,
-                                              int sign) {
+                                              int sign, const int* __restrict__ rowLimit, int skipIn, int skipOut) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* tile = tw + Ny;
     const int kx0 = blockIdx.x * KT;
+    const int lim = rowLimit ? rowLimit[blockIdx.x] : Ny;
     const size_t plane = (static_cast<size_t>(blockIdx.z) * Nz + blockIdx.y) * static_cast<size_t>(Ny) * nkxP;
     const int nel = Ny * KT;
+    loadTwiddles(tw, twY, Ny);
     for (int e = threadIdx.x; e < nel; e += blockDim.x) {
         const int kk = e % KT, iy = e / KT;
-        tile[e] = in[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk];
+        const bool zero = skipIn && iy > lim && iy < Ny - lim;
+        tile[e] = zero ? cx<T>{T(0), T(0)} : in[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk];
     }
     __syncthreads();
-    fftLds(tile, Ny, log2Ny, KT, 1, KT, true, twY, sign);
+    fftLds(tile, Ny, log2Ny, KT, 1, KT, true, tw, sign);
     for (int e = threadIdx.x; e < nel; e += blockDim.x) {
         const int kk = e % KT, iy = e / KT;
+        if (skipOut && iy > lim && iy < Ny - lim) continue;
         out[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk] = tile[e];
     }
 }
@@ -283,31 +306,41 @@ __device__ __forceinline__ cx<T> hermitianX(const cx<T>* __restrict__ G, size_t 
 
 // Per (y, z) row: c2r of the NCH channels along x (two real transforms per
 // complex FFT), the NT advection products and r2c of the products.
-template <typename T, int NCH, int NT>
+template <typename T, int NCH, int NT, int RB>
 __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P, int Nx,
                                                 int log2Nx, int Ny, int Nz, int nkx, int nkxP,
                                                 const cx<T>* __restrict__ twX) {
     constexpr int PAIRS = (NCH + 1) / 2;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int NxP = ldsLength(Nx);
-    const int y = blockIdx.x, z = blockIdx.y;
-    const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+    cx<T>* buf = tw + Nx;  // [RB][PAIRS][NxP]
+    const int nrows = Ny * Nz;
+    const int row0 = blockIdx.x * RB;  // row = z*Ny + y
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
-    for (int e = threadIdx.x; e < PAIRS * Nx; e += blockDim.x) {
-        const int p = e / Nx, k = e % Nx;
-        const cx<T> A = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
-        const cx<T> B = (2 * p + 1 < NCH) ? hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx) : cx<T>{T(0), T(0)};
-        buf[p * NxP + ldsIndex(k, 1)] = {A.re - B.im, A.im + B.re};
+    loadTwiddles(tw, twX, Nx);
+    for (int e = threadIdx.x; e < RB * PAIRS * Nx; e += blockDim.x) {
+        const int k = e % Nx, rp = e / Nx;  // rp = rr*PAIRS + p
+        const int p = rp % PAIRS, row = row0 + rp / PAIRS;
+        cx<T> val{T(0), T(0)};
+        if (row < nrows) {
+            const size_t rowBase = static_cast<size_t>(row) * nkxP;
+            const cx<T> A = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
+            const cx<T> B = (2 * p + 1 < NCH) ? hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx) : cx<T>{T(0), T(0)};
+            val = {A.re - B.im, A.im + B.re};
+        }
+        buf[rp * NxP + ldsIndex(k, 1)] = val;
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, NxP, PAIRS, false, twX, +1);
-    for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+    fftLds(buf, Nx, log2Nx, 1, NxP, RB * PAIRS, false, tw, +1);
+    for (int e = threadIdx.x; e < RB * Nx; e += blockDim.x) {
+        const int x = e % Nx, rr = e / Nx;
         const int xl = ldsIndex(x, 1);
+        cx<T>* rb = buf + rr * PAIRS * NxP;
         T ch[2 * PAIRS];
 #pragma unroll
         for (int p = 0; p < PAIRS; ++p) {
-            const cx<T> v = buf[p * NxP + xl];
+            const cx<T> v = rb[p * NxP + xl];
             ch[2 * p] = v.re;
             ch[2 * p + 1] = v.im;
         }
@@ -315,21 +348,24 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
         T F[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int t = 0; t < NT; ++t) F[t] = -(U * ch[3 + 3 * t] + V * ch[4 + 3 * t] + W * ch[5 + 3 * t]);
-        buf[xl] = {F[0], F[1]};
-        buf[NxP + xl] = {F[2], F[3]};
+        rb[xl] = {F[0], F[1]};
+        rb[NxP + xl] = {F[2], F[3]};
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, NxP, 2, false, twX, -1);
-    const size_t tStride = chStride;
-    for (int e = threadIdx.x; e < 2 * nkx; e += blockDim.x) {
-        const int q = e / nkx, k = e % nkx;
-        const cx<T> Za = buf[q * NxP + ldsIndex(k, 1)];
-        const cx<T> Zb = buf[q * NxP + ldsIndex(k == 0 ? 0 : Nx - k, 1)];
+    // forward transforms: pairs 0 and 1 of every row
+    fftLds(buf, Nx, log2Nx, 1, Batch{2 * RB, 2, NxP, PAIRS * NxP}, false, tw, -1);
+    for (int e = threadIdx.x; e < RB * 2 * nkx; e += blockDim.x) {
+        const int k = e % nkx, rq = e / nkx;
+        const int q = rq % 2, rr = rq / 2;
+        const int row = row0 + rr;
+        if (row >= nrows) continue;
+        const cx<T>* rb = buf + (rr * PAIRS + q) * NxP;
+        const cx<T> Za = rb[ldsIndex(k, 1)];
+        const cx<T> Zb = rb[ldsIndex(k == 0 ? 0 : Nx - k, 1)];
         const T h = T(0.5);
-        const cx<T> Fa{(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
-        const cx<T> Fb{(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
-        if (2 * q < NT) P[(2 * q) * tStride + rowBase + k] = Fa;
-        if (2 * q + 1 < NT) P[(2 * q + 1) * tStride + rowBase + k] = Fb;
+        const size_t o = static_cast<size_t>(row) * nkxP + k;
+        if (2 * q < NT) P[(2 * q) * chStride + o] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
+        if (2 * q + 1 < NT) P[(2 * q + 1) * chStride + o] = {(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
     }
 }
 
@@ -362,61 +398,71 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
 }
 
 // Vertical analysis (MFMA) + wave-vortex projection + flux store or RK4 stage.
-// One wave: 8 consecutive columns x 16 vertical modes j, all targets.
+// One workgroup owns 8 consecutive columns and has one wave per 16-row tile
+// of j; each target's Q columns (all z) are staged in LDS once and shared by
+// the waves.
 template <typename T, int NT>
-__global__ __launch_bounds__(256) void k_proj(const cx<T>* __restrict__ Q, const ColumnDev* __restrict__ cols,
-                                              int ncol, const T* __restrict__ anaC, const T* __restrict__ anaS,
-                                              int Ja, int Za, int Nz, int Nj, int Ny, int nkxP,
-                                              int targetCode, int hydro, ProjTables tb, double elapsed,
-                                              double hscale, Epilogue ep) {
+__global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, const ColumnDev* __restrict__ cols,
+                                               int ncol, const T* __restrict__ anaC, const T* __restrict__ anaS,
+                                               int Ja, int Za, int Nz, int Nj, int Ny, int nkxP,
+                                               int targetCode, int hydro, ProjTables tb, double elapsed,
+                                               double hscale, Epilogue ep) {
     using MF = Mfma<T>;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [Za][8]
+    __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int ncg = (ncol + 7) / 8;
-    const int njt = Ja / 16;
-    if (wave >= ncg * njt) return;
-    const int cg = wave % ncg;
-    const int jt = wave / ncg;
+    const int jt = threadIdx.x >> 6;
+    const int cg = blockIdx.x;
+    if (threadIdx.x < 8) {
+        const int col = cg * 8 + threadIdx.x;
+        if (col < ncol) scol[threadIdx.x] = cols[col];
+        else scol[threadIdx.x] = ColumnDev{0, 0, -1, kColCompletion, 0.0, 0.0, 0.0, 0.0, 0.0};
+    }
     const int n = lane & 15, kq = lane >> 4;
-    const int col = cg * 8 + (n >> 1);
     const int part = n & 1;
-    const bool colValid = col < ncol;
-    ColumnDev cd{};
-    if (colValid) cd = cols[col];
     const size_t tStride = static_cast<size_t>(Nz) * Ny * nkxP;
-    const size_t colOff = static_cast<size_t>(cd.iy) * nkxP + cd.kx;
+    const int nsteps = Za / 4;
 
     typename MF::acc acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = MF::zero();
-    const int nsteps = Za / 4;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
+        __syncthreads();  // scol ready / previous target consumed
+        for (int e = threadIdx.x; e < Za * 8; e += blockDim.x) {
+            const int c = e % 8, z = e / 8;
+            const ColumnDev& cc = scol[c];
+            cx<T> v{T(0), T(0)};
+            if (cc.wv >= 0 && z < Nz) v = Q[t * tStride + (static_cast<size_t>(z) * Ny + cc.iy) * nkxP + cc.kx];
+            qs[e] = v;
+        }
+        __syncthreads();
         const int target = (targetCode >> (4 * t)) & 0xF;
         const T* A = target < 2 ? anaC : anaS;
         typename MF::acc a4 = MF::zero();
         for (int s = 0; s < nsteps; ++s) {
-            const int z = 4 * s + kq;
-            T b = T(0);
-            if (colValid && z < Nz) {
-                const cx<T> q = Q[t * tStride + static_cast<size_t>(z) * Ny * nkxP + colOff];
-                b = part ? q.im : q.re;
-            }
-            const T a = A[(static_cast<size_t>(jt) * nsteps + s) * 64 + lane];
-            a4 = MF::run(a, b, a4);
+            const cx<T> q = qs[(4 * s + kq) * 8 + (n >> 1)];
+            a4 = MF::run(A[(static_cast<size_t>(jt) * nsteps + s) * 64 + lane], part ? q.im : q.re, a4);
         }
         acc[t] = a4;
     }
-    // complex coefficient c_t(j, column) for rows j = 16 jt + row(kq, i)
-    double cre[4][4], cim[4][4];
+    const ColumnDev cd = scol[n >> 1];
+    const bool colValid = cd.wv >= 0;
+    // complex coefficient c_t(j, column) for rows j = 16 jt + row(kq, i); the
+    // even lane of a pair keeps items 0,1 and the odd lane items 2,3
+    double cre[4][2], cim[4][2];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const double mine = static_cast<double>(acc[t][i]);
             const double other = __shfl_xor(mine, 1);
-            cre[t][i] = part ? other : mine;
-            cim[t][i] = part ? mine : other;
+            const double re = part ? other : mine, im = part ? mine : other;
+            if ((i >> 1) == part) {
+                cre[t][i & 1] = re;
+                cim[t][i & 1] = im;
+            }
         }
     }
     if (!colValid || (cd.flags & kColCompletion)) return;
@@ -436,7 +482,7 @@ __global__ __launch_bounds__(256) void k_proj(const cx<T>* __restrict__ Q, const
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int target = (targetCode >> (4 * t)) & 0xF;
-            cx<double> value{cre[t][i], conj ? -cim[t][i] : cim[t][i]};
+            cx<double> value{cre[t][ii], conj ? -cim[t][ii] : cim[t][ii]};
             value = cscale(value, hscale);
             cx<double> a0{0.0, 0.0};
             if (target == 0) a0 = cscale(cmuli(value, -cd.l), a0v);
@@ -551,8 +597,9 @@ __global__ __launch_bounds__(256) void k_projfwd(const cx<T>* __restrict__ Q, co
     const int ncg = (ncol + 7) / 8;
     const int njt = Ja / 16;
     if (wave >= ncg * njt) return;
-    const int cg = wave % ncg;
-    const int jt = wave / ncg;
+    // consecutive waves share a column group, so its Q columns stay in L2
+    const int jt = wave % njt;
+    const int cg = wave / njt;
     const int n = lane & 15, kq = lane >> 4;
     const int col = cg * 8 + (n >> 1);
     const int part = n & 1;
@@ -731,7 +778,9 @@ struct KernelImpl {
         apmW, inertialScale, fWaveScale, gWaveScale, Fg, Gg;
     DeviceBuffer synC, synS, anaC, anaS, twX, twY, mj;
     DeviceBuffer FV, H, G, P, Q, stage, weighted, partial, scalar;
-    DeviceBuffer fieldsScratch;
+    DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
+    double retainedRowFraction = 1.0;
+    size_t maxLdsPerBlock = 65536;
 
     bool instrument = false;
     std::vector<StageRecord> records;
@@ -743,7 +792,7 @@ struct KernelImpl {
 
     size_t persistentBytes() const {
         size_t s = 0;
-        for (const DeviceBuffer* b : {&cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
+        for (const DeviceBuffer* b : {&rowLimit, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
                                       &anaS, &twX, &twY, &mj, &FV, &H, &G, &P, &Q, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
@@ -846,9 +895,10 @@ struct KernelImpl {
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx <= 512, Ny <= 4096)."};
         int maxLds = 0;
         hipCheck(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+        maxLdsPerBlock = static_cast<size_t>(maxLds);
         const size_t elem = prec == WV_PRECISION_FP64 ? 16 : 8;
         const size_t xLds = pairs * static_cast<size_t>(ldsLength(static_cast<int>(cfg.Nx))) * elem;
-        const size_t yLds = static_cast<size_t>(KT) * cfg.Ny * elem;
+        const size_t yLds = (static_cast<size_t>(KT) + 1) * cfg.Ny * elem;
         if (xLds > static_cast<size_t>(maxLds) || yLds > static_cast<size_t>(maxLds))
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid needs more LDS per workgroup than the device provides."};
         // columns
@@ -860,6 +910,19 @@ struct KernelImpl {
             hc[i] = {c.iy, c.kx, c.wv, c.flags, fm.k, fm.l, fm.Kh, fm.cosAlpha, fm.sinAlpha};
         }
         upload(cols, hc);
+        {
+            const size_t tiles = d.nkxP / static_cast<size_t>(KT);
+            std::vector<int> lim(tiles, 0);
+            for (const Column& c : d.columns) {
+                const long ky = c.iy <= static_cast<int>(cfg.Ny / 2) ? c.iy : c.iy - static_cast<long>(cfg.Ny);
+                int& L = lim[static_cast<size_t>(c.kx) / static_cast<size_t>(KT)];
+                L = std::max(L, static_cast<int>(std::labs(ky)));
+            }
+            size_t kept = 0;
+            for (int L : lim) kept += std::min(cfg.Ny, 2 * static_cast<size_t>(L) + 1);
+            retainedRowFraction = static_cast<double>(kept) / static_cast<double>(tiles * cfg.Ny);
+            upload(rowLimit, lim);
+        }
         upload(omega, d.omega);
         upload(UAp, d.UAp);
         upload(VAp, d.VAp);
@@ -928,16 +991,14 @@ struct KernelImpl {
 
     template <typename T>
     void launchSynth(const SynthPlan& sp, hipStream_t s) {
-        constexpr int ZT = 3;
+        constexpr int ZT = sizeof(T) == 8 ? 2 : 3;  // z tiles per task (FP64 accumulators are twice the size)
         const int ncg = static_cast<int>((ncol + 7) / 8);
-        const int ztiles = static_cast<int>(d.Zs / 16);
-        const int nzc = (ztiles + ZT - 1) / ZT;
-        const int waves = ncg * sp.ngroups * nzc;
         int nch = 0;
         for (int g = 0; g < sp.ngroups; ++g) nch += sp.g[g].nch;
         const double bytes = static_cast<double>(nch) * d.c.Nz * ncol * sizeof(cx<T>) + 4.0 * d.M * sizeof(cx<T>);
+        const size_t lds = 4 * d.Js * 8 * sizeof(cx<T>);
         timed("synth", bytes, s, [&] {
-            hipLaunchKernelGGL((k_synth<T, ZT>), dim3((waves + 3) / 4), dim3(256), 0, s, FV.as<cx<T>>(),
+            hipLaunchKernelGGL((k_synth<T, ZT>), dim3(static_cast<unsigned>(ncg)), dim3(256), lds, s, FV.as<cx<T>>(),
                                cols.as<ColumnDev>(), static_cast<int>(ncol), static_cast<int>(d.M), synC.as<T>(),
                                synS.as<T>(), static_cast<int>(d.Zs), static_cast<int>(d.Js),
                                static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
@@ -945,56 +1006,78 @@ struct KernelImpl {
         });
     }
 
+    // skipIn: input rows outside the tile's retained ky band are zeros;
+    // skipOut: output rows outside it are not needed.
     template <typename T>
-    void launchYfft(const DeviceBuffer& in, DeviceBuffer& out, int nch, int sign, hipStream_t s, const char* name) {
-        const size_t lds = static_cast<size_t>(KT) * d.c.Ny * sizeof(cx<T>);
-        const double bytes = 2.0 * nch * d.c.Nz * d.c.Ny * d.nkxP * sizeof(cx<T>);
+    void launchYfft(const DeviceBuffer& in, DeviceBuffer& out, int nch, int sign, hipStream_t s, const char* name,
+                    bool skipIn, bool skipOut) {
+        const size_t lds = (static_cast<size_t>(KT) + 1) * d.c.Ny * sizeof(cx<T>);
+        const double frac = (skipIn || skipOut) ? 0.5 * (1.0 + retainedRowFraction) : 1.0;
+        const double bytes = 2.0 * frac * nch * d.c.Nz * d.c.Ny * d.nkxP * sizeof(cx<T>);
         timed(name, bytes, s, [&] {
             hipLaunchKernelGGL(k_yfft<T>, dim3(static_cast<unsigned>(d.nkxP / KT), static_cast<unsigned>(d.c.Nz), nch),
                                dim3(256), lds, s, in.as<cx<T>>(), out.as<cx<T>>(), static_cast<int>(d.c.Ny),
                                log2Exact(d.c.Ny), static_cast<int>(d.nkxP), KT, static_cast<int>(d.c.Nz),
-                               twY.as<cx<T>>(), sign);
+                               twY.as<cx<T>>(), sign, rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0);
         });
+    }
+
+    template <typename T, int RB>
+    size_t xfusedLds() const {
+        const int pairs = (NCH + 1) / 2;
+        return (static_cast<size_t>(d.c.Nx) + static_cast<size_t>(RB) * pairs * ldsLength(static_cast<int>(d.c.Nx))) *
+               sizeof(cx<T>);
     }
 
     template <typename T>
     void launchXfused(hipStream_t s) {
-        const int pairs = (NCH + 1) / 2;
-        const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         const double bytes = (static_cast<double>(NCH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>);
-        const dim3 grid(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz));
         const int Nx = static_cast<int>(d.c.Nx), l2 = log2Exact(d.c.Nx), Ny = static_cast<int>(d.c.Ny),
                   Nz = static_cast<int>(d.c.Nz), nkx = static_cast<int>(d.nkx), nkxP = static_cast<int>(d.nkxP);
+        const int nrows = Ny * Nz;
+        const bool two = xfusedLds<T, 2>() <= maxLdsPerBlock;
+        const size_t lds = two ? xfusedLds<T, 2>() : xfusedLds<T, 1>();
+        const dim3 grid(static_cast<unsigned>(two ? (nrows + 1) / 2 : nrows));
         timed("xfused", bytes, s, [&] {
-            if (NT == 4)
-                hipLaunchKernelGGL((k_xfused<T, 15, 4>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, l2,
-                                   Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
-            else
-                hipLaunchKernelGGL((k_xfused<T, 12, 3>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, l2,
-                                   Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+            if (NT == 4) {
+                if (two)
+                    hipLaunchKernelGGL((k_xfused<T, 15, 4, 2>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
+                                       Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+                else
+                    hipLaunchKernelGGL((k_xfused<T, 15, 4, 1>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
+                                       Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+            } else {
+                if (two)
+                    hipLaunchKernelGGL((k_xfused<T, 12, 3, 2>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
+                                       Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+                else
+                    hipLaunchKernelGGL((k_xfused<T, 12, 3, 1>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
+                                       Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+            }
         });
     }
 
     template <typename T>
     void launchProj(double elapsed, const Epilogue& ep, hipStream_t s) {
         const int ncg = static_cast<int>((ncol + 7) / 8);
-        const int waves = ncg * static_cast<int>(d.Ja / 16);
         ProjTables tb{omega.as<double>(), A0Vort.as<double>(), A0Buoy.as<double>(), NA0.as<double>(),
                       ApmN.as<double>(), ApmDScaled.as<double>(), ApmD.as<cx<double>>(), apmW.as<double>(),
                       inertialScale.as<double>()};
         const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
         const double bytes = static_cast<double>(NT) * d.c.Nz * ncol * sizeof(cx<T>) + d.M * (48.0 + 48.0 + 48.0);
         timed("proj", bytes, s, [&] {
-            const dim3 grid((waves + 3) / 4), block(256);
+            const int njt = static_cast<int>(d.Ja / 16);
+            const dim3 grid(static_cast<unsigned>(ncg)), block(static_cast<unsigned>(64 * njt));
+            const size_t lds = d.Za * 8 * sizeof(cx<T>);
             const int ncolI = static_cast<int>(ncol), Ja = static_cast<int>(d.Ja), Za = static_cast<int>(d.Za),
                       Nz = static_cast<int>(d.c.Nz), Nj = static_cast<int>(d.c.Nj), Ny = static_cast<int>(d.c.Ny),
                       nkxP = static_cast<int>(d.nkxP), hydro = d.c.isHydrostatic ? 1 : 0;
             if (NT == 4)
-                hipLaunchKernelGGL((k_proj<T, 4>), grid, block, 0, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
+                hipLaunchKernelGGL((k_proj<T, 4>), grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
                                    anaC.as<T>(), anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, targetCode, hydro, tb,
                                    elapsed, hscale, ep);
             else
-                hipLaunchKernelGGL((k_proj<T, 3>), grid, block, 0, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
+                hipLaunchKernelGGL((k_proj<T, 3>), grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
                                    anaC.as<T>(), anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, targetCode, hydro, tb,
                                    elapsed, hscale, ep);
         });
@@ -1006,9 +1089,9 @@ struct KernelImpl {
         const double elapsed = t - t0;
         launchFields<T>(Ap, Am, A0, elapsed, s);
         launchSynth<T>(fluxPlan(), s);
-        launchYfft<T>(H, G, NCH, +1, s, "yfft_inv");
+        launchYfft<T>(H, G, NCH, +1, s, "yfft_inv", true, false);
         launchXfused<T>(s);
-        launchYfft<T>(P, Q, NT, -1, s, "yfft_fwd");
+        launchYfft<T>(P, Q, NT, -1, s, "yfft_fwd", false, true);
         launchProj<T>(elapsed, ep, s);
     }
 
@@ -1027,7 +1110,7 @@ struct KernelImpl {
         if (nf == 4) sp.g[0] = {4, {0, 1, 2, 3}, {0, 0, 0, 0}, {0, 0, 1, 1}, {0, 1, 2, 3}};
         else sp.g[0] = {3, {0, 1, 2, 0}, {0, 0, 0, 0}, {0, 0, 1, 0}, {0, 1, 2, 0}};
         launchSynth<T>(sp, s);
-        launchYfft<T>(H, G, nf, +1, s, "yfft_inv");
+        launchYfft<T>(H, G, nf, +1, s, "yfft_inv", true, false);
         const int pairs = (nf + 1) / 2;
         const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xinv", static_cast<double>(nf) * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
@@ -1051,7 +1134,7 @@ struct KernelImpl {
                                static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
                                static_cast<int>(d.nkxP), twX.as<cx<T>>());
         });
-        launchYfft<T>(P, Q, nf, -1, s, "yfft_fwd");
+        launchYfft<T>(P, Q, nf, -1, s, "yfft_fwd", false, true);
         const int ncg = static_cast<int>((ncol + 7) / 8);
         const int waves = ncg * static_cast<int>(d.Ja / 16);
         ProjTables tb{omega.as<double>(), A0Vort.as<double>(), A0Buoy.as<double>(), NA0.as<double>(),
@@ -1093,7 +1176,7 @@ struct KernelImpl {
         const int a = fFamily ? 0 : 1, b = 1 - a;
         sp.g[0] = {4, {0, 0, 0, 0}, {0, 1, 2, 3}, {a, a, a, b}, {0, 1, 2, 3}};
         launchSynth<T>(sp, s);
-        launchYfft<T>(H, G, 4, +1, s, "yfft_inv");
+        launchYfft<T>(H, G, 4, +1, s, "yfft_inv", true, false);
         const size_t lds = static_cast<size_t>(2) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xinv", 4.0 * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
             hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
