@@ -57,21 +57,56 @@ struct Mfma<double> {
 
 // ----------------------------------------------------------------- FFT ----
 // In-place batched complex FFT in LDS, all threads of the block participate.
-// Element n of transform b lives at buf[lds(n)*sn + b*sb] where lds(n) = n for
-// strided layouts (sn > 1) and n + n/16 for unit-stride layouts (sn == 1, one
-// pad slot per 16 entries so that radix-16 passes are free of bank
-// conflicts).  nb transforms of length N.
-// X_k = sum_n x_n exp(sign*2*pi*i*n*k/N); tw[q] = exp(-2*pi*i*q/N).
+// Two batch layouts:
+//   BatchRows<NB1>: transform b is contiguous (unit stride, one pad slot per 16
+//     entries so that radix-16 passes are free of bank conflicts) and starts at
+//     (b % NB1)*sb + (b / NB1)*sb2; consecutive threads walk one transform.
+//   BatchCols: nb = 2^log2nb interleaved transforms, element n of transform b
+//     at n*nb + b; consecutive threads walk the transforms.
+// X_k = sum_n x_n exp(SIGN*2*pi*i*n*k/N); tw[q] = exp(-2*pi*i*q/N).
 // Power-of-two N runs Stockham autosort passes of radix 16 (then 8, 4 or 2
-// for the remaining factor), each butterfly an in-register DFT; other N use a
-// direct DFT.  A pass holds at most 16 complex values per thread, so the
-// caller must launch at least nb*N/16 threads.  The caller synchronizes
-// before the call; the function returns after a final barrier.
+// for the remaining factor), each butterfly an in-register DFT, with all index
+// arithmetic in shifts and masks; other N use a direct DFT.  A pass holds at
+// most 16 complex values per thread, so the caller must launch at least
+// nb*N/16 threads.  The caller synchronizes before the call; the function
+// returns after a final barrier.
 constexpr int kFftRegs = 16;
 
 __device__ __forceinline__ int ldsIndex(int n, int sn) { return sn == 1 ? n + (n >> 4) : n * sn; }
 // padded length of a unit-stride transform
 __host__ __device__ constexpr int ldsLength(int N) { return N + (N >> 4); }
+
+template <int NB1>
+struct BatchRows {
+    int nb, sb, sb2;
+    __device__ __forceinline__ int base(int b) const { return (b % NB1) * sb + (b / NB1) * sb2; }
+    __device__ __forceinline__ static int at(int n) { return n + (n >> 4); }
+    // item -> (transform, position) when each transform holds 2^log2n items
+    __device__ __forceinline__ void split(int item, int log2n, int& b, int& j) const {
+        b = item >> log2n;
+        j = item & ((1 << log2n) - 1);
+    }
+    __device__ __forceinline__ void splitN(int item, int n, int& b, int& j) const {
+        b = item / n;
+        j = item - b * n;
+    }
+};
+
+struct BatchCols {
+    int log2nb;
+    __device__ __forceinline__ int nbt() const { return 1 << log2nb; }
+    __device__ __forceinline__ int base(int b) const { return b; }
+    __device__ __forceinline__ int at(int n) const { return n << log2nb; }
+    __device__ __forceinline__ void split(int item, int, int& b, int& j) const {
+        b = item & ((1 << log2nb) - 1);
+        j = item >> log2nb;
+    }
+    __device__ __forceinline__ void splitN(int item, int, int& b, int& j) const { split(item, 0, b, j); }
+};
+
+template <int NB1>
+__device__ __forceinline__ int batchCount(const BatchRows<NB1>& bt) { return bt.nb; }
+__device__ __forceinline__ int batchCount(const BatchCols& bt) { return bt.nbt(); }
 
 // exp(-2 pi i q / 16), q = 0..15
 __device__ __forceinline__ double w16cos(int q) {
@@ -85,8 +120,8 @@ __device__ __forceinline__ double w16sin(int q) { return -w16cos((q + 12) & 15);
 
 // In-register DFT of R = 2, 4, 8 or 16 points, natural order in and out
 // (radix-2 decimation in frequency, then a compile-time bit-reversal).
-template <typename T, int R>
-__device__ __forceinline__ void dftReg(cx<T>* v, int sign) {
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dftReg(cx<T>* v) {
 #pragma unroll
     for (int half = R / 2; half >= 1; half /= 2) {
 #pragma unroll
@@ -100,9 +135,9 @@ __device__ __forceinline__ void dftReg(cx<T>* v, int sign) {
                 if (qi == 0) {
                     v[s + q + half] = d;
                 } else if (qi == 4) {  // -/+ i
-                    v[s + q + half] = sign < 0 ? cx<T>{d.im, -d.re} : cx<T>{-d.im, d.re};
+                    v[s + q + half] = SIGN < 0 ? cx<T>{d.im, -d.re} : cx<T>{-d.im, d.re};
                 } else {
-                    const T wr = T(w16cos(qi)), wi = T(sign < 0 ? w16sin(qi) : -w16sin(qi));
+                    const T wr = T(w16cos(qi)), wi = T(SIGN < 0 ? w16sin(qi) : -w16sin(qi));
                     v[s + q + half] = {d.re * wr - d.im * wi, d.re * wi + d.im * wr};
                 }
             }
@@ -121,40 +156,41 @@ __device__ __forceinline__ void dftReg(cx<T>* v, int sign) {
     for (int k = 0; k < R; ++k) v[k] = tmp[k];
 }
 
-// Transform b of a batch lives at offset (b % nb1)*sb + (b / nb1)*sb2.
-struct Batch {
-    int nb, nb1, sb, sb2;
-    __device__ __forceinline__ int offset(int b) const { return (b % nb1) * sb + (b / nb1) * sb2; }
-};
+template <int R>
+constexpr int log2Radix() { return R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4; }
 
-template <typename T, int R>
-__device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, Batch bt, bool bFast,
-                                             const cx<T>* __restrict__ tw, int sign) {
+// One Stockham pass of radix R over transforms of length 2^log2N whose
+// sub-transforms so far have length 2^log2Ns.
+template <typename T, int R, int SIGN, typename Bt>
+__device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, const Bt& bt,
+                                             const cx<T>* __restrict__ tw) {
     constexpr int IPT = kFftRegs / R;
-    const int NR = N / R;
-    const int nb = bt.nb;
-    const int items = nb * NR;
-    const int tstep = N / (Ns * R);
+    constexpr int LR = log2Radix<R>();
+    const int log2NR = log2N - LR;
+    const int NR = 1 << log2NR;
+    const int items = batchCount(bt) << log2NR;
+    const int tshift = log2NR - log2Ns;  // twiddle stride N/(Ns R)
+    const int kmask = (1 << log2Ns) - 1;
     cx<T> v[IPT][R];
 #pragma unroll
     for (int it = 0; it < IPT; ++it) {
         const int item = threadIdx.x + it * blockDim.x;
         if (item < items) {
-            const int b = bFast ? item % nb : item / NR;
-            const int jj = bFast ? item / nb : item % NR;
-            const int k = jj % Ns;
-            const int boff = bt.offset(b);
+            int b, jj;
+            bt.split(item, log2NR, b, jj);
+            const int k = jj & kmask;
+            const int boff = bt.base(b);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                cx<T> x = buf[ldsIndex(jj + r * NR, sn) + boff];
+                cx<T> x = buf[bt.at(jj + r * NR) + boff];
                 if (r > 0 && k > 0) {
-                    cx<T> w = tw[r * k * tstep];
-                    if (sign > 0) w.im = -w.im;
+                    cx<T> w = tw[(r * k) << tshift];
+                    if (SIGN > 0) w.im = -w.im;
                     x = cmul(x, w);
                 }
                 v[it][r] = x;
             }
-            dftReg<T, R>(v[it], sign);
+            dftReg<T, R, SIGN>(v[it]);
         }
     }
     __syncthreads();
@@ -162,34 +198,32 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int N, int Ns, int sn, 
     for (int it = 0; it < IPT; ++it) {
         const int item = threadIdx.x + it * blockDim.x;
         if (item < items) {
-            const int b = bFast ? item % nb : item / NR;
-            const int jj = bFast ? item / nb : item % NR;
-            const int d = (jj / Ns) * Ns * R + (jj % Ns);
-            const int boff = bt.offset(b);
+            int b, jj;
+            bt.split(item, log2NR, b, jj);
+            const int d = ((jj >> log2Ns) << (log2Ns + LR)) + (jj & kmask);
+            const int boff = bt.base(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[ldsIndex(d + r * Ns, sn) + boff] = v[it][r];
+            for (int r = 0; r < R; ++r) buf[bt.at(d + (r << log2Ns)) + boff] = v[it][r];
         }
     }
     __syncthreads();
 }
 
-template <typename T>
-__device__ void directDft(cx<T>* buf, int N, int sn, Batch bt, bool bFast, const cx<T>* __restrict__ tw,
-                          int sign) {
+template <typename T, int SIGN, typename Bt>
+__device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restrict__ tw) {
     cx<T> out[kFftRegs];
-    const int nb = bt.nb;
-    const int items = nb * N;
+    const int items = batchCount(bt) * N;
 #pragma unroll
     for (int it = 0; it < kFftRegs; ++it) {
         const int item = threadIdx.x + it * blockDim.x;
         if (item < items) {
-            const int b = bFast ? item % nb : item / N;
-            const int k = bFast ? item / nb : item % N;
+            int b, k;
+            bt.splitN(item, N, b, k);
             cx<T> acc{T(0), T(0)};
             for (int n = 0; n < N; ++n) {
                 cx<T> w = tw[(n * k) % N];
-                if (sign > 0) w.im = -w.im;
-                acc = cadd(acc, cmul(buf[ldsIndex(n, sn) + bt.offset(b)], w));
+                if (SIGN > 0) w.im = -w.im;
+                acc = cadd(acc, cmul(buf[bt.at(n) + bt.base(b)], w));
             }
             out[it] = acc;
         }
@@ -199,37 +233,30 @@ __device__ void directDft(cx<T>* buf, int N, int sn, Batch bt, bool bFast, const
     for (int it = 0; it < kFftRegs; ++it) {
         const int item = threadIdx.x + it * blockDim.x;
         if (item < items) {
-            const int b = bFast ? item % nb : item / N;
-            const int k = bFast ? item / nb : item % N;
-            buf[ldsIndex(k, sn) + bt.offset(b)] = out[it];
+            int b, k;
+            bt.splitN(item, N, b, k);
+            buf[bt.at(k) + bt.base(b)] = out[it];
         }
     }
     __syncthreads();
 }
 
-template <typename T>
-__device__ void fftLds(cx<T>* buf, int N, int log2N, int sn, Batch bt, bool bFast, const cx<T>* __restrict__ tw,
-                       int sign) {
+// log2N < 0 marks a length that is not a power of two.
+template <typename T, int SIGN, typename Bt>
+__device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
     if (log2N < 0) {
-        directDft(buf, N, sn, bt, bFast, tw, sign);
+        directDft<T, SIGN>(buf, N, bt, tw);
         return;
     }
     // leading pass takes the factor 2^(log2N mod 4); the rest are radix 16
-    int Ns = 1;
+    int l2s = 0;
     switch (log2N & 3) {
-        case 1: stockhamPass<T, 2>(buf, N, Ns, sn, bt, bFast, tw, sign); Ns = 2; break;
-        case 2: stockhamPass<T, 4>(buf, N, Ns, sn, bt, bFast, tw, sign); Ns = 4; break;
-        case 3: stockhamPass<T, 8>(buf, N, Ns, sn, bt, bFast, tw, sign); Ns = 8; break;
+        case 1: stockhamPass<T, 2, SIGN>(buf, log2N, l2s, bt, tw); l2s = 1; break;
+        case 2: stockhamPass<T, 4, SIGN>(buf, log2N, l2s, bt, tw); l2s = 2; break;
+        case 3: stockhamPass<T, 8, SIGN>(buf, log2N, l2s, bt, tw); l2s = 3; break;
         default: break;
     }
-    for (; Ns < N; Ns *= 16) stockhamPass<T, 16>(buf, N, Ns, sn, bt, bFast, tw, sign);
-}
-
-// Single-level batch: transform b at offset b*sb.
-template <typename T>
-__device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, int sn, int sb, int nb, bool bFast,
-                                       const cx<T>* __restrict__ tw, int sign) {
-    fftLds(buf, N, log2N, sn, Batch{nb, nb, sb, 0}, bFast, tw, sign);
+    for (; l2s < log2N; l2s += 4) stockhamPass<T, 16, SIGN>(buf, log2N, l2s, bt, tw);
 }
 
 // Copies the N-entry twiddle table into LDS (block-wide; caller synchronizes).

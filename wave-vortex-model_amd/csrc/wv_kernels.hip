@@ -203,7 +203,25 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int r = 0; r < ZT; ++r) acc[q][r] = MF::zero();
+        // A operands are prefetched one step ahead (latency of the L2 loads
+        // overlaps the channel arithmetic and MFMAs of the current step)
+        T aC[ZT], aS[ZT], nC[ZT], nS[ZT];
+#pragma unroll
+        for (int r = 0; r < ZT; ++r) {
+            const int zt = zt0 + r < ztiles ? zt0 + r : ztiles - 1;
+            const size_t a_off = static_cast<size_t>(zt) * nsteps * 64 + lane;
+            aC[r] = synC[a_off];
+            aS[r] = synS[a_off];
+        }
         for (int s = 0; s < nsteps; ++s) {
+            const int sn = s + 1 < nsteps ? s + 1 : s;
+#pragma unroll
+            for (int r = 0; r < ZT; ++r) {
+                const int zt = zt0 + r < ztiles ? zt0 + r : ztiles - 1;
+                const size_t a_off = (static_cast<size_t>(zt) * nsteps + sn) * 64 + lane;
+                nC[r] = synC[a_off];
+                nS[r] = synS[a_off];
+            }
             const int j = 4 * s + kq;
             const bool endpoint = (j == 0 || j + 1 == Nz);
             const T mjj = j < Nj ? mj[j] : T(0);
@@ -228,16 +246,13 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
             }
 #pragma unroll
             for (int r = 0; r < ZT; ++r) {
-                const int zt = zt0 + r;
-                if (zt < ztiles) {
-                    // operands pre-arranged as [ztile][step][lane]: one coalesced load per wave
-                    const size_t a_off = (static_cast<size_t>(zt) * nsteps + s) * 64 + lane;
-                    const T aC = synC[a_off];
-                    const T aS = synS[a_off];
+                if (zt0 + r < ztiles) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        if (q < G.nch) acc[q][r] = MF::run(G.fam[q] ? aS : aC, b[q], acc[q][r]);
+                        if (q < G.nch) acc[q][r] = MF::run(G.fam[q] ? aS[r] : aC[r], b[q], acc[q][r]);
                 }
+                aC[r] = nC[r];
+                aS[r] = nS[r];
             }
         }
         if (!colValid) continue;
@@ -268,8 +283,9 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
 // needed (skipOut).
 template <typename T>
 __global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int Ny,
-                                              int log2Ny, int nkxP, int KT, int Nz, const cx<T>* __restrict__ twY,
+                                              int log2Ny, int nkxP, int log2KT, int Nz, const cx<T>* __restrict__ twY,
                                               int sign, const int* __restrict__ rowLimit, int skipIn, int skipOut) {
+    const int KT = 1 << log2KT;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
     cx<T>* tile = tw + Ny;
@@ -278,17 +294,43 @@ __global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T
     const size_t plane = (static_cast<size_t>(blockIdx.z) * Nz + blockIdx.y) * static_cast<size_t>(Ny) * nkxP;
     const int nel = Ny * KT;
     loadTwiddles(tw, twY, Ny);
-    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
-        const int kk = e % KT, iy = e / KT;
-        const bool zero = skipIn && iy > lim && iy < Ny - lim;
-        tile[e] = zero ? cx<T>{T(0), T(0)} : in[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk];
+    {
+        // all of a thread's loads are issued before its LDS stores
+        constexpr int kMaxPerThread = 16;  // nel <= 16 * blockDim (checked at setup)
+        cx<T> v[kMaxPerThread];
+#pragma unroll
+        for (int i = 0; i < kMaxPerThread; ++i) {
+            const int e = threadIdx.x + i * blockDim.x;
+            const int kk = e & (KT - 1), iy = e >> log2KT;
+            const bool zero = e >= nel || (skipIn && iy > lim && iy < Ny - lim);
+            v[i] = zero ? cx<T>{T(0), T(0)} : in[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk];
+        }
+#pragma unroll
+        for (int i = 0; i < kMaxPerThread; ++i) {
+            const int e = threadIdx.x + i * blockDim.x;
+            if (e < nel) tile[e] = v[i];
+        }
     }
     __syncthreads();
-    fftLds(tile, Ny, log2Ny, KT, 1, KT, true, tw, sign);
+    if (sign > 0)
+        fftLds<T, +1>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
+    else
+        fftLds<T, -1>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
     for (int e = threadIdx.x; e < nel; e += blockDim.x) {
-        const int kk = e % KT, iy = e / KT;
+        const int kk = e & (KT - 1), iy = e >> log2KT;
         if (skipOut && iy > lim && iy < Ny - lim) continue;
         out[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk] = tile[e];
+    }
+}
+
+// q = e / N, r = e % N; shifts when N = 2^log2N (log2N >= 0).
+__device__ __forceinline__ void divmodN(int e, int N, int log2N, int& q, int& r) {
+    if (log2N >= 0) {
+        q = e >> log2N;
+        r = e & (N - 1);
+    } else {
+        q = e / N;
+        r = e - q * N;
     }
 }
 
@@ -304,6 +346,23 @@ __device__ __forceinline__ cx<T> hermitianX(const cx<T>* __restrict__ G, size_t 
     return {T(0), T(0)};
 }
 
+// LDS layout of the x transforms: transform stride NxP = the padded length
+// rounded to 8 (f32) or 4 (f64) complex entries modulo one bank sweep, row
+// stride = PAIRS*NxP rounded to half a sweep, so that the four 16-thread
+// transforms a wave works on start in different bank groups.
+template <typename T>
+__host__ __device__ constexpr int bankSweep() { return sizeof(T) == 4 ? 32 : 16; }  // complex entries per 256 B
+template <typename T>
+__host__ __device__ constexpr int padTo(int base, int want) {
+    return base + (((want - base % bankSweep<T>()) % bankSweep<T>()) + bankSweep<T>()) % bankSweep<T>();
+}
+template <typename T>
+__host__ __device__ constexpr int xfusedStride(int Nx) { return padTo<T>(ldsLength(Nx), bankSweep<T>() / 4); }
+template <typename T>
+__host__ __device__ constexpr int xfusedRowStride(int Nx, int pairs) {
+    return padTo<T>(pairs * xfusedStride<T>(Nx), bankSweep<T>() / 2);
+}
+
 // Per (y, z) row: c2r of the NCH channels along x (two real transforms per
 // complex FFT), the NT advection products and r2c of the products.
 template <typename T, int NCH, int NT, int RB>
@@ -312,31 +371,51 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
                                                 const cx<T>* __restrict__ twX) {
     constexpr int PAIRS = (NCH + 1) / 2;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int NxP = ldsLength(Nx);
+    const int NxP = xfusedStride<T>(Nx);
+    const int RS = xfusedRowStride<T>(Nx, PAIRS);
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
-    cx<T>* buf = tw + Nx;  // [RB][PAIRS][NxP]
+    cx<T>* buf = tw + Nx;  // [RB][RS], row rr holds PAIRS transforms of stride NxP
     const int nrows = Ny * Nz;
     const int row0 = blockIdx.x * RB;  // row = z*Ny + y
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     loadTwiddles(tw, twX, Nx);
-    for (int e = threadIdx.x; e < RB * PAIRS * Nx; e += blockDim.x) {
-        const int k = e % Nx, rp = e / Nx;  // rp = rr*PAIRS + p
-        const int p = rp % PAIRS, row = row0 + rp / PAIRS;
-        cx<T> val{T(0), T(0)};
-        if (row < nrows) {
-            const size_t rowBase = static_cast<size_t>(row) * nkxP;
-            const cx<T> A = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
-            const cx<T> B = (2 * p + 1 < NCH) ? hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx) : cx<T>{T(0), T(0)};
-            val = {A.re - B.im, A.im + B.re};
+    // each (row, pair, k) entry is produced by one thread; a thread issues the
+    // global loads of kChunk entries before storing them to LDS
+    constexpr int kChunk = 4;
+    const int nel = RB * PAIRS * Nx;
+    for (int e0 = 0; e0 < nel; e0 += kChunk * blockDim.x) {
+        cx<T> va[kChunk], vb[kChunk];
+#pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+            const int e = e0 + threadIdx.x + i * blockDim.x;
+            int rp, k;
+            divmodN(e, Nx, log2Nx, rp, k);
+            const int p = rp % PAIRS, row = row0 + rp / PAIRS;
+            va[i] = vb[i] = cx<T>{T(0), T(0)};
+            if (e < nel && row < nrows) {
+                const size_t rowBase = static_cast<size_t>(row) * nkxP;
+                va[i] = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
+                if (2 * p + 1 < NCH) vb[i] = hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx);
+            }
         }
-        buf[rp * NxP + ldsIndex(k, 1)] = val;
+#pragma unroll
+        for (int i = 0; i < kChunk; ++i) {
+            const int e = e0 + threadIdx.x + i * blockDim.x;
+            if (e < nel) {
+                int rp, k;
+                divmodN(e, Nx, log2Nx, rp, k);
+                buf[(rp / PAIRS) * RS + (rp % PAIRS) * NxP + ldsIndex(k, 1)] = {va[i].re - vb[i].im,
+                                                                                va[i].im + vb[i].re};
+            }
+        }
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, NxP, RB * PAIRS, false, tw, +1);
+    fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<PAIRS>{RB * PAIRS, NxP, RS}, tw);
     for (int e = threadIdx.x; e < RB * Nx; e += blockDim.x) {
-        const int x = e % Nx, rr = e / Nx;
+        int rr, x;
+        divmodN(e, Nx, log2Nx, rr, x);
         const int xl = ldsIndex(x, 1);
-        cx<T>* rb = buf + rr * PAIRS * NxP;
+        cx<T>* rb = buf + rr * RS;
         T ch[2 * PAIRS];
 #pragma unroll
         for (int p = 0; p < PAIRS; ++p) {
@@ -353,13 +432,15 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
     }
     __syncthreads();
     // forward transforms: pairs 0 and 1 of every row
-    fftLds(buf, Nx, log2Nx, 1, Batch{2 * RB, 2, NxP, PAIRS * NxP}, false, tw, -1);
-    for (int e = threadIdx.x; e < RB * 2 * nkx; e += blockDim.x) {
-        const int k = e % nkx, rq = e / nkx;
-        const int q = rq % 2, rr = rq / 2;
+    fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
+    // wave w unpacks (row, pair) rq = w, w + nwaves, ...; its lanes walk k
+    const int nwaves = blockDim.x >> 6;
+    for (int rq = threadIdx.x >> 6; rq < RB * 2; rq += nwaves)
+    for (int k = threadIdx.x & 63; k < nkx; k += 64) {
+        const int q = rq & 1, rr = rq >> 1;
         const int row = row0 + rr;
         if (row >= nrows) continue;
-        const cx<T>* rb = buf + (rr * PAIRS + q) * NxP;
+        const cx<T>* rb = buf + rr * RS + q * NxP;
         const cx<T> Za = rb[ldsIndex(k, 1)];
         const cx<T> Zb = rb[ldsIndex(k == 0 ? 0 : Nx - k, 1)];
         const T h = T(0.5);
@@ -388,7 +469,7 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
         buf[p * NxP + ldsIndex(k, 1)] = {A.re - B.im, A.im + B.re};
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, NxP, pairs, false, twX, +1);
+    fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<1>{pairs, 0, NxP}, twX);
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
     for (int e = threadIdx.x; e < nch * Nx; e += blockDim.x) {
         const int c = e / Nx, x = e % Nx;
@@ -441,9 +522,13 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
         const int target = (targetCode >> (4 * t)) & 0xF;
         const T* A = target < 2 ? anaC : anaS;
         typename MF::acc a4 = MF::zero();
+        const T* Aj = A + static_cast<size_t>(jt) * nsteps * 64 + lane;
+        T a = Aj[0];
         for (int s = 0; s < nsteps; ++s) {
+            const T an = Aj[static_cast<size_t>(s + 1 < nsteps ? s + 1 : s) * 64];
             const cx<T> q = qs[(4 * s + kq) * 8 + (n >> 1)];
-            a4 = MF::run(A[(static_cast<size_t>(jt) * nsteps + s) * 64 + lane], part ? q.im : q.re, a4);
+            a4 = MF::run(a, part ? q.im : q.re, a4);
+            a = an;
         }
         acc[t] = a4;
     }
@@ -569,7 +654,7 @@ __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<
         buf[p * NxP + ldsIndex(x, 1)] = {a, b};
     }
     __syncthreads();
-    fftLds(buf, Nx, log2Nx, 1, NxP, pairs, false, twX, -1);
+    fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<1>{pairs, 0, NxP}, twX);
     const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     for (int e = threadIdx.x; e < pairs * nkx; e += blockDim.x) {
@@ -1017,7 +1102,7 @@ struct KernelImpl {
         timed(name, bytes, s, [&] {
             hipLaunchKernelGGL(k_yfft<T>, dim3(static_cast<unsigned>(d.nkxP / KT), static_cast<unsigned>(d.c.Nz), nch),
                                dim3(256), lds, s, in.as<cx<T>>(), out.as<cx<T>>(), static_cast<int>(d.c.Ny),
-                               log2Exact(d.c.Ny), static_cast<int>(d.nkxP), KT, static_cast<int>(d.c.Nz),
+                               log2Exact(d.c.Ny), static_cast<int>(d.nkxP), log2Exact(KT), static_cast<int>(d.c.Nz),
                                twY.as<cx<T>>(), sign, rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0);
         });
     }
@@ -1025,7 +1110,7 @@ struct KernelImpl {
     template <typename T, int RB>
     size_t xfusedLds() const {
         const int pairs = (NCH + 1) / 2;
-        return (static_cast<size_t>(d.c.Nx) + static_cast<size_t>(RB) * pairs * ldsLength(static_cast<int>(d.c.Nx))) *
+        return (static_cast<size_t>(d.c.Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(static_cast<int>(d.c.Nx), pairs)) *
                sizeof(cx<T>);
     }
 
@@ -1035,7 +1120,8 @@ struct KernelImpl {
         const int Nx = static_cast<int>(d.c.Nx), l2 = log2Exact(d.c.Nx), Ny = static_cast<int>(d.c.Ny),
                   Nz = static_cast<int>(d.c.Nz), nkx = static_cast<int>(d.nkx), nkxP = static_cast<int>(d.nkxP);
         const int nrows = Ny * Nz;
-        const bool two = xfusedLds<T, 2>() <= maxLdsPerBlock;
+        // two rows per block while four blocks still fit a CU's 160 KB of LDS
+        const bool two = xfusedLds<T, 2>() <= std::min<size_t>(maxLdsPerBlock, 40 * 1024);
         const size_t lds = two ? xfusedLds<T, 2>() : xfusedLds<T, 1>();
         const dim3 grid(static_cast<unsigned>(two ? (nrows + 1) / 2 : nrows));
         timed("xfused", bytes, s, [&] {
