@@ -26,6 +26,20 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "RK4 steps/s at 256×256×129: mixed vs FP64 GPU at ≤1e-6 rel energy error"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_rk4_256x256x129_mixed.md")
+# stage name -> kernel of the mixed-precision RHS
+STAGE_KERNEL = {"fields": "k_fields<float", "synth": "k_synth<float", "xfused": "k_xfused<float",
+                "proj": "k_proj<float"}
+
+
+def step_model_bytes(Nx, Ny, Nz, M, b_spec, b_real):
+    """Reference dataflow bytes of one nonhydrostatic RK4 step: 4 fluxes of
+    72h + 47r + 71.5m plus 25 unfused passes over the 3-array state."""
+    h = (Nx // 2 + 1) * Ny * Nz * b_spec
+    r = Nx * Ny * Nz * b_real
+    m = 16 * M
+    return 4 * (72 * h + 47 * r + 71.5 * m) + 25 * 3 * m
 
 
 def workload_config():
@@ -72,16 +86,17 @@ def stage_profile(k, state, dt, steps=2):
     return rec
 
 
-def cpu_baseline(cfg, state, dt, threads):
-    """Oracle FP64 RK4 step at the full configuration on the host cores."""
+def cpu_baseline(cfg, state, dt, threads, steps=4):
+    """Oracle FP64 RK4 steps at the full configuration on the host cores."""
     from oracle import Oracle
     Oracle.set_threads(threads)
     o = Oracle(cfg)
     t0 = time.perf_counter()
-    o.rk4(*state, t=0.0, dt=dt, nsteps=1)
+    o.rk4(*state, t=0.0, dt=dt, nsteps=steps)
     sec = time.perf_counter() - t0
-    return {"value": 1.0 / sec, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"1 FP64 RK4 step (4 fluxes) of the oracle at 256x256x129, {sec:.1f} s"}
+    return {"value": round(steps / sec, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} FP64 RK4 steps (4 fluxes each) of the oracle at 256x256x129 from the benchmark "
+                      f"state, {sec:.1f} s"}
 
 
 def main():
@@ -134,9 +149,19 @@ def main():
     dom = max(rec, key=lambda r: r["total_ms"])
     avg_ms = dom["total_ms"] / max(dom["launches"], 1)
     achieved = dom["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+    from wvamd.rocprof_summary import hbm_traffic
+    measured = hbm_traffic(TRAFFIC_SUMMARY, STAGE_KERNEL.get(dom["name"], "?"))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
-                "avg_launch_ms": round(avg_ms, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(measured["traffic_bytes"]) if measured else None,
+                "kernel": dom["name"], "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes_per_launch": round(dom["bytes_per_launch"])}
+    b32 = step_model_bytes(cfg["Nx"], cfg["Ny"], cfg["Nz"], k32.Nkl * k32.Nj, 8, 4)
+    b64 = step_model_bytes(cfg["Nx"], cfg["Ny"], cfg["Nz"], k64.Nkl * k64.Nj, 16, 8)
+    step_model = {"mixed_bytes_per_step": b32, "mixed_GBs": round(b32 * sps32 / 1e9, 1),
+                  "mixed_frac": round(b32 * sps32 / 1e9 / HBM_PEAK_GBS, 4),
+                  "fp64_bytes_per_step": b64, "fp64_GBs": round(b64 * sps64 / 1e9, 1),
+                  "fp64_frac": round(b64 * sps64 / 1e9 / HBM_PEAK_GBS, 4)}
     stages = {r["name"]: round(r["total_ms"] / max(r["launches"], 1), 4) for r in rec}
 
     cpu = None
@@ -153,7 +178,8 @@ def main():
                    "Lx_m": 200e3, "Lz_m": 1300.0, "Nj": 85, "Nkl": k64.Nkl, "dt_s": dt},
         "fp64_steps_per_s": round(sps64, 3), "fp64_ms_per_step": round(1e3 * sec64 / args.steps, 4),
         "mixed_over_fp64": round(sps32 / sps64, 3), "energy_rel_err_mixed_vs_fp64": energy_err,
-        "roofline": roofline, "stage_ms_per_launch_mixed": stages, "cpu_baseline": cpu,
+        "roofline": roofline, "reference_dataflow_roofline": step_model,
+        "stage_ms_per_launch_mixed": stages, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -281,12 +281,32 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile) {
             anaS[j * Za + iz] = (j >= 1 && iz >= 1 && iz + 1 < Nz) ? (2.0 * sinPiRatio(j * iz, L)) / static_cast<double>(L) : 0.0;
         }
     }
+    // Power-of-two n: the Stockham pass twiddles of the LDS FFT, pass (Ns, R)
+    // entry (r, k) at r*Ns + k - 1 holding exp(-2 pi i r k / (Ns R)), so that
+    // consecutive k (consecutive threads) read consecutive entries; the pass
+    // order is the FFT's (radix 2^(log2 n mod 4) first, then radix 16).
+    // Other n: the plain table exp(-2 pi i q / n) of the direct DFT.
     auto twiddles = [](size_t n, std::vector<double>& out) {
-        out.resize(2 * n);
-        for (size_t q = 0; q < n; ++q) {
-            out[2 * q] = cosPiRatio(2 * q, n);
-            out[2 * q + 1] = -sinPiRatio(2 * q, n);
+        out.assign(2 * n, 0.0);
+        auto put = [&](size_t idx, size_t q) {
+            out[2 * idx] = cosPiRatio(2 * q, n);
+            out[2 * idx + 1] = -sinPiRatio(2 * q, n);
+        };
+        if (n & (n - 1)) {
+            for (size_t q = 0; q < n; ++q) put(q, q);
+            return;
         }
+        int l2 = 0;
+        while ((size_t{1} << l2) < n) ++l2;
+        size_t Ns = 1;
+        auto pass = [&](size_t R) {
+            for (size_t r = 1; r < R; ++r)
+                for (size_t k = 0; k < Ns; ++k)
+                    if (r * Ns + k >= 1) put(r * Ns + k - 1, (r * k * (n / (Ns * R))) % n);
+            Ns *= R;
+        };
+        if (l2 % 4) pass(size_t{1} << (l2 % 4));
+        while (Ns < n) pass(16);
     };
     twiddles(c.Nx, twX);
     twiddles(c.Ny, twY);

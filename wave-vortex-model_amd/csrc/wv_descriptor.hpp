@@ -58,7 +58,7 @@ struct Descriptor {
     //             DCT-I / DST-I followed by the forward normalization.
     size_t Zs = 0, Js = 0, Ja = 0, Za = 0;
     std::vector<double> synC, synS, anaC, anaS;
-    std::vector<double> twX, twY;  // exp(-2*pi*i*q/N), interleaved
+    std::vector<double> twX, twY;  // FFT twiddles (Stockham pass order for powers of two), interleaved
 
     // Returns an empty string on success, otherwise the reference's message.
     std::string build(const wv_config& config, size_t kxTile);

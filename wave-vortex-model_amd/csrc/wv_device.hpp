@@ -63,7 +63,10 @@ struct Mfma<double> {
 //     (b % NB1)*sb + (b / NB1)*sb2; consecutive threads walk one transform.
 //   BatchCols: nb = 2^log2nb interleaved transforms, element n of transform b
 //     at n*nb + b; consecutive threads walk the transforms.
-// X_k = sum_n x_n exp(SIGN*2*pi*i*n*k/N); tw[q] = exp(-2*pi*i*q/N).
+// X_k = sum_n x_n exp(SIGN*2*pi*i*n*k/N).  Twiddles: for power-of-two N the
+// pass table, entry r*Ns + k - 1 = exp(-2*pi*i*r*k/(Ns*R)) for the pass that
+// combines sub-transforms of length Ns with radix R; otherwise
+// tw[q] = exp(-2*pi*i*q/N).
 // Power-of-two N runs Stockham autosort passes of radix 16 (then 8, 4 or 2
 // for the remaining factor), each butterfly an in-register DFT, with all index
 // arithmetic in shifts and masks; other N use a direct DFT.  A pass holds at
@@ -169,7 +172,6 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
     const int log2NR = log2N - LR;
     const int NR = 1 << log2NR;
     const int items = batchCount(bt) << log2NR;
-    const int tshift = log2NR - log2Ns;  // twiddle stride N/(Ns R)
     const int kmask = (1 << log2Ns) - 1;
     cx<T> v[IPT][R];
 #pragma unroll
@@ -184,7 +186,7 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
             for (int r = 0; r < R; ++r) {
                 cx<T> x = buf[bt.at(jj + r * NR) + boff];
                 if (r > 0 && k > 0) {
-                    cx<T> w = tw[(r * k) << tshift];
+                    cx<T> w = tw[(r << log2Ns) + k - 1];
                     if (SIGN > 0) w.im = -w.im;
                     x = cmul(x, w);
                 }

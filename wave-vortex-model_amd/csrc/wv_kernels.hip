@@ -379,35 +379,50 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
     const int row0 = blockIdx.x * RB;  // row = z*Ny + y
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     loadTwiddles(tw, twX, Nx);
-    // each (row, pair, k) entry is produced by one thread; a thread issues the
-    // global loads of kChunk entries before storing them to LDS
-    constexpr int kChunk = 4;
-    const int nel = RB * PAIRS * Nx;
-    for (int e0 = 0; e0 < nel; e0 += kChunk * blockDim.x) {
-        cx<T> va[kChunk], vb[kChunk];
+    // Transform t = rr*PAIRS + p packs channels 2p (real part) and 2p+1
+    // (imaginary part) of row rr.  Wave w fills transforms w, w + 4, ...: each
+    // retained entry k < nkx is read once and written at k and, conjugated, at
+    // Nx - k; the band between is zeroed.  A wave issues the loads of all its
+    // transforms (128 entries each) before storing them.
+    static_assert(RB * PAIRS <= 16, "at most four transforms per wave");
+    constexpr int kTPW = (RB * PAIRS + 3) / 4;  // transforms per wave
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int kc = 0; kc < nkx; kc += 128) {
+        cx<T> va[kTPW][2], vb[kTPW][2];
 #pragma unroll
-        for (int i = 0; i < kChunk; ++i) {
-            const int e = e0 + threadIdx.x + i * blockDim.x;
-            int rp, k;
-            divmodN(e, Nx, log2Nx, rp, k);
-            const int p = rp % PAIRS, row = row0 + rp / PAIRS;
-            va[i] = vb[i] = cx<T>{T(0), T(0)};
-            if (e < nel && row < nrows) {
-                const size_t rowBase = static_cast<size_t>(row) * nkxP;
-                va[i] = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
-                if (2 * p + 1 < NCH) vb[i] = hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx);
+        for (int i = 0; i < kTPW; ++i) {
+            const int t = wave + 4 * i;
+            const int rr = t / PAIRS, p = t % PAIRS;
+            const int row = row0 + rr;
+            const bool live = t < RB * PAIRS && row < nrows;
+            const cx<T>* A = G + (2 * p) * chStride + static_cast<size_t>(row) * nkxP;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = kc + lane + 64 * h;
+                const bool in = live && k < nkx;
+                va[i][h] = in ? A[k] : cx<T>{T(0), T(0)};
+                vb[i][h] = (in && 2 * p + 1 < NCH) ? A[chStride + k] : cx<T>{T(0), T(0)};
             }
         }
 #pragma unroll
-        for (int i = 0; i < kChunk; ++i) {
-            const int e = e0 + threadIdx.x + i * blockDim.x;
-            if (e < nel) {
-                int rp, k;
-                divmodN(e, Nx, log2Nx, rp, k);
-                buf[(rp / PAIRS) * RS + (rp % PAIRS) * NxP + ldsIndex(k, 1)] = {va[i].re - vb[i].im,
-                                                                                va[i].im + vb[i].re};
+        for (int i = 0; i < kTPW; ++i) {
+            const int t = wave + 4 * i;
+            if (t >= RB * PAIRS) continue;
+            cx<T>* z = buf + (t / PAIRS) * RS + (t % PAIRS) * NxP;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = kc + lane + 64 * h;
+                if (k >= nkx) continue;
+                cx<T> a = va[i][h], b = vb[i][h];
+                if (k == 0) a.im = b.im = T(0);
+                z[ldsIndex(k, 1)] = {a.re - b.im, a.im + b.re};
+                if (k > 0) z[ldsIndex(Nx - k, 1)] = {a.re + b.im, b.re - a.im};
             }
         }
+    }
+    for (int t = wave; t < RB * PAIRS; t += 4) {
+        cx<T>* z = buf + (t / PAIRS) * RS + (t % PAIRS) * NxP;
+        for (int k = nkx + lane; k <= Nx - nkx; k += 64) z[ldsIndex(k, 1)] = cx<T>{T(0), T(0)};
     }
     __syncthreads();
     fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<PAIRS>{RB * PAIRS, NxP, RS}, tw);
