@@ -254,6 +254,8 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile) {
         if (!(columns[ic].flags & kColCompletion)) wvColumn[static_cast<size_t>(columns[ic].wv)] = static_cast<int32_t>(ic);
     }
     nkxP = roundUp(nkx, kxTile);
+    kxv.resize(nkx);
+    for (size_t k = 0; k < nkx; ++k) kxv[k] = 2.0 * kPi * (static_cast<double>(k) / c.Lx);  // as the modes' k
 
     // Dense vertical operators.  Synthesis applies the unnormalized DCT-I / DST-I
     // of FFTW REDFT00 / RODFT00 (WVReferenceFFTEngine.cpp:96-101) to a spectrum

@@ -38,6 +38,7 @@ struct Descriptor {
     size_t NxHalf = 0, Nkl = 0, M = 0, R = 0;
     size_t nkx = 0;   // 1 + largest kx present
     size_t nkxP = 0;  // nkx padded to the y-FFT tile width
+    std::vector<double> kxv;  // x wavenumber of half-spectrum column kx < nkx
     double f = 0;
 
     std::vector<FourierMode> modes;

@@ -53,19 +53,23 @@ struct ColumnDev {
     double k, l, Kh, cosA, sinA;
 };
 
-// One synthesis group: up to four output channels derived from the evolved
-// spectra of the fields in `field` (0=u, 1=v, 2=w, 3=eta) by `op`
-// (0 = the field itself, 1 = d/dx, 2 = d/dy, 3 = d/dz), each followed by the
-// DCT-I (fam 0) or DST-I (fam 1).  Every channel carries the factor 1/2 of
-// the reference assembly and is zeroed at the vertical endpoints when its
-// output family is the sine family (...Kernel.cpp:240,286-288 and
-// WVCoefficientFormulas.hpp:42-58).
+// Vertical synthesis plan.  A channel is the evolved spectrum of field
+// `field` (0=u, 1=v, 2=w, 3=eta) or, with dz set, its z derivative (factor
+// -m_j into the sine family, +m_j into the cosine family), times the 1/2 of
+// the reference assembly (...Kernel.cpp:240,286-288, WVCoefficientFormulas.hpp:
+// 42-58), synthesized with the DCT-I (fam 0) or DST-I (fam 1) of its group.
+// The result goes to up to three half-spectrum channels: outV (the value),
+// outX (i k times it) and outY (i l times it), k and l the wavenumbers of the
+// storage row; -1 = not written.  Horizontal derivatives commute with the
+// vertical transform, so they cost a scaling instead of a synthesis.  The
+// sine-family endpoint zeroing is carried by the DST-I matrix, whose j = 0
+// column is zero.
+struct SynthChannel {
+    int field, dz, outV, outX, outY;
+};
 struct SynthGroup {
-    int nch;
-    int field[4];
-    int op[4];
-    int fam[4];
-    int outch[4];
+    int fam, nch;
+    SynthChannel ch[4];
 };
 constexpr int kMaxGroups = 8;
 struct SynthPlan {
@@ -105,6 +109,10 @@ struct Epilogue {
 };
 
 // ----------------------------------------------------------------- kernels -
+
+// Index of the calling wave in its workgroup, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int waveIndex() { return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)); }
+
 
 // Evolved field spectra per coefficient (WVCoefficientFormulas.hpp:15-33).
 template <typename T>
@@ -149,10 +157,10 @@ __global__ __launch_bounds__(256) void k_fgfields(const cx<double>* __restrict__
 }
 
 // Spectral assembly + vertical synthesis (MFMA).  One workgroup owns 8
-// consecutive columns: it stages their evolved field spectra (all j < Nj) in
-// LDS once, then its waves loop over (group, z-chunk) tasks, each computing ZT
-// tiles of 16 z levels for the group's channels:
-//   H[ch][z][iy][kx] = sum_j syn_fam(ch)[z][j] * value_ch(j, column).
+// consecutive columns: it stages their evolved field spectra (all j < Nj) and
+// m_j in LDS once, then its waves loop over (group, z-chunk) tasks, each
+// computing ZT tiles of 16 z levels for the group's channels:
+//   H[out][z][iy][kx] = sum_j syn_fam[z][j] * value_ch(j, column).
 // Complex values are two real GEMM columns; lane l owns GEMM column l%16
 // (column (l%16)/2, real or imaginary part) and k-row l/16.
 template <typename T, int ZT>
@@ -163,10 +171,11 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
                                                cx<T>* __restrict__ H) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* fv = reinterpret_cast<cx<T>*>(smem);  // [4 fields][Js][8 columns]
+    cx<T>* fv = reinterpret_cast<cx<T>*>(smem);      // [4 fields][Js][8 columns]
+    T* mjs = reinterpret_cast<T*>(fv + 4 * Js * 8);  // [Js]
     __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
-    const int waveInBlock = threadIdx.x >> 6;
+    const int waveInBlock = waveIndex();
     const int nwaves = blockDim.x >> 6;
     const int cg = blockIdx.x;
     const int ztiles = Zs / 16;
@@ -187,89 +196,93 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
         if (wv >= 0 && j < Nj) v = FV[static_cast<size_t>(f) * M + j + static_cast<size_t>(Nj) * wv];
         fv[(f * Js + j) * 8 + c] = v;
     }
+    for (int j = threadIdx.x; j < Js; j += blockDim.x) mjs[j] = j < Nj ? mj[j] : T(0);
     __syncthreads();
     const int n = lane & 15, kq = lane >> 4;
     const int c = n >> 1, part = n & 1;
     const ColumnDev cd = scol[c];
     const bool colValid = cd.wv >= 0;
-    const T kx = T(cd.k), ly = T(cd.l);
+    // Stored value = conj(mode value) on conjugated rows, real on
+    // self-conjugate rows; the storage row's wavenumbers are the mode's times
+    // the same sign.
+    const T rowSign = (cd.flags & kColConj) ? T(-1) : T(1);
+    const T partWeight = part ? ((cd.flags & kColSelfConj) ? T(0) : rowSign) : T(1);
+    const T ks = rowSign * T(cd.k), ls = rowSign * T(cd.l);
+    // lane reads the real (part 0) or imaginary (part 1) word of fv[.][j][c]
+    const T* fvw = reinterpret_cast<const T*>(fv) + 2 * c + part;
     for (int task = waveInBlock; task < plan.ngroups * nzc; task += nwaves) {
         const int gi = task % plan.ngroups;
         const int zc = task / plan.ngroups;
-        const SynthGroup& G = plan.g[gi];
+        const int fam = plan.g[gi].fam, nch = plan.g[gi].nch;
+        int fieldOff[4];
+        T coefV[4], coefZ[4];
+        // value channels: 1/2; z-derivative channels: -/+ m_j / 2
+        const T zsign = fam ? T(-0.5) : T(0.5);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool live = q < nch;
+            const bool dz = live && plan.g[gi].ch[q].dz;
+            fieldOff[q] = live ? plan.g[gi].ch[q].field * Js * 16 : 0;
+            coefV[q] = live && !dz ? T(0.5) * partWeight : T(0);
+            coefZ[q] = dz ? zsign * partWeight : T(0);
+        }
+        const T* A = fam ? synS : synC;
         const int zt0 = zc * ZT;
         typename MF::acc acc[4][ZT];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int r = 0; r < ZT; ++r) acc[q][r] = MF::zero();
-        // A operands are prefetched one step ahead (latency of the L2 loads
-        // overlaps the channel arithmetic and MFMAs of the current step)
-        T aC[ZT], aS[ZT], nC[ZT], nS[ZT];
+        // A operands are prefetched one step ahead
+        T a[ZT], an[ZT];
+        const T* Ar[ZT];
 #pragma unroll
         for (int r = 0; r < ZT; ++r) {
             const int zt = zt0 + r < ztiles ? zt0 + r : ztiles - 1;
-            const size_t a_off = static_cast<size_t>(zt) * nsteps * 64 + lane;
-            aC[r] = synC[a_off];
-            aS[r] = synS[a_off];
+            Ar[r] = A + static_cast<size_t>(zt) * nsteps * 64 + lane;
+            a[r] = Ar[r][0];
         }
         for (int s = 0; s < nsteps; ++s) {
-            const int sn = s + 1 < nsteps ? s + 1 : s;
+            const int sn = (s + 1 < nsteps ? s + 1 : s) * 64;
 #pragma unroll
-            for (int r = 0; r < ZT; ++r) {
-                const int zt = zt0 + r < ztiles ? zt0 + r : ztiles - 1;
-                const size_t a_off = (static_cast<size_t>(zt) * nsteps + sn) * 64 + lane;
-                nC[r] = synC[a_off];
-                nS[r] = synS[a_off];
-            }
+            for (int r = 0; r < ZT; ++r) an[r] = Ar[r][sn];
             const int j = 4 * s + kq;
-            const bool endpoint = (j == 0 || j + 1 == Nz);
-            const T mjj = j < Nj ? mj[j] : T(0);
-            T b[4] = {T(0), T(0), T(0), T(0)};
+            const T mjj = mjs[j];
+            const T* fj = fvw + j * 16;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (q < G.nch) {
-                    const cx<T> f = fv[(G.field[q] * Js + j) * 8 + c];
-                    const bool sineOut = G.fam[q] == 1;
-                    cx<T> val;
-                    switch (G.op[q]) {
-                        case 0: val = f; break;
-                        case 1: val = cmuli(f, kx); break;
-                        case 2: val = cmuli(f, ly); break;
-                        default: val = cscale(f, sineOut ? -mjj : mjj); break;
-                    }
-                    val = (sineOut && endpoint) ? cx<T>{T(0), T(0)} : cscale(val, T(0.5));
-                    if (cd.flags & kColConj) val.im = -val.im;
-                    if (cd.flags & kColSelfConj) val.im = T(0);
-                    b[q] = part ? val.im : val.re;
+                if (q < nch) {
+                    const T b = fj[fieldOff[q]] * (coefV[q] + coefZ[q] * mjj);
+#pragma unroll
+                    for (int r = 0; r < ZT; ++r)
+                        if (zt0 + r < ztiles) acc[q][r] = MF::run(a[r], b, acc[q][r]);
                 }
             }
 #pragma unroll
-            for (int r = 0; r < ZT; ++r) {
-                if (zt0 + r < ztiles) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        if (q < G.nch) acc[q][r] = MF::run(G.fam[q] ? aS[r] : aC[r], b[q], acc[q][r]);
-                }
-                aC[r] = nC[r];
-                aS[r] = nS[r];
-            }
+            for (int r = 0; r < ZT; ++r) a[r] = an[r];
         }
-        if (!colValid) continue;
+        // outputs: the value and i k / i l times it (the partner lane n^1
+        // holds the other part of the same complex entry)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            if (q >= G.nch) continue;
+            if (q >= nch) continue;
+            const int oV = plan.g[gi].ch[q].outV, oX = plan.g[gi].ch[q].outX, oY = plan.g[gi].ch[q].outY;
 #pragma unroll
             for (int r = 0; r < ZT; ++r) {
                 const int zt = zt0 + r;
                 if (zt >= ztiles) continue;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
+                    const T v = acc[q][r][i];
+                    const T other = __shfl_xor(v, 1);
+                    const T rot = part ? other : -other;  // part of i * (re + i im)
                     const int z = 16 * zt + MF::row(kq, i);
-                    if (z < Nz) {
-                        T* dst = reinterpret_cast<T*>(
-                            &H[((static_cast<size_t>(G.outch[q]) * Nz + z) * Ny + cd.iy) * nkxP + cd.kx]);
-                        dst[part] = acc[q][r][i];
+                    if (z < Nz && colValid) {
+                        const size_t o = (static_cast<size_t>(z) * Ny + cd.iy) * nkxP + cd.kx;
+                        const size_t chs = static_cast<size_t>(Nz) * Ny * nkxP;
+                        if (oV >= 0) reinterpret_cast<T*>(&H[oV * chs + o])[part] = v;
+                        if (oX >= 0) reinterpret_cast<T*>(&H[oX * chs + o])[part] = ks * rot;
+                        if (oY >= 0) reinterpret_cast<T*>(&H[oY * chs + o])[part] = ls * rot;
                     }
                 }
             }
@@ -363,13 +376,51 @@ __host__ __device__ constexpr int xfusedRowStride(int Nx, int pairs) {
     return padTo<T>(pairs * xfusedStride<T>(Nx), bankSweep<T>() / 2);
 }
 
-// Per (y, z) row: c2r of the NCH channels along x (two real transforms per
-// complex FFT), the NT advection products and r2c of the products.
-template <typename T, int NCH, int NT, int RB>
+// Spatial channels of the flux products, packed two per complex x transform.
+// Pair p takes half-spectrum channel srcA[p] (real part of the transform) and
+// srcB[p] (imaginary part, -1 = none), each multiplied by i k when ik*[p] is
+// set.  A complex FFT's rounding error scales with the larger of its two
+// signals, so each pair holds quantities of one kind and size (the reference
+// transforms every channel on its own).  Products: F_t = -(U d/dx T +
+// V d/dy T + W d/dz T) with the spatial channel numbers (2p + part) of
+// (dx, dy, dz) T in prod[t] and of U, V, W in uvw.
+// Nonhydrostatic (half-spectrum channels u v w eta, dy u..eta, dz u..eta):
+//   (U, V) (dxu, dyu) (dxv, dyv) (dxw, dyw) (dxeta, dyeta) (dzu, dzv) (W, dzw) (dzeta, -)
+// Hydrostatic (u v w eta, dy u v eta, dz u v eta):
+//   (U, V) (dxu, dyu) (dxv, dyv) (dxeta, dyeta) (dzu, dzv) (W, -) (dzeta, -)
+template <int NT>
+struct XMap;
+template <>
+struct XMap<4> {
+    static constexpr int pairs = 8;
+    static constexpr int srcA[8] = {0, 0, 1, 2, 3, 8, 2, 11};
+    static constexpr int ikA[8] = {0, 1, 1, 1, 1, 0, 0, 0};
+    static constexpr int srcB[8] = {1, 4, 5, 6, 7, 9, 10, -1};
+    static constexpr int ikB[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    static constexpr int uvw[3] = {0, 1, 12};
+    static constexpr int prod[4][3] = {{2, 3, 10}, {4, 5, 11}, {6, 7, 13}, {8, 9, 14}};
+};
+template <>
+struct XMap<3> {
+    static constexpr int pairs = 7;
+    static constexpr int srcA[7] = {0, 0, 1, 3, 7, 2, 9};
+    static constexpr int ikA[7] = {0, 1, 1, 1, 0, 0, 0};
+    static constexpr int srcB[7] = {1, 4, 5, 6, 8, -1, -1};
+    static constexpr int ikB[7] = {0, 0, 0, 0, 0, 0, 0};
+    static constexpr int uvw[3] = {0, 1, 10};
+    static constexpr int prod[3][3] = {{2, 3, 8}, {4, 5, 9}, {6, 7, 12}};
+};
+
+// Per (y, z) row: c2r of the spatial channels along x (two real transforms
+// per complex FFT), the NT advection products and r2c of the products.
+// kxv[k] = the x wavenumber of half-spectrum column k.
+template <typename T, int NT, int RB>
 __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P, int Nx,
                                                 int log2Nx, int Ny, int Nz, int nkx, int nkxP,
-                                                const cx<T>* __restrict__ twX) {
-    constexpr int PAIRS = (NCH + 1) / 2;
+                                                const cx<T>* __restrict__ twX, const T* __restrict__ kxv,
+                                                T etaScale) {
+    using XM = XMap<NT>;
+    constexpr int PAIRS = XM::pairs;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NxP = xfusedStride<T>(Nx);
     const int RS = xfusedRowStride<T>(Nx, PAIRS);
@@ -379,14 +430,14 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
     const int row0 = blockIdx.x * RB;  // row = z*Ny + y
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     loadTwiddles(tw, twX, Nx);
-    // Transform t = rr*PAIRS + p packs channels 2p (real part) and 2p+1
-    // (imaginary part) of row rr.  Wave w fills transforms w, w + 4, ...: each
-    // retained entry k < nkx is read once and written at k and, conjugated, at
-    // Nx - k; the band between is zeroed.  A wave issues the loads of all its
-    // transforms (128 entries each) before storing them.
+    // Transform t = rr*PAIRS + p packs spatial channels 2p (real part) and
+    // 2p+1 (imaginary part) of row rr.  Wave w fills transforms w, w + 4, ...:
+    // each retained entry k < nkx is read once and written at k and,
+    // conjugated, at Nx - k; the band between is zeroed.  A wave issues the
+    // loads of all its transforms (128 entries each) before storing them.
     static_assert(RB * PAIRS <= 16, "at most four transforms per wave");
     constexpr int kTPW = (RB * PAIRS + 3) / 4;  // transforms per wave
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = waveIndex();
     for (int kc = 0; kc < nkx; kc += 128) {
         cx<T> va[kTPW][2], vb[kTPW][2];
 #pragma unroll
@@ -395,25 +446,37 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
             const int rr = t / PAIRS, p = t % PAIRS;
             const int row = row0 + rr;
             const bool live = t < RB * PAIRS && row < nrows;
-            const cx<T>* A = G + (2 * p) * chStride + static_cast<size_t>(row) * nkxP;
+            const int sa = XM::srcA[p], sb = XM::srcB[p];
+            const cx<T>* A = G + sa * chStride + static_cast<size_t>(row) * nkxP;
+            const cx<T>* B = G + (sb < 0 ? 0 : sb) * chStride + static_cast<size_t>(row) * nkxP;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = kc + lane + 64 * h;
                 const bool in = live && k < nkx;
                 va[i][h] = in ? A[k] : cx<T>{T(0), T(0)};
-                vb[i][h] = (in && 2 * p + 1 < NCH) ? A[chStride + k] : cx<T>{T(0), T(0)};
+                vb[i][h] = (in && sb >= 0 && sb != sa) ? B[k] : va[i][h];
             }
+        }
+        T kk[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = kc + lane + 64 * h;
+            kk[h] = k < nkx ? kxv[k] : T(0);
         }
 #pragma unroll
         for (int i = 0; i < kTPW; ++i) {
             const int t = wave + 4 * i;
             if (t >= RB * PAIRS) continue;
-            cx<T>* z = buf + (t / PAIRS) * RS + (t % PAIRS) * NxP;
+            const int p = t % PAIRS;
+            cx<T>* z = buf + (t / PAIRS) * RS + p * NxP;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = kc + lane + 64 * h;
                 if (k >= nkx) continue;
                 cx<T> a = va[i][h], b = vb[i][h];
+                if (XM::ikA[p]) a = cmuli(a, kk[h]);
+                if (XM::ikB[p]) b = cmuli(b, kk[h]);
+                if (XM::srcB[p] < 0) b = cx<T>{T(0), T(0)};
                 if (k == 0) a.im = b.im = T(0);
                 z[ldsIndex(k, 1)] = {a.re - b.im, a.im + b.re};
                 if (k > 0) z[ldsIndex(Nx - k, 1)] = {a.re + b.im, b.re - a.im};
@@ -438,19 +501,22 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
             ch[2 * p] = v.re;
             ch[2 * p + 1] = v.im;
         }
-        const T U = ch[0], V = ch[1], W = ch[2];
+        const T U = ch[XM::uvw[0]], V = ch[XM::uvw[1]], W = ch[XM::uvw[2]];
         T F[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int t = 0; t < NT; ++t) F[t] = -(U * ch[3 + 3 * t] + V * ch[4 + 3 * t] + W * ch[5 + 3 * t]);
+        for (int t = 0; t < NT; ++t)
+            F[t] = -(U * ch[XM::prod[t][0]] + V * ch[XM::prod[t][1]] + W * ch[XM::prod[t][2]]);
+        // F_eta shares a transform with F_w: scaled by etaScale (a power of two
+        // near N0) to the size of an acceleration, removed exactly below
         rb[xl] = {F[0], F[1]};
-        rb[NxP + xl] = {F[2], F[3]};
+        rb[NxP + xl] = {F[2], F[3] * etaScale};
     }
     __syncthreads();
     // forward transforms: pairs 0 and 1 of every row
     fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
     // wave w unpacks (row, pair) rq = w, w + nwaves, ...; its lanes walk k
     const int nwaves = blockDim.x >> 6;
-    for (int rq = threadIdx.x >> 6; rq < RB * 2; rq += nwaves)
+    for (int rq = waveIndex(); rq < RB * 2; rq += nwaves)
     for (int k = threadIdx.x & 63; k < nkx; k += 64) {
         const int q = rq & 1, rr = rq >> 1;
         const int row = row0 + rr;
@@ -458,10 +524,10 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
         const cx<T>* rb = buf + rr * RS + q * NxP;
         const cx<T> Za = rb[ldsIndex(k, 1)];
         const cx<T> Zb = rb[ldsIndex(k == 0 ? 0 : Nx - k, 1)];
-        const T h = T(0.5);
+        const T h = T(0.5), hb = q == 1 ? h / etaScale : h;
         const size_t o = static_cast<size_t>(row) * nkxP + k;
         if (2 * q < NT) P[(2 * q) * chStride + o] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
-        if (2 * q + 1 < NT) P[(2 * q + 1) * chStride + o] = {(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
+        if (2 * q + 1 < NT) P[(2 * q + 1) * chStride + o] = {(Za.im + Zb.im) * hb, (Zb.re - Za.re) * hb};
     }
 }
 
@@ -508,7 +574,7 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [Za][8]
     __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
-    const int jt = threadIdx.x >> 6;
+    const int jt = waveIndex();
     const int cg = blockIdx.x;
     if (threadIdx.x < 8) {
         const int col = cg * 8 + threadIdx.x;
@@ -693,7 +759,7 @@ __global__ __launch_bounds__(256) void k_projfwd(const cx<T>* __restrict__ Q, co
                                                  cx<double>* __restrict__ outAm, cx<double>* __restrict__ outA0) {
     using MF = Mfma<T>;
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int wave = blockIdx.x * (blockDim.x >> 6) + waveIndex();
     const int ncg = (ncol + 7) / 8;
     const int njt = Ja / 16;
     if (wave >= ncg * njt) return;
@@ -869,14 +935,14 @@ struct KernelImpl {
     Descriptor d;
     wv_precision precision = WV_PRECISION_FP64;
     int device = 0;
-    int NT = 4, NCH = 15, targetCode = 0, KT = 16;
+    int NT = 4, NCH = 15, NH = 12, targetCode = 0, KT = 16;  // targets, spatial and half-spectrum channels
     size_t ncol = 0;
     size_t tsize = 16;  // bytes of one field complex value
     std::atomic<bool> executing{false};
 
     DeviceBuffer cols, omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled, ApmD, ApmTE, A0TE,
         apmW, inertialScale, fWaveScale, gWaveScale, Fg, Gg;
-    DeviceBuffer synC, synS, anaC, anaS, twX, twY, mj;
+    DeviceBuffer synC, synS, anaC, anaS, twX, twY, mj, kxv;
     DeviceBuffer FV, H, G, P, Q, stage, weighted, partial, scalar;
     DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
     double retainedRowFraction = 1.0;
@@ -894,7 +960,7 @@ struct KernelImpl {
         size_t s = 0;
         for (const DeviceBuffer* b : {&rowLimit, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &twX, &twY, &mj, &FV, &H, &G, &P, &Q, &stage, &weighted, &partial, &scalar})
+                                      &anaS, &twX, &twY, &mj, &kxv, &FV, &H, &G, &P, &Q, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
         return s;
     }
@@ -968,6 +1034,7 @@ struct KernelImpl {
         uploadAs<T>(twX, d.twX);
         uploadAs<T>(twY, d.twY);
         uploadAs<T>(mj, d.m);
+        uploadAs<T>(kxv, d.kxv);
         tsize = csize<T>();
     }
 
@@ -989,8 +1056,9 @@ struct KernelImpl {
         const bool hydro = cfg.isHydrostatic != 0;
         NT = hydro ? 3 : 4;
         NCH = 3 + 3 * NT;
+        NH = hydro ? 10 : 12;
         targetCode = hydro ? (0 | (1 << 4) | (3 << 8)) : (0 | (1 << 4) | (2 << 8) | (3 << 12));
-        const size_t pairs = (static_cast<size_t>(NCH) + 1) / 2;
+        const size_t pairs = static_cast<size_t>(NT == 4 ? XMap<4>::pairs : XMap<3>::pairs);
         if (pairs * cfg.Nx > static_cast<size_t>(kFftRegs) * 256 || static_cast<size_t>(KT) * cfg.Ny > static_cast<size_t>(kFftRegs) * 256)
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx <= 512, Ny <= 4096)."};
         int maxLds = 0;
@@ -1049,8 +1117,8 @@ struct KernelImpl {
         const size_t plane = cfg.Ny * d.nkxP;
         const size_t chField = cfg.Nz * plane;
         FV.alloc(4 * d.M * tsize);
-        H.alloc(static_cast<size_t>(NCH) * chField * tsize);
-        G.alloc(static_cast<size_t>(NCH) * chField * tsize);
+        H.alloc(static_cast<size_t>(NH) * chField * tsize);
+        G.alloc(static_cast<size_t>(NH) * chField * tsize);
         P.alloc(static_cast<size_t>(NT) * chField * tsize);
         Q.alloc(static_cast<size_t>(NT) * chField * tsize);
         hipCheck(hipMemset(H.p, 0, H.bytes), "hipMemset");
@@ -1063,16 +1131,18 @@ struct KernelImpl {
     }
 
     // ---- pipeline -----------------------------------------------------------
+    // Half-spectrum channels of the flux (nonhydrostatic / hydrostatic):
+    //   u v w eta, d/dy of each target, d/dz of each target
+    // (the d/dx channels are formed in the x transform).
     SynthPlan fluxPlan() const {
         SynthPlan sp{};
-        sp.ngroups = 1 + NT;
-        sp.g[0] = {3, {0, 1, 2, 0}, {0, 0, 0, 0}, {0, 0, 1, 0}, {0, 1, 2, 0}};
-        for (int t = 0; t < NT; ++t) {
-            const int target = (targetCode >> (4 * t)) & 0xF;
-            const int famXY = target < 2 ? 0 : 1;
-            const int famZ = 1 - famXY;
-            sp.g[1 + t] = {3, {target, target, target, 0}, {1, 2, 3, 0}, {famXY, famXY, famZ, 0},
-                           {3 + 3 * t, 4 + 3 * t, 5 + 3 * t, 0}};
+        sp.ngroups = 2;
+        if (NT == 4) {
+            sp.g[0] = {0, 4, {{0, 0, 0, -1, 4}, {1, 0, 1, -1, 5}, {2, 1, 10, -1, -1}, {3, 1, 11, -1, -1}}};
+            sp.g[1] = {1, 4, {{2, 0, 2, -1, 6}, {3, 0, 3, -1, 7}, {0, 1, 8, -1, -1}, {1, 1, 9, -1, -1}}};
+        } else {
+            sp.g[0] = {0, 3, {{0, 0, 0, -1, 4}, {1, 0, 1, -1, 5}, {3, 1, 9, -1, -1}, {}}};
+            sp.g[1] = {1, 4, {{2, 0, 2, -1, -1}, {3, 0, 3, -1, 6}, {0, 1, 7, -1, -1}, {1, 1, 8, -1, -1}}};
         }
         return sp;
     }
@@ -1093,12 +1163,18 @@ struct KernelImpl {
     void launchSynth(const SynthPlan& sp, hipStream_t s) {
         constexpr int ZT = sizeof(T) == 8 ? 2 : 3;  // z tiles per task (FP64 accumulators are twice the size)
         const int ncg = static_cast<int>((ncol + 7) / 8);
-        int nch = 0;
-        for (int g = 0; g < sp.ngroups; ++g) nch += sp.g[g].nch;
-        const double bytes = static_cast<double>(nch) * d.c.Nz * ncol * sizeof(cx<T>) + 4.0 * d.M * sizeof(cx<T>);
-        const size_t lds = 4 * d.Js * 8 * sizeof(cx<T>);
+        int nout = 0;
+        for (int g = 0; g < sp.ngroups; ++g)
+            for (int q = 0; q < sp.g[g].nch; ++q)
+                nout += (sp.g[g].ch[q].outV >= 0) + (sp.g[g].ch[q].outX >= 0) + (sp.g[g].ch[q].outY >= 0);
+        const double bytes = static_cast<double>(nout) * d.c.Nz * ncol * sizeof(cx<T>) + 4.0 * d.M * sizeof(cx<T>);
+        const size_t lds = 4 * d.Js * 8 * sizeof(cx<T>) + d.Js * sizeof(T);
+        // waves per workgroup: the largest of 4, 3, 2, 1 that divides the tasks
+        const int tasks = sp.ngroups * static_cast<int>((d.Zs / 16 + ZT - 1) / ZT);
+        int waves = 4;
+        while (tasks % waves) --waves;
         timed("synth", bytes, s, [&] {
-            hipLaunchKernelGGL((k_synth<T, ZT>), dim3(static_cast<unsigned>(ncg)), dim3(256), lds, s, FV.as<cx<T>>(),
+            hipLaunchKernelGGL((k_synth<T, ZT>), dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s, FV.as<cx<T>>(),
                                cols.as<ColumnDev>(), static_cast<int>(ncol), static_cast<int>(d.M), synC.as<T>(),
                                synS.as<T>(), static_cast<int>(d.Zs), static_cast<int>(d.Js),
                                static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
@@ -1124,14 +1200,14 @@ struct KernelImpl {
 
     template <typename T, int RB>
     size_t xfusedLds() const {
-        const int pairs = (NCH + 1) / 2;
+        const int pairs = NT == 4 ? XMap<4>::pairs : XMap<3>::pairs;
         return (static_cast<size_t>(d.c.Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(static_cast<int>(d.c.Nx), pairs)) *
                sizeof(cx<T>);
     }
 
     template <typename T>
     void launchXfused(hipStream_t s) {
-        const double bytes = (static_cast<double>(NCH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>);
+        const double bytes = (static_cast<double>(NH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>);
         const int Nx = static_cast<int>(d.c.Nx), l2 = log2Exact(d.c.Nx), Ny = static_cast<int>(d.c.Ny),
                   Nz = static_cast<int>(d.c.Nz), nkx = static_cast<int>(d.nkx), nkxP = static_cast<int>(d.nkxP);
         const int nrows = Ny * Nz;
@@ -1140,20 +1216,22 @@ struct KernelImpl {
         const size_t lds = two ? xfusedLds<T, 2>() : xfusedLds<T, 1>();
         const dim3 grid(static_cast<unsigned>(two ? (nrows + 1) / 2 : nrows));
         timed("xfused", bytes, s, [&] {
+            const cx<T>* g = G.as<cx<T>>();
+            cx<T>* pp = P.as<cx<T>>();
+            const cx<T>* tw = twX.as<cx<T>>();
+            const T* kv = kxv.as<T>();
+            // power of two nearest N0 (exact scaling of the eta product)
+            const T es = static_cast<T>(std::ldexp(1.0, static_cast<int>(std::lround(std::log2(d.c.N0 > 0 ? d.c.N0 : 1.0)))));
             if (NT == 4) {
                 if (two)
-                    hipLaunchKernelGGL((k_xfused<T, 15, 4, 2>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
-                                       Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+                    hipLaunchKernelGGL((k_xfused<T, 4, 2>), grid, dim3(256), lds, s, g, pp, Nx, l2, Ny, Nz, nkx, nkxP, tw, kv, es);
                 else
-                    hipLaunchKernelGGL((k_xfused<T, 15, 4, 1>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
-                                       Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+                    hipLaunchKernelGGL((k_xfused<T, 4, 1>), grid, dim3(256), lds, s, g, pp, Nx, l2, Ny, Nz, nkx, nkxP, tw, kv, es);
             } else {
                 if (two)
-                    hipLaunchKernelGGL((k_xfused<T, 12, 3, 2>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
-                                       Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+                    hipLaunchKernelGGL((k_xfused<T, 3, 2>), grid, dim3(256), lds, s, g, pp, Nx, l2, Ny, Nz, nkx, nkxP, tw, kv, es);
                 else
-                    hipLaunchKernelGGL((k_xfused<T, 12, 3, 1>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
-                                       Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>());
+                    hipLaunchKernelGGL((k_xfused<T, 3, 1>), grid, dim3(256), lds, s, g, pp, Nx, l2, Ny, Nz, nkx, nkxP, tw, kv, es);
             }
         });
     }
@@ -1190,7 +1268,7 @@ struct KernelImpl {
         const double elapsed = t - t0;
         launchFields<T>(Ap, Am, A0, elapsed, s);
         launchSynth<T>(fluxPlan(), s);
-        launchYfft<T>(H, G, NCH, +1, s, "yfft_inv", true, false);
+        launchYfft<T>(H, G, NH, +1, s, "yfft_inv", true, false);
         launchXfused<T>(s);
         launchYfft<T>(P, Q, NT, -1, s, "yfft_fwd", false, true);
         launchProj<T>(elapsed, ep, s);
@@ -1207,9 +1285,10 @@ struct KernelImpl {
                  hipStream_t s) {
         launchFields<T>(Ap, Am, A0, t - t0, s);
         SynthPlan sp{};
-        sp.ngroups = 1;
-        if (nf == 4) sp.g[0] = {4, {0, 1, 2, 3}, {0, 0, 0, 0}, {0, 0, 1, 1}, {0, 1, 2, 3}};
-        else sp.g[0] = {3, {0, 1, 2, 0}, {0, 0, 0, 0}, {0, 0, 1, 0}, {0, 1, 2, 0}};
+        sp.ngroups = 2;
+        sp.g[0] = {0, 2, {{0, 0, 0, -1, -1}, {1, 0, 1, -1, -1}, {}, {}}};
+        if (nf == 4) sp.g[1] = {1, 2, {{2, 0, 2, -1, -1}, {3, 0, 3, -1, -1}, {}, {}}};
+        else sp.g[1] = {1, 1, {{2, 0, 2, -1, -1}, {}, {}, {}}};
         launchSynth<T>(sp, s);
         launchYfft<T>(H, G, nf, +1, s, "yfft_inv", true, false);
         const int pairs = (nf + 1) / 2;
@@ -1273,9 +1352,10 @@ struct KernelImpl {
                                static_cast<int>(d.c.Nj), M, fFamily ? 1 : 0, FV.as<cx<T>>());
         });
         SynthPlan sp{};
-        sp.ngroups = 1;
+        sp.ngroups = 2;
         const int a = fFamily ? 0 : 1, b = 1 - a;
-        sp.g[0] = {4, {0, 0, 0, 0}, {0, 1, 2, 3}, {a, a, a, b}, {0, 1, 2, 3}};
+        sp.g[0] = {a, 1, {{0, 0, 0, 1, 2}, {}, {}, {}}};
+        sp.g[1] = {b, 1, {{0, 1, 3, -1, -1}, {}, {}, {}}};
         launchSynth<T>(sp, s);
         launchYfft<T>(H, G, 4, +1, s, "yfft_inv", true, false);
         const size_t lds = static_cast<size_t>(2) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
