@@ -156,6 +156,13 @@ __global__ __launch_bounds__(256) void k_fgfields(const cx<double>* __restrict__
     FV[idx] = {T(v.re), T(v.im)};
 }
 
+// Column stride (complex entries) of the synthesis staging buffer: at least
+// Js, and 8 words modulo 64 banks apart (f32: Jp = 4 mod 32; f64: Jp = 2 mod 16).
+template <typename T>
+__host__ __device__ constexpr int synthJStride(int Js) {
+    return sizeof(T) == 4 ? Js + ((4 - Js % 32) % 32 + 32) % 32 : Js + ((2 - Js % 16) % 16 + 16) % 16;
+}
+
 // Spectral assembly + vertical synthesis (MFMA).  One workgroup owns 8
 // consecutive columns: it stages their evolved field spectra (all j < Nj) and
 // m_j in LDS once, then its waves loop over (group, z-chunk) tasks, each
@@ -171,8 +178,12 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
                                                cx<T>* __restrict__ H) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* fv = reinterpret_cast<cx<T>*>(smem);      // [4 fields][Js][8 columns]
-    T* mjs = reinterpret_cast<T*>(fv + 4 * Js * 8);  // [Js]
+    // [4 fields][8 columns][Jp]: the staging stores run along j, and the
+    // column stride puts the 8 columns x 4 rows x 2 parts that a wave reads
+    // in distinct banks
+    const int Jp = synthJStride<T>(Js);
+    cx<T>* fv = reinterpret_cast<cx<T>*>(smem);
+    T* mjs = reinterpret_cast<T*>(fv + 4 * 8 * Jp);  // [Js]
     __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
     const int waveInBlock = waveIndex();
@@ -187,14 +198,31 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
         else scol[threadIdx.x] = ColumnDev{0, 0, -1, 0, 0.0, 0.0, 0.0, 0.0, 0.0};
     }
     __syncthreads();
-    // stage spectra: fv[f][j][c], loads contiguous in j for each column
-    for (int e = threadIdx.x; e < 4 * Js * 8; e += blockDim.x) {
-        const int j = e % Js, fc = e / Js;
-        const int c = fc % 8, f = fc / 8;
-        const int wv = scol[c].wv;
-        cx<T> v{T(0), T(0)};
-        if (wv >= 0 && j < Nj) v = FV[static_cast<size_t>(f) * M + j + static_cast<size_t>(Nj) * wv];
-        fv[(f * Js + j) * 8 + c] = v;
+    // stage spectra: loads contiguous in j for each column, issued kB at a
+    // time from clamped addresses (unconditional loads)
+    {
+        constexpr int kB = 4;
+        const int tot = 4 * Js * 8;
+        for (int e0 = 0; e0 < tot; e0 += kB * static_cast<int>(blockDim.x)) {
+            cx<T> v[kB];
+            bool ok[kB];
+#pragma unroll
+            for (int i = 0; i < kB; ++i) {
+                const int e = e0 + threadIdx.x + i * blockDim.x;
+                const int ec = e < tot ? e : tot - 1;
+                const int j = ec % Js, fc = ec / Js;
+                const int c = fc % 8, f = fc / 8;
+                const int wv = scol[c].wv;
+                v[i] = FV[static_cast<size_t>(f) * M + (j < Nj ? j : Nj - 1) + static_cast<size_t>(Nj) * (wv >= 0 ? wv : 0)];
+                ok[i] = wv >= 0 && j < Nj;
+            }
+            __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
+#pragma unroll
+            for (int i = 0; i < kB; ++i) {
+                const int e = e0 + threadIdx.x + i * blockDim.x;
+                if (e < tot) fv[(e / Js) * Jp + e % Js] = ok[i] ? v[i] : cx<T>{T(0), T(0)};
+            }
+        }
     }
     for (int j = threadIdx.x; j < Js; j += blockDim.x) mjs[j] = j < Nj ? mj[j] : T(0);
     __syncthreads();
@@ -208,8 +236,8 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
     const T rowSign = (cd.flags & kColConj) ? T(-1) : T(1);
     const T partWeight = part ? ((cd.flags & kColSelfConj) ? T(0) : rowSign) : T(1);
     const T ks = rowSign * T(cd.k), ls = rowSign * T(cd.l);
-    // lane reads the real (part 0) or imaginary (part 1) word of fv[.][j][c]
-    const T* fvw = reinterpret_cast<const T*>(fv) + 2 * c + part;
+    // lane reads the real (part 0) or imaginary (part 1) word of fv[.][c][j]
+    const T* fvw = reinterpret_cast<const T*>(fv) + 2 * c * Jp + part;
     for (int task = waveInBlock; task < plan.ngroups * nzc; task += nwaves) {
         const int gi = task % plan.ngroups;
         const int zc = task / plan.ngroups;
@@ -222,7 +250,7 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
         for (int q = 0; q < 4; ++q) {
             const bool live = q < nch;
             const bool dz = live && plan.g[gi].ch[q].dz;
-            fieldOff[q] = live ? plan.g[gi].ch[q].field * Js * 16 : 0;
+            fieldOff[q] = live ? plan.g[gi].ch[q].field * 16 * Jp : 0;
             coefV[q] = live && !dz ? T(0.5) * partWeight : T(0);
             coefZ[q] = dz ? zsign * partWeight : T(0);
         }
@@ -233,33 +261,48 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
         for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int r = 0; r < ZT; ++r) acc[q][r] = MF::zero();
-        // A operands are prefetched one step ahead
-        T a[ZT], an[ZT];
+        // A operands in register batches of kS steps: two named batches used in
+        // turn (no register copies, so each wait covers only the batch in use)
+        constexpr int kS = 4;
         const T* Ar[ZT];
 #pragma unroll
         for (int r = 0; r < ZT; ++r) {
             const int zt = zt0 + r < ztiles ? zt0 + r : ztiles - 1;
             Ar[r] = A + static_cast<size_t>(zt) * nsteps * 64 + lane;
-            a[r] = Ar[r][0];
         }
-        for (int s = 0; s < nsteps; ++s) {
-            const int sn = (s + 1 < nsteps ? s + 1 : s) * 64;
+        auto load = [&](T (&dst)[ZT][kS], int s0) {
 #pragma unroll
-            for (int r = 0; r < ZT; ++r) an[r] = Ar[r][sn];
-            const int j = 4 * s + kq;
-            const T mjj = mjs[j];
-            const T* fj = fvw + j * 16;
+            for (int r = 0; r < ZT; ++r)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (q < nch) {
-                    const T b = fj[fieldOff[q]] * (coefV[q] + coefZ[q] * mjj);
+                for (int u = 0; u < kS; ++u) dst[r][u] = Ar[r][static_cast<size_t>(min(s0 + u, nsteps - 1)) * 64];
+        };
+        auto run = [&](const T (&a)[ZT][kS], int s0) {
 #pragma unroll
-                    for (int r = 0; r < ZT; ++r)
-                        if (zt0 + r < ztiles) acc[q][r] = MF::run(a[r], b, acc[q][r]);
+            for (int u = 0; u < kS; ++u) {
+                const int st = s0 + u;
+                if (st < nsteps) {
+                    const int j = 4 * st + kq;
+                    const T mjj = mjs[j];
+                    const T* fj = fvw + 2 * j;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (q < nch) {
+                            const T b = fj[fieldOff[q]] * (coefV[q] + coefZ[q] * mjj);
+#pragma unroll
+                            for (int r = 0; r < ZT; ++r)
+                                if (zt0 + r < ztiles) acc[q][r] = MF::run(a[r][u], b, acc[q][r]);
+                        }
+                    }
                 }
             }
-#pragma unroll
-            for (int r = 0; r < ZT; ++r) a[r] = an[r];
+        };
+        T a0[ZT][kS], a1[ZT][kS];
+        load(a0, 0);
+        for (int s0 = 0; s0 < nsteps; s0 += 2 * kS) {
+            load(a1, s0 + kS);
+            run(a0, s0);
+            load(a0, s0 + 2 * kS);
+            run(a1, s0 + kS);
         }
         // outputs: the value and i k / i l times it (the partner lane n^1
         // holds the other part of the same complex entry)
@@ -314,14 +357,19 @@ __global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T
 #pragma unroll
         for (int i = 0; i < kMaxPerThread; ++i) {
             const int e = threadIdx.x + i * blockDim.x;
-            const int kk = e & (KT - 1), iy = e >> log2KT;
-            const bool zero = e >= nel || (skipIn && iy > lim && iy < Ny - lim);
-            v[i] = zero ? cx<T>{T(0), T(0)} : in[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk];
+            const int ec = e < nel ? e : nel - 1;  // clamped: loads are unconditional
+            const int kk = ec & (KT - 1), iy = ec >> log2KT;
+            // rows known to be zero read row 0 instead (loaded by other lanes)
+            const bool zero = skipIn && iy > lim && iy < Ny - lim;
+            v[i] = in[plane + static_cast<size_t>(zero ? 0 : iy) * nkxP + kx0 + kk];
         }
+        __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
 #pragma unroll
         for (int i = 0; i < kMaxPerThread; ++i) {
             const int e = threadIdx.x + i * blockDim.x;
-            if (e < nel) tile[e] = v[i];
+            const int iy = e >> log2KT;
+            const bool zero = skipIn && iy > lim && iy < Ny - lim;
+            if (e < nel) tile[e] = zero ? cx<T>{T(0), T(0)} : v[i];
         }
     }
     __syncthreads();
@@ -360,21 +408,13 @@ __device__ __forceinline__ cx<T> hermitianX(const cx<T>* __restrict__ G, size_t 
 }
 
 // LDS layout of the x transforms: transform stride NxP = the padded length
-// rounded to 8 (f32) or 4 (f64) complex entries modulo one bank sweep, row
-// stride = PAIRS*NxP rounded to half a sweep, so that the four 16-thread
-// transforms a wave works on start in different bank groups.
+// (N + N/16, conflict-free radix-16 passes; for f32 it is 16 entries modulo a
+// 256-byte bank sweep, so the two transforms a half-wave reads fill disjoint
+// banks), row stride PAIRS*NxP.
 template <typename T>
-__host__ __device__ constexpr int bankSweep() { return sizeof(T) == 4 ? 32 : 16; }  // complex entries per 256 B
+__host__ __device__ constexpr int xfusedStride(int Nx) { return ldsLength(Nx); }
 template <typename T>
-__host__ __device__ constexpr int padTo(int base, int want) {
-    return base + (((want - base % bankSweep<T>()) % bankSweep<T>()) + bankSweep<T>()) % bankSweep<T>();
-}
-template <typename T>
-__host__ __device__ constexpr int xfusedStride(int Nx) { return padTo<T>(ldsLength(Nx), bankSweep<T>() / 4); }
-template <typename T>
-__host__ __device__ constexpr int xfusedRowStride(int Nx, int pairs) {
-    return padTo<T>(pairs * xfusedStride<T>(Nx), bankSweep<T>() / 2);
-}
+__host__ __device__ constexpr int xfusedRowStride(int Nx, int pairs) { return pairs * xfusedStride<T>(Nx); }
 
 // Spatial channels of the flux products, packed two per complex x transform.
 // Pair p takes half-spectrum channel srcA[p] (real part of the transform) and
@@ -440,6 +480,7 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
     const int lane = threadIdx.x & 63, wave = waveIndex();
     for (int kc = 0; kc < nkx; kc += 128) {
         cx<T> va[kTPW][2], vb[kTPW][2];
+        bool liveA[kTPW][2], liveB[kTPW][2];
 #pragma unroll
         for (int i = 0; i < kTPW; ++i) {
             const int t = wave + 4 * i;
@@ -447,22 +488,28 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
             const int row = row0 + rr;
             const bool live = t < RB * PAIRS && row < nrows;
             const int sa = XM::srcA[p], sb = XM::srcB[p];
-            const cx<T>* A = G + sa * chStride + static_cast<size_t>(row) * nkxP;
-            const cx<T>* B = G + (sb < 0 ? 0 : sb) * chStride + static_cast<size_t>(row) * nkxP;
+            // addresses clamped into the arrays so that every load is issued
+            // unconditionally (no branch, no wait between them)
+            const size_t rowOff = static_cast<size_t>(row < nrows ? row : nrows - 1) * nkxP;
+            const cx<T>* A = G + sa * chStride + rowOff;
+            const cx<T>* B = G + (sb < 0 ? sa : sb) * chStride + rowOff;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = kc + lane + 64 * h;
-                const bool in = live && k < nkx;
-                va[i][h] = in ? A[k] : cx<T>{T(0), T(0)};
-                vb[i][h] = (in && sb >= 0 && sb != sa) ? B[k] : va[i][h];
+                const int kcl = k < nkx ? k : nkx - 1;
+                va[i][h] = A[kcl];
+                vb[i][h] = B[kcl];
+                liveA[i][h] = live && k < nkx;
+                liveB[i][h] = live && k < nkx && sb >= 0;
             }
         }
         T kk[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int k = kc + lane + 64 * h;
-            kk[h] = k < nkx ? kxv[k] : T(0);
+            kk[h] = kxv[k < nkx ? k : nkx - 1];
         }
+        __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
 #pragma unroll
         for (int i = 0; i < kTPW; ++i) {
             const int t = wave + 4 * i;
@@ -473,10 +520,10 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
             for (int h = 0; h < 2; ++h) {
                 const int k = kc + lane + 64 * h;
                 if (k >= nkx) continue;
-                cx<T> a = va[i][h], b = vb[i][h];
+                cx<T> a = liveA[i][h] ? va[i][h] : cx<T>{T(0), T(0)};
+                cx<T> b = liveB[i][h] ? vb[i][h] : cx<T>{T(0), T(0)};
                 if (XM::ikA[p]) a = cmuli(a, kk[h]);
                 if (XM::ikB[p]) b = cmuli(b, kk[h]);
-                if (XM::srcB[p] < 0) b = cx<T>{T(0), T(0)};
                 if (k == 0) a.im = b.im = T(0);
                 z[ldsIndex(k, 1)] = {a.re - b.im, a.im + b.re};
                 if (k > 0) z[ldsIndex(Nx - k, 1)] = {a.re + b.im, b.re - a.im};
@@ -571,7 +618,7 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
                                                double hscale, Epilogue ep) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [Za][8]
+    cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [NT][Za][8]
     __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
     const int jt = waveIndex();
@@ -585,33 +632,74 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     const int part = n & 1;
     const size_t tStride = static_cast<size_t>(Nz) * Ny * nkxP;
     const int nsteps = Za / 4;
-
+    __syncthreads();  // scol ready
+    // stage the columns of every target, qs[t][z][c]; a thread issues kB
+    // loads before storing them
+    {
+        constexpr int kB = 4;
+        const int tot = NT * Za * 8;
+        for (int e0 = 0; e0 < tot; e0 += kB * blockDim.x) {
+            cx<T> v[kB];
+            bool ok[kB];
+#pragma unroll
+            for (int i = 0; i < kB; ++i) {
+                const int e = e0 + threadIdx.x + i * blockDim.x;
+                const int ec = e < tot ? e : tot - 1;  // clamped: loads are unconditional
+                const int c = ec & 7, zt = ec >> 3;
+                const int t = zt / Za, z = zt - t * Za;
+                const ColumnDev& cc = scol[c];
+                const int zc = z < Nz ? z : Nz - 1;
+                v[i] = Q[t * tStride + (static_cast<size_t>(zc) * Ny + cc.iy) * nkxP + cc.kx];
+                ok[i] = e < tot && cc.wv >= 0 && z < Nz;
+            }
+            __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
+#pragma unroll
+            for (int i = 0; i < kB; ++i) {
+                const int e = e0 + threadIdx.x + i * blockDim.x;
+                if (e < tot) qs[e] = ok[i] ? v[i] : cx<T>{T(0), T(0)};
+            }
+        }
+    }
+    __syncthreads();
+    // analysis: each family's matrix is streamed once for all its targets,
+    // in register batches of kS steps loaded one batch ahead
     typename MF::acc acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = MF::zero();
+    const T* qw = reinterpret_cast<const T*>(qs) + 2 * (n >> 1) + part;  // [t][z][c] words
+    constexpr int kS = 8;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        __syncthreads();  // scol ready / previous target consumed
-        for (int e = threadIdx.x; e < Za * 8; e += blockDim.x) {
-            const int c = e % 8, z = e / 8;
-            const ColumnDev& cc = scol[c];
-            cx<T> v{T(0), T(0)};
-            if (cc.wv >= 0 && z < Nz) v = Q[t * tStride + (static_cast<size_t>(z) * Ny + cc.iy) * nkxP + cc.kx];
-            qs[e] = v;
+    for (int fam = 0; fam < 2; ++fam) {
+        const T* Aj = (fam ? anaS : anaC) + static_cast<size_t>(jt) * nsteps * 64 + lane;
+        // two named batches used in turn (no register copies, so the waits
+        // before the MFMAs cover only the batch in use)
+        auto load = [&](T (&dst)[kS], int s0) {
+#pragma unroll
+            for (int u = 0; u < kS; ++u) dst[u] = Aj[static_cast<size_t>(min(s0 + u, nsteps - 1)) * 64];
+        };
+        auto run = [&](const T (&a)[kS], int s0) {
+#pragma unroll
+            for (int u = 0; u < kS; ++u) {
+                const int st = s0 + u;
+                if (st < nsteps) {
+                    const int zrow = 4 * st + kq;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const int target = (targetCode >> (4 * t)) & 0xF;
+                        if ((target >= 2) == (fam == 1))
+                            acc[t] = MF::run(a[u], qw[2 * ((t * Za + zrow) * 8)], acc[t]);
+                    }
+                }
+            }
+        };
+        T a0[kS], a1[kS];
+        load(a0, 0);
+        for (int s0 = 0; s0 < nsteps; s0 += 2 * kS) {
+            load(a1, s0 + kS);
+            run(a0, s0);
+            load(a0, s0 + 2 * kS);
+            run(a1, s0 + kS);
         }
-        __syncthreads();
-        const int target = (targetCode >> (4 * t)) & 0xF;
-        const T* A = target < 2 ? anaC : anaS;
-        typename MF::acc a4 = MF::zero();
-        const T* Aj = A + static_cast<size_t>(jt) * nsteps * 64 + lane;
-        T a = Aj[0];
-        for (int s = 0; s < nsteps; ++s) {
-            const T an = Aj[static_cast<size_t>(s + 1 < nsteps ? s + 1 : s) * 64];
-            const cx<T> q = qs[(4 * s + kq) * 8 + (n >> 1)];
-            a4 = MF::run(a, part ? q.im : q.re, a4);
-            a = an;
-        }
-        acc[t] = a4;
     }
     const ColumnDev cd = scol[n >> 1];
     const bool colValid = cd.wv >= 0;
@@ -1168,7 +1256,7 @@ struct KernelImpl {
             for (int q = 0; q < sp.g[g].nch; ++q)
                 nout += (sp.g[g].ch[q].outV >= 0) + (sp.g[g].ch[q].outX >= 0) + (sp.g[g].ch[q].outY >= 0);
         const double bytes = static_cast<double>(nout) * d.c.Nz * ncol * sizeof(cx<T>) + 4.0 * d.M * sizeof(cx<T>);
-        const size_t lds = 4 * d.Js * 8 * sizeof(cx<T>) + d.Js * sizeof(T);
+        const size_t lds = 4 * 8 * static_cast<size_t>(synthJStride<T>(static_cast<int>(d.Js))) * sizeof(cx<T>) + d.Js * sizeof(T);
         // waves per workgroup: the largest of 4, 3, 2, 1 that divides the tasks
         const int tasks = sp.ngroups * static_cast<int>((d.Zs / 16 + ZT - 1) / ZT);
         int waves = 4;
@@ -1247,7 +1335,7 @@ struct KernelImpl {
         timed("proj", bytes, s, [&] {
             const int njt = static_cast<int>(d.Ja / 16);
             const dim3 grid(static_cast<unsigned>(ncg)), block(static_cast<unsigned>(64 * njt));
-            const size_t lds = d.Za * 8 * sizeof(cx<T>);
+            const size_t lds = static_cast<size_t>(NT) * d.Za * 8 * sizeof(cx<T>);
             const int ncolI = static_cast<int>(ncol), Ja = static_cast<int>(d.Ja), Za = static_cast<int>(d.Za),
                       Nz = static_cast<int>(d.c.Nz), Nj = static_cast<int>(d.c.Nj), Ny = static_cast<int>(d.c.Ny),
                       nkxP = static_cast<int>(d.nkxP), hydro = d.c.isHydrostatic ? 1 : 0;
