@@ -86,6 +86,16 @@ def stage_profile(k, state, dt, steps=2):
     return rec
 
 
+def host_flux_rate(k, state, calls=5):
+    """Flux evaluations/s through the host-memory entry point (copies in and
+    out over PCIe each call, as the reference's host-memory contract does)."""
+    k.nonlinear_flux_host(*state, t=0.0)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        k.nonlinear_flux_host(*state, t=0.0)
+    return calls / (time.perf_counter() - t0)
+
+
 def cpu_baseline(cfg, state, dt, threads, steps=4):
     """Oracle FP64 RK4 steps at the full configuration on the host cores."""
     from oracle import Oracle
@@ -145,6 +155,8 @@ def main():
     sps32 = args.steps / sec32
     sps64 = args.steps / sec64
 
+    host32 = host_flux_rate(k32, state)
+    host64 = host_flux_rate(k64, state)
     rec = stage_profile(k32, state, dt)
     dom = max(rec, key=lambda r: r["total_ms"])
     avg_ms = dom["total_ms"] / max(dom["launches"], 1)
@@ -179,6 +191,8 @@ def main():
         "fp64_steps_per_s": round(sps64, 3), "fp64_ms_per_step": round(1e3 * sec64 / args.steps, 4),
         "mixed_over_fp64": round(sps32 / sps64, 3), "energy_rel_err_mixed_vs_fp64": energy_err,
         "roofline": roofline, "reference_dataflow_roofline": step_model,
+        "host_memory_flux_per_s": {"mixed": round(host32, 2), "fp64": round(host64, 2),
+                                   "note": "PCIe-inclusive: state copied in and flux copied out per call"},
         "stage_ms_per_launch_mixed": stages, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

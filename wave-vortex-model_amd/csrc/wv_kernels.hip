@@ -1014,7 +1014,7 @@ struct StageRecord {
     std::string name;
     double totalMs = 0.0;
     uint64_t launches = 0;
-    double bytesPerLaunch = 0.0;
+    double totalBytes = 0.0;  // algorithmic bytes summed over the timed launches
 };
 
 }  // namespace
@@ -1040,6 +1040,7 @@ struct KernelImpl {
     std::vector<StageRecord> records;
     struct Pending {
         int record;
+        double bytes;
         hipEvent_t a, b;
     };
     std::vector<Pending> pending;
@@ -1053,10 +1054,10 @@ struct KernelImpl {
         return s;
     }
 
-    int recordIndex(const char* name, double bytes) {
+    int recordIndex(const char* name) {
         for (size_t i = 0; i < records.size(); ++i)
             if (records[i].name == name) return static_cast<int>(i);
-        records.push_back({name, 0.0, 0, bytes});
+        records.push_back({name, 0.0, 0, 0.0});
         return static_cast<int>(records.size() - 1);
     }
 
@@ -1067,7 +1068,7 @@ struct KernelImpl {
             hipCheck(hipGetLastError(), name);
             return;
         }
-        Pending pd{recordIndex(name, bytes), nullptr, nullptr};
+        Pending pd{recordIndex(name), bytes, nullptr, nullptr};
         hipCheck(hipEventCreate(&pd.a), "hipEventCreate");
         hipCheck(hipEventCreate(&pd.b), "hipEventCreate");
         hipCheck(hipEventRecord(pd.a, s), "hipEventRecord");
@@ -1084,6 +1085,7 @@ struct KernelImpl {
             hipCheck(hipEventElapsedTime(&ms, pd.a, pd.b), "hipEventElapsedTime");
             records[pd.record].totalMs += ms;
             records[pd.record].launches += 1;
+            records[pd.record].totalBytes += pd.bytes;
             (void)hipEventDestroy(pd.a);
             (void)hipEventDestroy(pd.b);
         }
@@ -1238,7 +1240,9 @@ struct KernelImpl {
     template <typename T>
     void launchFields(const double* Ap, const double* Am, const double* A0, double elapsed, hipStream_t s) {
         const int M = static_cast<int>(d.M);
-        timed("fields", 48.0 * M + 136.0 * M + 4.0 * M * sizeof(cx<T>), s, [&] {
+        // reads Ap, Am, A0 (48 B) and omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0
+        // (104 B) per coefficient, writes four field spectra
+        timed("fields", 152.0 * M + 4.0 * M * sizeof(cx<T>), s, [&] {
             hipLaunchKernelGGL(k_fields<T>, dim3((M + 255) / 256), dim3(256), 0, s,
                                reinterpret_cast<const cx<double>*>(Ap), reinterpret_cast<const cx<double>*>(Am),
                                reinterpret_cast<const cx<double>*>(A0), omega.as<double>(), UAp.as<cx<double>>(),
@@ -1331,7 +1335,12 @@ struct KernelImpl {
                       ApmN.as<double>(), ApmDScaled.as<double>(), ApmD.as<cx<double>>(), apmW.as<double>(),
                       inertialScale.as<double>()};
         const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
-        const double bytes = static_cast<double>(NT) * d.c.Nz * ncol * sizeof(cx<T>) + d.M * (48.0 + 48.0 + 48.0);
+        // Q columns of every target, the projection tables (48 B per
+        // coefficient) and the epilogue's coefficient arrays: flux out 48 B;
+        // RK stage 1 reads y, writes w and the stage (144 B); stages 2, 3 read
+        // w, y and write w, stage (192 B); stage 4 reads w, y and writes y (144 B)
+        const double epBytes = ep.mode == kFluxOut ? 48.0 : (ep.mode == kRk2 || ep.mode == kRk3) ? 192.0 : 144.0;
+        const double bytes = static_cast<double>(NT) * d.c.Nz * ncol * sizeof(cx<T>) + d.M * (48.0 + epBytes);
         timed("proj", bytes, s, [&] {
             const int njt = static_cast<int>(d.Ja / 16);
             const dim3 grid(static_cast<unsigned>(ncg)), block(static_cast<unsigned>(64 * njt));
@@ -1837,7 +1846,9 @@ int wv_stage_timings(wv_kernel* kernel, wv_stage_timing* out, int capacity) {
         std::snprintf(out[i].name, sizeof(out[i].name), "%s", r[static_cast<size_t>(i)].name.c_str());
         out[i].total_ms = r[static_cast<size_t>(i)].totalMs;
         out[i].launches = r[static_cast<size_t>(i)].launches;
-        out[i].bytes_per_launch = r[static_cast<size_t>(i)].bytesPerLaunch;
+        out[i].bytes_per_launch = r[static_cast<size_t>(i)].launches
+                                      ? r[static_cast<size_t>(i)].totalBytes / static_cast<double>(r[static_cast<size_t>(i)].launches)
+                                      : 0.0;
     }
     return static_cast<int>(r.size());
 }
