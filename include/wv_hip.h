@@ -85,6 +85,34 @@ typedef struct wv_stage_timing {
  * this library's own HIP kernels.  `device` selects the HIP device. */
 wv_status wv_create(const wv_config* config, wv_precision precision, int device, wv_kernel** kernel);
 void wv_destroy(wv_kernel* kernel);
+/* Vertical operators of an arbitrary stratification N^2(z) with a rigid lid, as
+ * WVGeometryDoublyPeriodicStratified holds them
+ * (@WVGeometryDoublyPeriodicStratified/WVGeometryDoublyPeriodicStratified.m:117,
+ * @WVStratification/verticalProjectionOperatorsWithRigidLid.m:17-72); matrices
+ * column-major as in MATLAB.  h[0] (the barotropic entry) is not used. */
+typedef struct wv_vertical_operators {
+    const double* z;      /* [Nz] quadrature points, -Lz .. 0 */
+    const double* N2;     /* [Nz] N^2 at z */
+    const double* dLnN2;  /* [Nz] d ln N^2 / dz at z */
+    const double* h;      /* [Nj] equivalent depths h_0 */
+    const double* P;      /* [Nj] F preconditioner P0 */
+    const double* Q;      /* [Nj] G preconditioner Q0 */
+    const double* PFinv;  /* [Nz x Nj] PF0inv */
+    const double* PF;     /* [Nj x Nz] PF0 */
+    const double* QGinv;  /* [Nz x Nj] QG0inv */
+    const double* QG;     /* [Nj x Nz] QG0 */
+} wv_vertical_operators;
+
+/* The hydrostatic transform of an arbitrary stratification
+ * (WVTransformHydrostatic, @WVTransformHydrostatic/WVTransformHydrostatic.m;
+ * nonlinearFlux :381-393 with WVNonlinearAdvection.addHydrostaticSpatialForcing,
+ * Forcing/WVNonlinearAdvection.m:103-107).  config->isHydrostatic must be set;
+ * config->N0 is a representative buoyancy frequency (scaling only).  The
+ * handle supports every entry point below with the dense F/G operators in
+ * place of the DCT/DST. */
+wv_status wv_create_hydrostatic(const wv_config* config, const wv_vertical_operators* operators,
+                                wv_precision precision, int device, wv_kernel** kernel);
+
 /* Message of the last failing call on this thread (or of the last create). */
 const char* wv_last_error(void);
 uint32_t wv_contract_version(void);

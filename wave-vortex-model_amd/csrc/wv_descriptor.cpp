@@ -55,8 +55,9 @@ double sinPiRatio(size_t p, size_t q) {
 
 }  // namespace
 
-std::string Descriptor::build(const wv_config& cfg, size_t kxTile) {
+std::string Descriptor::build(const wv_config& cfg, size_t kxTile, const Stratification* strat) {
     c = cfg;
+    stratified = strat != nullptr;
     if (c.contractVersion != WV_CONTRACT_VERSION) return "Unsupported kernel contract version.";
     if (c.Nx < 2 || c.Ny < 2 || c.Nz < 3 || c.Nj < 1 || c.Nj > c.Nz - 1)
         return "Nx and Ny must be at least 2, Nz at least 3, and Nj must lie in [1,Nz-1].";
@@ -108,6 +109,22 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile) {
     const double N02 = c.N0 * c.N0, f2 = f * f;
     if (!c.isHydrostatic && N02 <= f2)
         return "Nonhydrostatic constant stratification requires N0 squared greater than f squared.";
+    if (stratified) {
+        if (!c.isHydrostatic) return "Arbitrary stratification is supported for the hydrostatic transform only.";
+        const size_t nz = c.Nz, nj = c.Nj;
+        if (strat->z.size() != nz || strat->N2.size() != nz || strat->dLnN2.size() != nz || strat->h.size() != nj ||
+            strat->P.size() != nj || strat->Q.size() != nj || strat->PFinv.size() != nz * nj ||
+            strat->QGinv.size() != nz * nj || strat->PF.size() != nj * nz || strat->QG.size() != nj * nz)
+            return "Vertical operators do not match Nz and Nj.";
+        for (const auto* v : {&strat->z, &strat->N2, &strat->dLnN2, &strat->h, &strat->P, &strat->Q, &strat->PFinv,
+                              &strat->QGinv, &strat->PF, &strat->QG})
+            for (double x : *v)
+                if (!std::isfinite(x)) return "Vertical operators must be finite.";
+        for (size_t j = 1; j < nj; ++j)
+            if (!(strat->h[j] > 0.0)) return "Equivalent depths must be positive.";
+        for (size_t j = 0; j < nj; ++j)
+            if (strat->P[j] == 0.0 || strat->Q[j] == 0.0) return "Preconditioners must be nonzero.";
+    }
     z.resize(Nz);
     h0.resize(Nj);
     m.resize(Nj);
@@ -117,12 +134,18 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile) {
     gWaveScale.resize(Nj);
     apmW.assign(Nj, 0.0);
     const double dz = c.Lz / static_cast<double>(Nz - 1);
-    for (size_t i = 0; i < Nz; ++i) z[i] = dz * static_cast<double>(i) - c.Lz;
+    for (size_t i = 0; i < Nz; ++i) z[i] = stratified ? strat->z[i] : dz * static_cast<double>(i) - c.Lz;
     for (size_t j = 0; j < Nj; ++j) {
-        m[j] = static_cast<double>(j) * kPi / c.Lz;
-        h0[j] = j == 0 ? c.Lz : N02 / (c.g * m[j] * m[j]);
+        // arbitrary N^2: h_pm = h_0 from the eigenproblem; the barotropic entry
+        // takes Lz as in the constant-N tables (it enters the energy factors only)
+        m[j] = stratified ? 0.0 : static_cast<double>(j) * kPi / c.Lz;
+        h0[j] = j == 0 ? c.Lz : stratified ? strat->h[j] : N02 / (c.g * m[j] * m[j]);
     }
     for (size_t j = 0; j < Nj; ++j) {
+        if (stratified) {  // F and G are the normalized modes themselves
+            Fg[j] = Gg[j] = gWaveScale[j] = 1.0;
+            continue;
+        }
         const double sign = j % 2 == 0 ? 1.0 : -1.0;
         Fg[j] = j == 0 ? 2.0 : sign * h0[j] * m[j] * std::sqrt(2.0 * kModalGravity / (c.Lz * N02));
         Gg[j] = j == 0 ? 1.0 : sign * std::sqrt(2.0 * kModalGravity / (c.Lz * N02));
@@ -143,7 +166,7 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile) {
             const double sign = j % 2 == 0 ? 1.0 : -1.0;
             const bool wave = Kh > 0.0 && j > 0, inertial = Kh == 0.0, geostrophic = Kh > 0.0, mda = Kh == 0.0 && j > 0;
             double hpm = 1.0;
-            if (j > 0) hpm = c.isHydrostatic ? N02 / (c.g * Mj * Mj) : (N02 - f2) / (c.g * (Mj * Mj + Kh2));
+            if (j > 0) hpm = stratified ? h0[j] : c.isHydrostatic ? N02 / (c.g * Mj * Mj) : (N02 - f2) / (c.g * (Mj * Mj + Kh2));
             double Fw = Fg[j], Gw = Gg[j];
             if (!c.isHydrostatic) {
                 Fw = j == 0 ? 2.0 : sign * hpm * Mj * std::sqrt(2.0 * kModalGravity / (c.Lz * (N02 - f2)));
@@ -154,7 +177,7 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile) {
             omega[idx] = om;
             fWaveScale[idx] = Fg[j] / Fwg;
             if (im == 0) inertialScale[j] = 0.5 * Fwg / Fg[j];
-            const double prefactor = sign * std::sqrt(c.g * c.Lz / (2.0 * (N02 - f2)));
+            const double prefactor = stratified ? 0.0 : sign * std::sqrt(c.g * c.Lz / (2.0 * (N02 - f2)));
             ApmDScaled[idx] = (Mj / 2.0) * prefactor;
             if (im == 0) apmW[j] = prefactor;
             if (wave) {
@@ -273,7 +296,26 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile) {
     synS.assign(Zs * Js, 0.0);
     anaC.assign(Ja * Za, 0.0);
     anaS.assign(Ja * Za, 0.0);
-    for (size_t iz = 0; iz < Nz; ++iz) {
+    if (stratified) {
+        synGz.assign(Zs * Js, 0.0);
+        synFh.assign(Zs * Js, 0.0);
+        synEz.assign(Zs * Js, 0.0);
+        for (size_t iz = 0; iz < Nz; ++iz) {
+            for (size_t j = 0; j < Nj; ++j) {
+                const double F = strat->PFinv[iz + Nz * j] * strat->P[j];
+                const double G = strat->QGinv[iz + Nz * j] * strat->Q[j];
+                const double Fh = j == 0 ? 0.0 : F / strat->h[j];
+                synC[iz * Js + j] = F;
+                synS[iz * Js + j] = G;
+                synGz[iz * Js + j] = -(strat->N2[iz] / c.g) * G;
+                synFh[iz * Js + j] = Fh;
+                synEz[iz * Js + j] = Fh + strat->dLnN2[iz] * G;
+                anaC[j * Za + iz] = strat->PF[j + Nj * iz] / strat->P[j];
+                anaS[j * Za + iz] = strat->QG[j + Nj * iz] / strat->Q[j];
+            }
+        }
+    }
+    for (size_t iz = 0; iz < Nz && !stratified; ++iz) {
         for (size_t j = 0; j < Nj; ++j) {
             synC[iz * Js + j] = (j == 0 ? 1.0 : 2.0) * cosPiRatio(j * iz, L);
             const bool interior = iz >= 1 && iz + 1 <= Nz - 1 && j >= 1;

@@ -33,8 +33,21 @@ struct Column {
     uint32_t flags = 0;
 };
 
+// Vertical structure of an arbitrary stratification N^2(z) (rigid lid,
+// hydrostatic): the operators WVGeometryDoublyPeriodicStratified holds
+// (@WVGeometryDoublyPeriodicStratified/WVGeometryDoublyPeriodicStratified.m:117,
+// @WVStratification/verticalProjectionOperatorsWithRigidLid.m:17-72), all
+// column-major as in MATLAB.
+struct Stratification {
+    std::vector<double> z, N2, dLnN2;  // [Nz] quadrature points and N^2, d ln N^2 / dz there
+    std::vector<double> h, P, Q;       // [Nj]
+    std::vector<double> PFinv, QGinv;  // [Nz x Nj]
+    std::vector<double> PF, QG;        // [Nj x Nz]
+};
+
 struct Descriptor {
     wv_config c{};
+    bool stratified = false;  // dense F/G operators of an arbitrary N^2(z) instead of the DCT/DST
     size_t NxHalf = 0, Nkl = 0, M = 0, R = 0;
     size_t nkx = 0;   // 1 + largest kx present
     size_t nkxP = 0;  // nkx padded to the y-FFT tile width
@@ -57,12 +70,18 @@ struct Descriptor {
     //             the unnormalized DCT-I / DST-I restricted to j < Nj.
     //   ana[C|S]  [Ja x Za] row-major (j, z): physical z -> coefficient j, the
     //             DCT-I / DST-I followed by the forward normalization.
+    // For an arbitrary stratification synC/synS hold the F and G mode values
+    // (PFinv*P, QGinv*Q), anaC/anaS the projections (PF/P, QG/Q), and
+    //   synGz = -(N^2/g) G            (d/dz of an F-family field, diffZF.m:30-35)
+    //   synFh = F / h                 (d/dz of a G-family field, diffZG.m:26-28)
+    //   synEz = F / h + dLnN2 * G     (d/dz eta + eta dLnN2, WVNonlinearAdvection.m:106)
     size_t Zs = 0, Js = 0, Ja = 0, Za = 0;
-    std::vector<double> synC, synS, anaC, anaS;
+    std::vector<double> synC, synS, anaC, anaS, synGz, synFh, synEz;
     std::vector<double> twX, twY;  // FFT twiddles (Stockham pass order for powers of two), interleaved
 
     // Returns an empty string on success, otherwise the reference's message.
-    std::string build(const wv_config& config, size_t kxTile);
+    // strat != nullptr selects the arbitrary-stratification vertical structure.
+    std::string build(const wv_config& config, size_t kxTile, const Stratification* strat = nullptr);
     // Named table export (same names as the reference descriptor fields).
     long table(const std::string& name, double* out) const;
 };

@@ -54,27 +54,37 @@ struct ColumnDev {
 };
 
 // Vertical synthesis plan.  A channel is the evolved spectrum of field
-// `field` (0=u, 1=v, 2=w, 3=eta) or, with dz set, its z derivative (factor
-// -m_j into the sine family, +m_j into the cosine family), times the 1/2 of
-// the reference assembly (...Kernel.cpp:240,286-288, WVCoefficientFormulas.hpp:
-// 42-58), synthesized with the DCT-I (fam 0) or DST-I (fam 1) of its group.
+// `field` (0=u, 1=v, 2=w, 3=eta) times valueScale or, with dz set, its z
+// derivative in spectral form (factor -m_j/2 into the sine family, +m_j/2
+// into the cosine family: the constant-N assembly, ...Kernel.cpp:240,286-288,
+// WVCoefficientFormulas.hpp:42-58, whose value channels carry valueScale 1/2),
+// synthesized with the group's matrix `mat`:
+//   constant N:  0 DCT-I, 1 DST-I
+//   arbitrary N: 0 F, 1 G, 2 -(N^2/g) G, 3 F/h, 4 F/h + dLnN2 G (z derivatives
+//                are dense operators, valueScale 1)
 // The result goes to up to three half-spectrum channels: outV (the value),
 // outX (i k times it) and outY (i l times it), k and l the wavenumbers of the
 // storage row; -1 = not written.  Horizontal derivatives commute with the
 // vertical transform, so they cost a scaling instead of a synthesis.  The
-// sine-family endpoint zeroing is carried by the DST-I matrix, whose j = 0
-// column is zero.
+// sine-family endpoint zeroing is carried by the DST-I / G matrices, whose
+// j = 0 column and end rows are zero.
 struct SynthChannel {
     int field, dz, outV, outX, outY;
 };
 struct SynthGroup {
-    int fam, nch;
+    int mat, nch;
     SynthChannel ch[4];
 };
 constexpr int kMaxGroups = 8;
+constexpr int kSynthMats = 5;
 struct SynthPlan {
     int ngroups;
+    double valueScale;
     SynthGroup g[kMaxGroups];
+};
+template <typename T>
+struct SynthMats {
+    const T* m[kSynthMats];
 };
 
 enum EpilogueMode : int { kFluxOut = 0, kRk1 = 1, kRk2 = 2, kRk3 = 3, kRk4 = 4 };
@@ -172,9 +182,8 @@ __host__ __device__ constexpr int synthJStride(int Js) {
 // (column (l%16)/2, real or imaginary part) and k-row l/16.
 template <typename T, int ZT>
 __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
-                                               int ncol, int M, const T* __restrict__ synC,
-                                               const T* __restrict__ synS, int Zs, int Js, int Nz, int Nj, int Ny,
-                                               int nkxP, const T* __restrict__ mj, SynthPlan plan,
+                                               int ncol, int M, SynthMats<T> mats, int Zs, int Js, int Nz,
+                                               int Nj, int Ny, int nkxP, const T* __restrict__ mj, SynthPlan plan,
                                                cx<T>* __restrict__ H) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -241,20 +250,21 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
     for (int task = waveInBlock; task < plan.ngroups * nzc; task += nwaves) {
         const int gi = task % plan.ngroups;
         const int zc = task / plan.ngroups;
-        const int fam = plan.g[gi].fam, nch = plan.g[gi].nch;
+        const int mat = plan.g[gi].mat, nch = plan.g[gi].nch;
         int fieldOff[4];
         T coefV[4], coefZ[4];
-        // value channels: 1/2; z-derivative channels: -/+ m_j / 2
-        const T zsign = fam ? T(-0.5) : T(0.5);
+        // value channels: valueScale; spectral z-derivative channels: -/+ m_j / 2
+        const T zsign = mat == 1 ? T(-0.5) : T(0.5);
+        const T vscale = T(plan.valueScale);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const bool live = q < nch;
             const bool dz = live && plan.g[gi].ch[q].dz;
             fieldOff[q] = live ? plan.g[gi].ch[q].field * 16 * Jp : 0;
-            coefV[q] = live && !dz ? T(0.5) * partWeight : T(0);
+            coefV[q] = live && !dz ? vscale * partWeight : T(0);
             coefZ[q] = dz ? zsign * partWeight : T(0);
         }
-        const T* A = fam ? synS : synC;
+        const T* A = mats.m[mat];
         const int zt0 = zc * ZT;
         typename MF::acc acc[4][ZT];
 #pragma unroll
@@ -1030,7 +1040,7 @@ struct KernelImpl {
 
     DeviceBuffer cols, omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled, ApmD, ApmTE, A0TE,
         apmW, inertialScale, fWaveScale, gWaveScale, Fg, Gg;
-    DeviceBuffer synC, synS, anaC, anaS, twX, twY, mj, kxv;
+    DeviceBuffer synC, synS, anaC, anaS, synGz, synFh, synEz, twX, twY, mj, kxv;
     DeviceBuffer FV, H, G, P, Q, stage, weighted, partial, scalar;
     DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
     double retainedRowFraction = 1.0;
@@ -1049,7 +1059,7 @@ struct KernelImpl {
         size_t s = 0;
         for (const DeviceBuffer* b : {&rowLimit, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &twX, &twY, &mj, &kxv, &FV, &H, &G, &P, &Q, &stage, &weighted, &partial, &scalar})
+                                      &anaS, &synGz, &synFh, &synEz, &twX, &twY, &mj, &kxv, &FV, &H, &G, &P, &Q, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
         return s;
     }
@@ -1121,6 +1131,11 @@ struct KernelImpl {
         uploadAs<T>(synS, mfmaOperand(d.synS, d.Zs, d.Js));
         uploadAs<T>(anaC, mfmaOperand(d.anaC, d.Ja, d.Za));
         uploadAs<T>(anaS, mfmaOperand(d.anaS, d.Ja, d.Za));
+        if (d.stratified) {
+            uploadAs<T>(synGz, mfmaOperand(d.synGz, d.Zs, d.Js));
+            uploadAs<T>(synFh, mfmaOperand(d.synFh, d.Zs, d.Js));
+            uploadAs<T>(synEz, mfmaOperand(d.synEz, d.Zs, d.Js));
+        }
         uploadAs<T>(twX, d.twX);
         uploadAs<T>(twY, d.twY);
         uploadAs<T>(mj, d.m);
@@ -1128,7 +1143,7 @@ struct KernelImpl {
         tsize = csize<T>();
     }
 
-    void setup(const wv_config& cfg, wv_precision prec, int dev) {
+    void setup(const wv_config& cfg, wv_precision prec, int dev, const Stratification* strat = nullptr) {
         precision = prec;
         device = dev;
         hipCheck(hipSetDevice(dev), "hipSetDevice");
@@ -1136,7 +1151,7 @@ struct KernelImpl {
         while (KT & (KT - 1)) --KT;  // power of two
         std::string err;
         try {
-            err = d.build(cfg, static_cast<size_t>(KT));
+            err = d.build(cfg, static_cast<size_t>(KT), strat);
         } catch (const std::overflow_error& e) {
             throw Failure{WV_SIZE_OVERFLOW, e.what()};
         } catch (const std::bad_alloc&) {
@@ -1223,11 +1238,20 @@ struct KernelImpl {
     // ---- pipeline -----------------------------------------------------------
     // Half-spectrum channels of the flux (nonhydrostatic / hydrostatic):
     //   u v w eta, d/dy of each target, d/dz of each target
-    // (the d/dx channels are formed in the x transform).
+    // (the d/dx channels are formed in the x transform).  For an arbitrary
+    // stratification the eta target's z channel is d/dz eta + eta dLnN2.
     SynthPlan fluxPlan() const {
         SynthPlan sp{};
+        sp.valueScale = 0.5;
         sp.ngroups = 2;
-        if (NT == 4) {
+        if (d.stratified) {
+            sp.valueScale = 1.0;
+            sp.ngroups = 4;
+            sp.g[0] = {0, 2, {{0, 0, 0, -1, 4}, {1, 0, 1, -1, 5}, {}, {}}};
+            sp.g[1] = {1, 2, {{2, 0, 2, -1, -1}, {3, 0, 3, -1, 6}, {}, {}}};
+            sp.g[2] = {2, 2, {{0, 0, 7, -1, -1}, {1, 0, 8, -1, -1}, {}, {}}};
+            sp.g[3] = {4, 1, {{3, 0, 9, -1, -1}, {}, {}, {}}};
+        } else if (NT == 4) {
             sp.g[0] = {0, 4, {{0, 0, 0, -1, 4}, {1, 0, 1, -1, 5}, {2, 1, 10, -1, -1}, {3, 1, 11, -1, -1}}};
             sp.g[1] = {1, 4, {{2, 0, 2, -1, 6}, {3, 0, 3, -1, 7}, {0, 1, 8, -1, -1}, {1, 1, 9, -1, -1}}};
         } else {
@@ -1235,6 +1259,12 @@ struct KernelImpl {
             sp.g[1] = {1, 4, {{2, 0, 2, -1, -1}, {3, 0, 3, -1, 6}, {0, 1, 7, -1, -1}, {1, 1, 8, -1, -1}}};
         }
         return sp;
+    }
+
+    template <typename T>
+    SynthMats<T> synthMats() const {
+        return SynthMats<T>{{synC.as<T>(), synS.as<T>(), d.stratified ? synGz.as<T>() : nullptr,
+                             d.stratified ? synFh.as<T>() : nullptr, d.stratified ? synEz.as<T>() : nullptr}};
     }
 
     template <typename T>
@@ -1267,8 +1297,8 @@ struct KernelImpl {
         while (tasks % waves) --waves;
         timed("synth", bytes, s, [&] {
             hipLaunchKernelGGL((k_synth<T, ZT>), dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s, FV.as<cx<T>>(),
-                               cols.as<ColumnDev>(), static_cast<int>(ncol), static_cast<int>(d.M), synC.as<T>(),
-                               synS.as<T>(), static_cast<int>(d.Zs), static_cast<int>(d.Js),
+                               cols.as<ColumnDev>(), static_cast<int>(ncol), static_cast<int>(d.M), synthMats<T>(),
+                               static_cast<int>(d.Zs), static_cast<int>(d.Js),
                                static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
                                static_cast<int>(d.nkxP), mj.as<T>(), sp, H.as<cx<T>>());
         });
@@ -1382,6 +1412,7 @@ struct KernelImpl {
                  hipStream_t s) {
         launchFields<T>(Ap, Am, A0, t - t0, s);
         SynthPlan sp{};
+        sp.valueScale = d.stratified ? 1.0 : 0.5;
         sp.ngroups = 2;
         sp.g[0] = {0, 2, {{0, 0, 0, -1, -1}, {1, 0, 1, -1, -1}, {}, {}}};
         if (nf == 4) sp.g[1] = {1, 2, {{2, 0, 2, -1, -1}, {3, 0, 3, -1, -1}, {}, {}}};
@@ -1451,8 +1482,12 @@ struct KernelImpl {
         SynthPlan sp{};
         sp.ngroups = 2;
         const int a = fFamily ? 0 : 1, b = 1 - a;
+        sp.valueScale = d.stratified ? 1.0 : 0.5;
         sp.g[0] = {a, 1, {{0, 0, 0, 1, 2}, {}, {}, {}}};
-        sp.g[1] = {b, 1, {{0, 1, 3, -1, -1}, {}, {}, {}}};
+        if (d.stratified)  // d/dz of F is -(N^2/g) G, of G is F/h
+            sp.g[1] = {fFamily ? 2 : 3, 1, {{0, 0, 3, -1, -1}, {}, {}, {}}};
+        else
+            sp.g[1] = {b, 1, {{0, 1, 3, -1, -1}, {}, {}, {}}};
         launchSynth<T>(sp, s);
         launchYfft<T>(H, G, 4, +1, s, "yfft_inv", true, false);
         const size_t lds = static_cast<size_t>(2) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
@@ -1571,6 +1606,43 @@ wv_status wv_create(const wv_config* config, wv_precision precision, int device,
         if (device < 0 || device >= count) return fail(WV_INVALID_CONFIGURATION, "Invalid HIP device index.");
         std::unique_ptr<wv_kernel> k(new wv_kernel());
         k->impl.setup(*config, precision, device);
+        *kernel = k.release();
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_create_hydrostatic(const wv_config* config, const wv_vertical_operators* ops, wv_precision precision,
+                                int device, wv_kernel** kernel) {
+    if (!config || !ops || !kernel)
+        return fail(WV_INVALID_POINTER, "wv_create_hydrostatic requires a configuration, operators and an output handle.");
+    *kernel = nullptr;
+    if (precision != WV_PRECISION_FP64 && precision != WV_PRECISION_MIXED)
+        return fail(WV_INVALID_CONFIGURATION, "Unknown precision.");
+    for (const double* p : {ops->z, ops->N2, ops->dLnN2, ops->h, ops->P, ops->Q, ops->PFinv, ops->PF, ops->QGinv, ops->QG})
+        if (!p) return fail(WV_INVALID_POINTER, "Vertical operator pointers must be non-null.");
+    if (!config->isHydrostatic)
+        return fail(WV_INVALID_CONFIGURATION, "Arbitrary stratification is supported for the hydrostatic transform only.");
+    if (config->Nz < 3 || config->Nj < 1 || config->Nj > config->Nz - 1)
+        return fail(WV_INVALID_CONFIGURATION, "Nz must be at least 3 and Nj must lie in [1,Nz-1].");
+    return guarded([&] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+            return fail(WV_UNSUPPORTED_OPERATION, "No HIP device is available.");
+        if (device < 0 || device >= count) return fail(WV_INVALID_CONFIGURATION, "Invalid HIP device index.");
+        const size_t nz = config->Nz, nj = config->Nj;
+        wvamd::Stratification st;
+        st.z.assign(ops->z, ops->z + nz);
+        st.N2.assign(ops->N2, ops->N2 + nz);
+        st.dLnN2.assign(ops->dLnN2, ops->dLnN2 + nz);
+        st.h.assign(ops->h, ops->h + nj);
+        st.P.assign(ops->P, ops->P + nj);
+        st.Q.assign(ops->Q, ops->Q + nj);
+        st.PFinv.assign(ops->PFinv, ops->PFinv + nz * nj);
+        st.QGinv.assign(ops->QGinv, ops->QGinv + nz * nj);
+        st.PF.assign(ops->PF, ops->PF + nj * nz);
+        st.QG.assign(ops->QG, ops->QG + nj * nz);
+        std::unique_ptr<wv_kernel> k(new wv_kernel());
+        k->impl.setup(*config, precision, device, &st);
         *kernel = k.release();
         return WV_SUCCESS;
     });

@@ -96,8 +96,13 @@ EXPORTED_SYMBOLS = [
     "wv_transform_to_spatial_domain_with_f_all_derivatives_device",
     "wv_transform_to_spatial_domain_with_g_all_derivatives_device",
     "wv_rk4_advance_device", "wv_total_energy_device", "wv_set_stage_instrumentation", "wv_stage_timings",
-    "wv_reset_stage_timings",
+    "wv_reset_stage_timings", "wv_create_hydrostatic",
 ]
+
+
+class VerticalOperators(C.Structure):
+    """wv_vertical_operators: pointers to the arrays of an arbitrary stratification."""
+    _fields_ = [(name, C.c_void_p) for name in ("z", "N2", "dLnN2", "h", "P", "Q", "PFinv", "PF", "QGinv", "QG")]
 
 
 def library() -> C.CDLL:
@@ -111,6 +116,8 @@ def library() -> C.CDLL:
     vp, dp, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)
     lib.wv_create.restype = C.c_int
     lib.wv_create.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.POINTER(vp)]
+    lib.wv_create_hydrostatic.restype = C.c_int
+    lib.wv_create_hydrostatic.argtypes = [C.POINTER(Config), C.POINTER(VerticalOperators), C.c_int, C.c_int, C.POINTER(vp)]
     lib.wv_destroy.argtypes = [vp]
     lib.wv_last_error.restype = C.c_char_p
     lib.wv_contract_version.restype = C.c_uint32
@@ -178,8 +185,15 @@ class WVTransformConstantStratificationKernel:
         self.precision = precision
         self.device = int(device)
         h = C.c_void_p()
-        _check(lib.wv_create(C.byref(_config_struct(config)), PRECISIONS[precision], self.device, C.byref(h)))
+        _check(self._create(lib, config, precision, h))
         self._h = h
+        self._finish(lib, config)
+
+    def _create(self, lib, config, precision, h):
+        return lib.wv_create(C.byref(_config_struct(config)), PRECISIONS[precision], self.device, C.byref(h))
+
+    def _finish(self, lib, config):
+        h = self._h
         self.Nkl = int(lib.wv_nkl(h))
         self.Nj = int(config["Nj"])
         self.M = self.Nkl * self.Nj
@@ -354,3 +368,36 @@ class WVTransformConstantStratificationKernel:
 
     def reset_stage_timings(self):
         library().wv_reset_stage_timings(self._h)
+
+
+class WVTransformHydrostaticKernel(WVTransformConstantStratificationKernel):
+    """Hydrostatic wave-vortex kernel of an arbitrary stratification N^2(z)
+    (the reference's WVTransformHydrostatic), built from the vertical
+    operators of wvamd.stratification.vertical_operators (or any dict with
+    the same MATLAB-named arrays).  Same methods as the constant-N kernel;
+    the fields of the inverse transform are (u, v, w, eta) and the forward
+    transform takes (u, v, eta)."""
+
+    _OPS = ("z", "N2", "dLnN2", "h", "P", "Q", "PFinv", "PF", "QGinv", "QG")
+
+    def __init__(self, config: dict, operators: dict, precision: str = "fp64", device: int = 0):
+        cfg = dict(config)
+        if not cfg.get("isHydrostatic"):
+            raise ValueError("the arbitrary-stratification transform is hydrostatic: set isHydrostatic")
+        Nz, Nj = int(cfg["Nz"]), int(cfg["Nj"])
+        shapes = {"z": (Nz,), "N2": (Nz,), "dLnN2": (Nz,), "h": (Nj,), "P": (Nj,), "Q": (Nj,),
+                  "PFinv": (Nz, Nj), "PF": (Nj, Nz), "QGinv": (Nz, Nj), "QG": (Nj, Nz)}
+        # MATLAB column-major layout
+        self._ops_arrays = {}
+        for name in self._OPS:
+            a = np.asarray(operators[name], dtype=np.float64)
+            if a.shape != shapes[name]:
+                raise ValueError(f"operator {name} has shape {a.shape}, expected {shapes[name]}")
+            self._ops_arrays[name] = np.asfortranarray(a).ravel(order="F").copy()
+        self.operators = {k: np.asarray(operators[k]) for k in self._OPS}
+        super().__init__(cfg, precision, device)
+
+    def _create(self, lib, config, precision, h):
+        ops = VerticalOperators(*[self._ops_arrays[n].ctypes.data for n in self._OPS])
+        return lib.wv_create_hydrostatic(C.byref(_config_struct(config)), C.byref(ops), PRECISIONS[precision],
+                                         self.device, C.byref(h))
