@@ -187,6 +187,39 @@ wv_status wv_rk4_advance_device(wv_kernel* kernel, double* Ap, double* Am, doubl
 wv_status wv_total_energy_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0,
                                  double* energy, void* stream);
 
+/* --- adjoint (vector-Jacobian products) ---------------------------------------
+ * No counterpart in the reference, which has no adjoint.  Cotangents use the
+ * real inner product <a, b> = Re sum conj(a) b over the coefficient arrays (and
+ * sum a b over real fields), so that <J dy, L> = <dy, J^T L>; the gradient of a
+ * real cost with respect to a complex coefficient is dJ/dRe + i dJ/dIm.  The
+ * first adjoint call on a handle allocates the adjoint workspace (transposed
+ * operators, three spectral buffers and seven coefficient triples). */
+
+/* (Gp, Gm, G0) = J^T (Lp, Lm, L0), J the Jacobian of nonlinearFlux at (t, state). */
+wv_status wv_nonlinear_flux_vjp_device(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                                       const double* A0, const double* Lp, const double* Lm, const double* L0,
+                                       double* Gp, double* Gm, double* G0, void* stream);
+
+/* Transpose of transformWaveVortexToUVWEta: field cotangents [Nx,Ny,Nz,4] ->
+ * coefficient cotangents. */
+wv_status wv_transform_wave_vortex_to_uvweta_vjp_device(wv_kernel* kernel, const double* fieldsBar, double t,
+                                                        double t0, double* Gp, double* Gm, double* G0, void* stream);
+
+/* One RK4 step of size h from the state (Ap, Am, A0) at time t: on entry
+ * (Lp, Lm, L0) is the cotangent of the state after the step, on return that of
+ * the state before it.  The stages are recomputed from the given state. */
+wv_status wv_rk4_step_vjp_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0, double t,
+                                 double t0, double h, double* Lp, double* Lm, double* L0, void* stream);
+
+/* nsteps RK4 steps from (Ap, Am, A0) at time t (the state is not modified): on
+ * entry (Lp, Lm, L0) is the cotangent of the final state, on return that of the
+ * initial state.  Forward states are recomputed from checkpoints kept every
+ * `interval` steps; device memory for ceil(nsteps/interval) + interval states
+ * is held for the duration of the call. */
+wv_status wv_rk4_adjoint_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0, double t,
+                                double t0, double h, int nsteps, int interval, double* Lp, double* Lm, double* L0,
+                                void* stream);
+
 /* --- instrumentation (setStageInstrumentation, ...Kernel.hpp:68) ----------- */
 wv_status wv_set_stage_instrumentation(wv_kernel* kernel, int enabled);
 /* Copies up to `capacity` stage records; returns the number of stages. */

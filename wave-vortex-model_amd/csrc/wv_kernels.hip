@@ -80,6 +80,7 @@ constexpr int kSynthMats = 5;
 struct SynthPlan {
     int ngroups;
     double valueScale;
+    int skipCompletion;  // leave the Hermitian-completion columns untouched
     SynthGroup g[kMaxGroups];
 };
 template <typename T>
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
                     const T other = __shfl_xor(v, 1);
                     const T rot = part ? other : -other;  // part of i * (re + i im)
                     const int z = 16 * zt + MF::row(kq, i);
-                    if (z < Nz && colValid) {
+                    if (z < Nz && colValid && !(plan.skipCompletion && (cd.flags & kColCompletion))) {
                         const size_t o = (static_cast<size_t>(z) * Ny + cd.iy) * nkxP + cd.kx;
                         const size_t chs = static_cast<size_t>(Nz) * Ny * nkxP;
                         if (oV >= 0) reinterpret_cast<T*>(&H[oV * chs + o])[part] = v;
@@ -818,7 +819,7 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
 template <typename T>
 __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<T>* __restrict__ P, int nf, int Nx,
                                               int log2Nx, int Ny, int Nz, int nkx, int nkxP,
-                                              const cx<T>* __restrict__ twX) {
+                                              const cx<T>* __restrict__ twX, int adjointWeights) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int pairs = (nf + 1) / 2;
@@ -840,7 +841,9 @@ __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<
         const int q = e / nkx, k = e % nkx;
         const cx<T> Za = buf[q * NxP + ldsIndex(k, 1)];
         const cx<T> Zb = buf[q * NxP + ldsIndex(k == 0 ? 0 : Nx - k, 1)];
-        const T h = T(0.5);
+        // adjointWeights: the transpose of the Hermitian c2r (twice the r2c
+        // for kx > 0; kx = 0 is real either way)
+        const T h = adjointWeights && k > 0 ? T(1) : T(0.5);
         P[(2 * q) * chStride + rowBase + k] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
         if (2 * q + 1 < nf) P[(2 * q + 1) * chStride + rowBase + k] = {(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
     }
@@ -974,6 +977,8 @@ __global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__
     }
     if (threadIdx.x == 0) out[0] = s[0];
 }
+
+#include "wv_adjoint_kernels.hpp"
 
 // ------------------------------------------------------------------- host --
 namespace {
@@ -1281,8 +1286,11 @@ struct KernelImpl {
         });
     }
 
+    // FV -> H with the synthesis matrices unless given other buffers/matrices.
     template <typename T>
-    void launchSynth(const SynthPlan& sp, hipStream_t s) {
+    void launchSynth(const SynthPlan& sp, hipStream_t s, const DeviceBuffer* in = nullptr,
+                     const DeviceBuffer* out = nullptr, const SynthMats<T>* mats = nullptr,
+                     const char* name = "synth") {
         constexpr int ZT = sizeof(T) == 8 ? 2 : 3;  // z tiles per task (FP64 accumulators are twice the size)
         const int ncg = static_cast<int>((ncol + 7) / 8);
         int nout = 0;
@@ -1295,12 +1303,13 @@ struct KernelImpl {
         const int tasks = sp.ngroups * static_cast<int>((d.Zs / 16 + ZT - 1) / ZT);
         int waves = 4;
         while (tasks % waves) --waves;
-        timed("synth", bytes, s, [&] {
-            hipLaunchKernelGGL((k_synth<T, ZT>), dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s, FV.as<cx<T>>(),
-                               cols.as<ColumnDev>(), static_cast<int>(ncol), static_cast<int>(d.M), synthMats<T>(),
-                               static_cast<int>(d.Zs), static_cast<int>(d.Js),
-                               static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
-                               static_cast<int>(d.nkxP), mj.as<T>(), sp, H.as<cx<T>>());
+        timed(name, bytes, s, [&] {
+            hipLaunchKernelGGL((k_synth<T, ZT>), dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s,
+                               in ? in->as<cx<T>>() : FV.as<cx<T>>(), cols.as<ColumnDev>(), static_cast<int>(ncol),
+                               static_cast<int>(d.M), mats ? *mats : synthMats<T>(), static_cast<int>(d.Zs),
+                               static_cast<int>(d.Js), static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj),
+                               static_cast<int>(d.c.Ny), static_cast<int>(d.nkxP), mj.as<T>(), sp,
+                               out ? out->as<cx<T>>() : H.as<cx<T>>());
         });
     }
 
@@ -1440,7 +1449,7 @@ struct KernelImpl {
             hipLaunchKernelGGL(k_xfwd<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
                                lds, s, fields, P.as<cx<T>>(), nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
                                static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
-                               static_cast<int>(d.nkxP), twX.as<cx<T>>());
+                               static_cast<int>(d.nkxP), twX.as<cx<T>>(), 0);
         });
         launchYfft<T>(P, Q, nf, -1, s, "yfft_fwd", false, true);
         const int ncg = static_cast<int>((ncol + 7) / 8);
@@ -1512,6 +1521,315 @@ struct KernelImpl {
         hipCheck(hipMemcpyAsync(&e, scalar.p, sizeof(double), hipMemcpyDeviceToHost, s), "energy copy");
         hipCheck(hipStreamSynchronize(s), "energy sync");
         return e;
+    }
+
+    // ---- adjoint ---------------------------------------------------------------
+    // Buffers and transposed operators of the vector-Jacobian products,
+    // allocated on the first adjoint call (none of the forward entry points
+    // touch them).
+    bool adjointReady = false;
+    DeviceBuffer anaTC, anaTS, synT[kSynthMats], Qs, Gs, colbuf, mainColumn, compColumn;
+    DeviceBuffer rkWork;  // 7 coefficient triples: s2, s3, s4, kbar, acc, tmp, ywork
+
+    static std::vector<double> transposeOp(const std::vector<double>& m, size_t rows, size_t cols, size_t orow,
+                                           size_t ocol, size_t nr, size_t nc) {
+        // out[orow x ocol] with out[c][r] = m[r][c] for r < nr, c < nc
+        std::vector<double> out(orow * ocol, 0.0);
+        for (size_t r = 0; r < nr && r < rows; ++r)
+            for (size_t c = 0; c < nc && c < cols; ++c) out[c * ocol + r] = m[r * cols + c];
+        return out;
+    }
+
+    template <typename T>
+    void setupAdjointTyped() {
+        const size_t Nz = d.c.Nz, Nj = d.c.Nj;
+        // analysis [Ja x Za] -> synthesis layout [Zs x Js]
+        uploadAs<T>(anaTC, mfmaOperand(transposeOp(d.anaC, d.Ja, d.Za, d.Zs, d.Js, Nj, Nz), d.Zs, d.Js));
+        uploadAs<T>(anaTS, mfmaOperand(transposeOp(d.anaS, d.Ja, d.Za, d.Zs, d.Js, Nj, Nz), d.Zs, d.Js));
+        // synthesis [Zs x Js] -> analysis layout [Ja x Za]
+        const std::vector<double>* syn[kSynthMats] = {&d.synC, &d.synS, &d.synGz, &d.synFh, &d.synEz};
+        for (int m = 0; m < kSynthMats; ++m)
+            if (!syn[m]->empty())
+                uploadAs<T>(synT[m], mfmaOperand(transposeOp(*syn[m], d.Zs, d.Js, d.Ja, d.Za, Nz, Nj), d.Ja, d.Za));
+    }
+
+    void ensureAdjoint() {
+        if (adjointReady) return;
+        if (precision == WV_PRECISION_FP64) setupAdjointTyped<double>();
+        else setupAdjointTyped<float>();
+        const size_t plane = static_cast<size_t>(d.c.Ny) * d.nkxP, chField = d.c.Nz * plane;
+        Qs.alloc(static_cast<size_t>(NT) * chField * tsize);
+        Gs.alloc(static_cast<size_t>(NH) * chField * tsize);
+        hipCheck(hipMemset(Qs.p, 0, Qs.bytes), "hipMemset");
+        hipCheck(hipMemset(Gs.p, 0, Gs.bytes), "hipMemset");
+        colbuf.alloc(4 * ncol * d.c.Nj * tsize);
+        std::vector<int> comp(d.Nkl, -1);
+        for (size_t ic = 0; ic < d.columns.size(); ++ic)
+            if (d.columns[ic].flags & kColCompletion) comp[static_cast<size_t>(d.columns[ic].wv)] = static_cast<int>(ic);
+        upload(mainColumn, std::vector<int>(d.wvColumn.begin(), d.wvColumn.end()));
+        upload(compColumn, comp);
+        rkWork.alloc(7 * 3 * d.M * sizeof(cx<double>));
+        hipCheck(hipDeviceSynchronize(), "adjoint setup");
+        adjointReady = true;
+    }
+
+    template <typename T>
+    SynthMats<T> anaTMats() const {
+        return SynthMats<T>{{anaTC.as<T>(), anaTS.as<T>(), nullptr, nullptr, nullptr}};
+    }
+    template <typename T>
+    SynthMats<T> synTMats() const {
+        return SynthMats<T>{{synT[0].as<T>(), synT[1].as<T>(), synT[2].as<T>(), synT[3].as<T>(), synT[4].as<T>()}};
+    }
+
+    // P^T as a synthesis: per-target cotangents of the analysed spectra (in
+    // FV) through the transposed analysis matrices into the main columns of Qs.
+    SynthPlan projAdjPlan() const {
+        SynthPlan sp{};
+        sp.valueScale = 1.0;
+        sp.skipCompletion = 1;
+        sp.ngroups = 2;
+        sp.g[0] = {0, 2, {{0, 0, 0, -1, -1}, {1, 0, 1, -1, -1}, {}, {}}};
+        if (NT == 4) sp.g[1] = {1, 2, {{2, 0, 2, -1, -1}, {3, 0, 3, -1, -1}, {}, {}}};
+        else sp.g[1] = {1, 1, {{2, 0, 2, -1, -1}, {}, {}, {}}};
+        return sp;
+    }
+
+    template <typename T>
+    void launchSynthAdj(const SynthPlan& sp, const DeviceBuffer& Hstar, hipStream_t s) {
+        const int ncg = static_cast<int>((ncol + 7) / 8);
+        const int njt = static_cast<int>(d.Ja / 16);
+        const size_t lds = 4 * d.Za * 8 * sizeof(cx<T>);
+        timed("synthadj", 0.0, s, [&] {
+            hipLaunchKernelGGL(k_synthadj<T>, dim3(static_cast<unsigned>(ncg)), dim3(static_cast<unsigned>(64 * njt)),
+                               lds, s, Hstar.as<cx<T>>(), cols.as<ColumnDev>(), static_cast<int>(ncol),
+                               synTMats<T>(), static_cast<int>(d.Ja), static_cast<int>(d.Za),
+                               static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
+                               static_cast<int>(d.nkxP), mj.as<T>(), sp, colbuf.as<cx<T>>());
+        });
+    }
+
+    template <typename T>
+    void launchFieldsAdj(double elapsed, int mode, double* Gp, double* Gm, double* G0, hipStream_t s) {
+        const int M = static_cast<int>(d.M);
+        timed("fieldsadj", 0.0, s, [&] {
+            hipLaunchKernelGGL(k_fieldsadj<T>, dim3((M + 255) / 256), dim3(256), 0, s, colbuf.as<cx<T>>(),
+                               static_cast<int>(ncol), mainColumn.as<int>(), compColumn.as<int>(), omega.as<double>(),
+                               UAp.as<cx<double>>(), VAp.as<cx<double>>(), WAp.as<cx<double>>(), NAp.as<double>(),
+                               UA0.as<cx<double>>(), VA0.as<cx<double>>(), NA0.as<double>(), elapsed,
+                               static_cast<int>(d.c.Nj), M, mode, reinterpret_cast<cx<double>*>(Gp),
+                               reinterpret_cast<cx<double>*>(Gm), reinterpret_cast<cx<double>*>(G0));
+        });
+    }
+
+    // (Gp, Gm, G0) = J(y)^T (Lp, Lm, L0) for the flux at time t.
+    template <typename T>
+    void fluxVjp(const double* Ap, const double* Am, const double* A0, double t, double t0, const double* Lp,
+                 const double* Lm, const double* L0, double* Gp, double* Gm, double* G0, hipStream_t s) {
+        const double elapsed = t - t0;
+        // forward spectra at y (as rhs)
+        launchFields<T>(Ap, Am, A0, elapsed, s);
+        launchSynth<T>(fluxPlan(), s);
+        launchYfft<T>(H, G, NH, +1, s, "yfft_inv", true, false);
+        // P^T
+        const int M = static_cast<int>(d.M);
+        ProjTables tb{omega.as<double>(), A0Vort.as<double>(), A0Buoy.as<double>(), NA0.as<double>(),
+                      ApmN.as<double>(), ApmDScaled.as<double>(), ApmD.as<cx<double>>(), apmW.as<double>(),
+                      inertialScale.as<double>()};
+        const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
+        const int hydro = d.c.isHydrostatic ? 1 : 0;
+        timed("projadj", 0.0, s, [&] {
+            auto* lp = reinterpret_cast<const cx<double>*>(Lp);
+            auto* lm = reinterpret_cast<const cx<double>*>(Lm);
+            auto* l0 = reinterpret_cast<const cx<double>*>(L0);
+            if (NT == 4)
+                hipLaunchKernelGGL((k_projadj<T, 4>), dim3((M + 255) / 256), dim3(256), 0, s, lp, lm, l0,
+                                   cols.as<ColumnDev>(), mainColumn.as<int>(), static_cast<int>(d.c.Nj), M,
+                                   targetCode, hydro, tb, elapsed, hscale, FV.as<cx<T>>());
+            else
+                hipLaunchKernelGGL((k_projadj<T, 3>), dim3((M + 255) / 256), dim3(256), 0, s, lp, lm, l0,
+                                   cols.as<ColumnDev>(), mainColumn.as<int>(), static_cast<int>(d.c.Nj), M,
+                                   targetCode, hydro, tb, elapsed, hscale, FV.as<cx<T>>());
+        });
+        const SynthMats<T> at = anaTMats<T>();
+        launchSynth<T>(projAdjPlan(), s, &FV, &Qs, &at, "synth_projadj");
+        launchYfft<T>(Qs, P, NT, +1, s, "yfft_projadj", true, false);
+        // products transposed along x
+        const int Nx = static_cast<int>(d.c.Nx), l2 = log2Exact(d.c.Nx), Ny = static_cast<int>(d.c.Ny),
+                  Nz = static_cast<int>(d.c.Nz), nkx = static_cast<int>(d.nkx), nkxP = static_cast<int>(d.nkxP);
+        const int pairs = NT == 4 ? XMap<4>::pairs : XMap<3>::pairs;
+        const size_t lds = (static_cast<size_t>(Nx) + static_cast<size_t>(pairs + 2) * xfusedStride<T>(Nx)) *
+                           sizeof(cx<T>);
+        timed("xadj", 0.0, s, [&] {
+            const dim3 grid(static_cast<unsigned>(Ny * Nz));
+            if (NT == 4)
+                hipLaunchKernelGGL((k_xadj<T, 4>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
+                                   Gs.as<cx<T>>(), Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>(), kxv.as<T>());
+            else
+                hipLaunchKernelGGL((k_xadj<T, 3>), grid, dim3(256), lds, s, G.as<cx<T>>(), P.as<cx<T>>(),
+                                   Gs.as<cx<T>>(), Nx, l2, Ny, Nz, nkx, nkxP, twX.as<cx<T>>(), kxv.as<T>());
+        });
+        launchYfft<T>(Gs, G, NH, -1, s, "yfft_adj", false, true);
+        launchSynthAdj<T>(fluxPlan(), G, s);
+        launchFieldsAdj<T>(elapsed, 0, Gp, Gm, G0, s);
+    }
+
+    // Transpose of transformWaveVortexToUVWEta: cotangent fields [4][Nz][Ny][Nx]
+    // (u, v, w, eta) -> coefficient cotangents.
+    template <typename T>
+    void inverseVjp(const double* fieldsBar, double t, double t0, double* Gp, double* Gm, double* G0, hipStream_t s) {
+        const int nf = 4;
+        const int pairs = 2;
+        const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
+        timed("xfwd_adj", 0.0, s, [&] {
+            hipLaunchKernelGGL(k_xfwd<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
+                               lds, s, fieldsBar, Gs.as<cx<T>>(), nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
+                               static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
+                               static_cast<int>(d.nkxP), twX.as<cx<T>>(), 1);
+        });
+        launchYfft<T>(Gs, G, nf, -1, s, "yfft_adj", false, true);
+        SynthPlan sp{};
+        sp.valueScale = d.stratified ? 1.0 : 0.5;
+        sp.ngroups = 2;
+        sp.g[0] = {0, 2, {{0, 0, 0, -1, -1}, {1, 0, 1, -1, -1}, {}, {}}};
+        sp.g[1] = {1, 2, {{2, 0, 2, -1, -1}, {3, 0, 3, -1, -1}, {}, {}}};
+        launchSynthAdj<T>(sp, G, s);
+        launchFieldsAdj<T>(t - t0, 0, Gp, Gm, G0, s);
+    }
+
+    void fluxVjpAny(const double* Ap, const double* Am, const double* A0, double t, double t0, const double* Lp,
+                    const double* Lm, const double* L0, double* Gp, double* Gm, double* G0, hipStream_t s) {
+        ensureAdjoint();
+        if (precision == WV_PRECISION_FP64) fluxVjp<double>(Ap, Am, A0, t, t0, Lp, Lm, L0, Gp, Gm, G0, s);
+        else fluxVjp<float>(Ap, Am, A0, t, t0, Lp, Lm, L0, Gp, Gm, G0, s);
+    }
+
+    // out_a = alpha x_a + beta y_a over the coefficient triples (pointers to
+    // the first of three consecutive arrays of M complex doubles each).
+    void axpby(const cx<double>* x, const cx<double>* y, double alpha, double beta, cx<double>* out, hipStream_t s) {
+        const int M = static_cast<int>(d.M);
+        hipLaunchKernelGGL(k_axpby3, dim3((M + 255) / 256), dim3(256), 0, s, x, x + M, x + 2 * M, y, y + M, y + 2 * M,
+                           alpha, beta, M, out, out + M, out + 2 * M);
+        hipCheck(hipGetLastError(), "axpby");
+    }
+
+    void fluxInto(const cx<double>* y, double t, double t0, cx<double>* out, hipStream_t s) {
+        const size_t M = d.M;
+        Epilogue ep{};
+        ep.mode = kFluxOut;
+        ep.out0 = out;
+        ep.out1 = out + M;
+        ep.out2 = out + 2 * M;
+        const double* yd = reinterpret_cast<const double*>(y);
+        rhsAny(yd, yd + 2 * M, yd + 4 * M, t, t0, ep, s);
+    }
+
+    void vjpInto(const cx<double>* y, double t, double t0, const cx<double>* lam, cx<double>* out, hipStream_t s) {
+        const size_t M = d.M;
+        const double* yd = reinterpret_cast<const double*>(y);
+        const double* ld = reinterpret_cast<const double*>(lam);
+        double* od = reinterpret_cast<double*>(out);
+        fluxVjpAny(yd, yd + 2 * M, yd + 4 * M, t, t0, ld, ld + 2 * M, ld + 4 * M, od, od + 2 * M, od + 4 * M, s);
+    }
+
+    // One classic RK4 step's transpose: ybar <- (d y_{n+1} / d y_n)^T ybar, the
+    // stages recomputed from y_n exactly as the forward step forms them.
+    void rk4StepVjp(const cx<double>* y, double t, double t0, double h, cx<double>* ybar, hipStream_t s) {
+        ensureAdjoint();
+        const size_t M3 = 3 * d.M;
+        cx<double>* w = rkWork.as<cx<double>>();
+        cx<double>* s2 = w;
+        cx<double>* s3 = w + M3;
+        cx<double>* s4 = w + 2 * M3;
+        cx<double>* kb = w + 3 * M3;
+        cx<double>* acc = w + 4 * M3;
+        cx<double>* tmp = w + 5 * M3;
+        fluxInto(y, t, t0, tmp, s);
+        axpby(y, tmp, 1.0, 0.5 * h, s2, s);
+        fluxInto(s2, t + 0.5 * h, t0, tmp, s);
+        axpby(y, tmp, 1.0, 0.5 * h, s3, s);
+        fluxInto(s3, t + 0.5 * h, t0, tmp, s);
+        axpby(y, tmp, 1.0, h, s4, s);
+        axpby(ybar, ybar, h / 6.0, 0.0, kb, s);  // kbar4
+        vjpInto(s4, t + h, t0, kb, tmp, s);
+        axpby(ybar, tmp, 1.0, 1.0, acc, s);
+        axpby(ybar, tmp, h / 3.0, h, kb, s);  // kbar3
+        vjpInto(s3, t + 0.5 * h, t0, kb, tmp, s);
+        axpby(acc, tmp, 1.0, 1.0, acc, s);
+        axpby(ybar, tmp, h / 3.0, 0.5 * h, kb, s);  // kbar2
+        vjpInto(s2, t + 0.5 * h, t0, kb, tmp, s);
+        axpby(acc, tmp, 1.0, 1.0, acc, s);
+        axpby(ybar, tmp, h / 6.0, 0.5 * h, kb, s);  // kbar1
+        vjpInto(y, t, t0, kb, tmp, s);
+        axpby(acc, tmp, 1.0, 1.0, ybar, s);
+    }
+
+    // One forward RK4 step in place (the arithmetic of wv_rk4_advance_device).
+    void rk4Step(cx<double>* y, double t, double t0, double h, hipStream_t s) {
+        const size_t M = d.M;
+        cx<double>* st = stage.as<cx<double>>();
+        cx<double>* w = weighted.as<cx<double>>();
+        Epilogue ep{};
+        ep.y0 = y;
+        ep.y1 = y + M;
+        ep.y2 = y + 2 * M;
+        ep.st0 = st;
+        ep.st1 = st + M;
+        ep.st2 = st + 2 * M;
+        ep.w0 = w;
+        ep.w1 = w + M;
+        ep.w2 = w + 2 * M;
+        const double* yd = reinterpret_cast<const double*>(y);
+        const double* sd = reinterpret_cast<const double*>(st);
+        ep.mode = kRk1;
+        ep.scale = 0.5 * h;
+        rhsAny(yd, yd + 2 * M, yd + 4 * M, t, t0, ep, s);
+        ep.mode = kRk2;
+        rhsAny(sd, sd + 2 * M, sd + 4 * M, t + 0.5 * h, t0, ep, s);
+        ep.mode = kRk3;
+        ep.scale = h;
+        rhsAny(sd, sd + 2 * M, sd + 4 * M, t + 0.5 * h, t0, ep, s);
+        ep.mode = kRk4;
+        ep.scale = h / 6.0;
+        rhsAny(sd, sd + 2 * M, sd + 4 * M, t + h, t0, ep, s);
+    }
+
+    // Transpose of nsteps RK4 steps from y0 at time t: ybar (cotangent of the
+    // final state) <- cotangent of y0.  The forward states are recomputed
+    // from checkpoints stored every `interval` steps (held on the device only
+    // for the duration of the call).
+    void rk4Adjoint(const cx<double>* y0, double t, double t0, double h, int nsteps, int interval, cx<double>* ybar,
+                    hipStream_t s) {
+        ensureAdjoint();
+        const size_t M3 = 3 * d.M, bytes = M3 * sizeof(cx<double>);
+        const int ncp = (nsteps + interval - 1) / interval;
+        DeviceBuffer cp, seg;
+        cp.alloc(static_cast<size_t>(std::max(ncp, 1)) * bytes);
+        seg.alloc(static_cast<size_t>(interval) * bytes);
+        cx<double>* yw = rkWork.as<cx<double>>() + 6 * M3;
+        // step start times accumulated as wv_rk4_advance_device accumulates them
+        std::vector<double> tn(static_cast<size_t>(nsteps) + 1, t);
+        for (int n = 0; n < nsteps; ++n) tn[n + 1] = tn[n] + h;
+        hipCheck(hipMemcpyAsync(yw, y0, bytes, hipMemcpyDeviceToDevice, s), "copy");
+        for (int n = 0; n < nsteps; ++n) {
+            if (n % interval == 0)
+                hipCheck(hipMemcpyAsync(cp.as<cx<double>>() + static_cast<size_t>(n / interval) * M3, yw, bytes,
+                                        hipMemcpyDeviceToDevice, s), "checkpoint");
+            if (n + 1 < nsteps) rk4Step(yw, tn[n], t0, h, s);  // the final state is not needed
+        }
+        for (int c = ncp - 1; c >= 0; --c) {
+            const int n0 = c * interval, n1 = std::min(nsteps, n0 + interval);
+            hipCheck(hipMemcpyAsync(yw, cp.as<cx<double>>() + static_cast<size_t>(c) * M3, bytes,
+                                    hipMemcpyDeviceToDevice, s), "restore");
+            for (int n = n0; n < n1; ++n) {
+                hipCheck(hipMemcpyAsync(seg.as<cx<double>>() + static_cast<size_t>(n - n0) * M3, yw, bytes,
+                                        hipMemcpyDeviceToDevice, s), "segment");
+                if (n + 1 < n1) rk4Step(yw, tn[n], t0, h, s);
+            }
+            for (int n = n1 - 1; n >= n0; --n)
+                rk4StepVjp(seg.as<cx<double>>() + static_cast<size_t>(n - n0) * M3, tn[n], t0, h, ybar, s);
+        }
+        hipCheck(hipStreamSynchronize(s), "adjoint");
     }
 };
 
@@ -1882,6 +2200,104 @@ wv_status wv_rk4_advance_device(wv_kernel* kernel, double* Ap, double* Am, doubl
             k.rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + h, t0, ep, S(stream));
             *t = ti + h;
         }
+        return WV_SUCCESS;
+    });
+}
+
+namespace {
+// The three coefficient arrays of a state or cotangent must be distinct.
+bool distinct3(const void* a, const void* b, const void* c) { return a != b && a != c && b != c; }
+}  // namespace
+
+wv_status wv_nonlinear_flux_vjp_device(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                                       const double* A0, const double* Lp, const double* Lm, const double* L0,
+                                       double* Gp, double* Gm, double* G0, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    if (!Ap || !Am || !A0 || !Lp || !Lm || !L0 || !Gp || !Gm || !G0)
+        return fail(WV_INVALID_POINTER, "Flux adjoint has a null data pointer.");
+    if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "Times must be finite.");
+    for (const double* o : {Gp, Gm, G0})
+        for (const double* i : {Ap, Am, A0, Lp, Lm, L0})
+            if (o == i) return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs may not alias the inputs.");
+    if (!distinct3(Gp, Gm, G0)) return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs must be distinct.");
+    KernelImpl& k = kernel->impl;
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        k.fluxVjpAny(Ap, Am, A0, t, t0, Lp, Lm, L0, Gp, Gm, G0, S(stream));
+        return WV_SUCCESS;
+    });
+}
+
+wv_status wv_transform_wave_vortex_to_uvweta_vjp_device(wv_kernel* kernel, const double* fieldsBar, double t,
+                                                        double t0, double* Gp, double* Gm, double* G0, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    if (!fieldsBar || !Gp || !Gm || !G0) return fail(WV_INVALID_POINTER, "Transform adjoint has a null data pointer.");
+    if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "Times must be finite.");
+    if (!distinct3(Gp, Gm, G0)) return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs must be distinct.");
+    KernelImpl& k = kernel->impl;
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        k.ensureAdjoint();
+        if (k.precision == WV_PRECISION_FP64) k.inverseVjp<double>(fieldsBar, t, t0, Gp, Gm, G0, S(stream));
+        else k.inverseVjp<float>(fieldsBar, t, t0, Gp, Gm, G0, S(stream));
+        return WV_SUCCESS;
+    });
+}
+
+namespace {
+// Copies a state or cotangent given as three arrays into one packed triple.
+void packTriple(const double* a, const double* b, const double* c, wvamd::cx<double>* dst, size_t M, hipStream_t s) {
+    const size_t bytes = M * sizeof(wvamd::cx<double>);
+    hipCheck(hipMemcpyAsync(dst, a, bytes, hipMemcpyDeviceToDevice, s), "pack");
+    hipCheck(hipMemcpyAsync(dst + M, b, bytes, hipMemcpyDeviceToDevice, s), "pack");
+    hipCheck(hipMemcpyAsync(dst + 2 * M, c, bytes, hipMemcpyDeviceToDevice, s), "pack");
+}
+void unpackTriple(const wvamd::cx<double>* src, double* a, double* b, double* c, size_t M, hipStream_t s) {
+    const size_t bytes = M * sizeof(wvamd::cx<double>);
+    hipCheck(hipMemcpyAsync(a, src, bytes, hipMemcpyDeviceToDevice, s), "unpack");
+    hipCheck(hipMemcpyAsync(b, src + M, bytes, hipMemcpyDeviceToDevice, s), "unpack");
+    hipCheck(hipMemcpyAsync(c, src + 2 * M, bytes, hipMemcpyDeviceToDevice, s), "unpack");
+}
+}  // namespace
+
+wv_status wv_rk4_step_vjp_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0, double t,
+                                 double t0, double h, double* Lp, double* Lm, double* L0, void* stream) {
+    return wv_rk4_adjoint_device(kernel, Ap, Am, A0, t, t0, h, 1, 1, Lp, Lm, L0, stream);
+}
+
+wv_status wv_rk4_adjoint_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0, double t,
+                                double t0, double h, int nsteps, int interval, double* Lp, double* Lm, double* L0,
+                                void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    if (!Ap || !Am || !A0 || !Lp || !Lm || !L0) return fail(WV_INVALID_POINTER, "RK4 adjoint has a null data pointer.");
+    if (!(h > 0.0) || !std::isfinite(h) || !std::isfinite(t) || !std::isfinite(t0))
+        return fail(WV_INVALID_CONFIGURATION, "RK4 step size must be finite and positive, times finite.");
+    if (nsteps < 0 || interval < 1) return fail(WV_INVALID_CONFIGURATION, "Step count must be nonnegative, interval positive.");
+    if (!distinct3(Lp, Lm, L0)) return fail(WV_OVERLAPPING_ARRAYS, "Cotangent arrays must be distinct.");
+    for (const double* o : {Lp, Lm, L0})
+        for (const double* i : {Ap, Am, A0})
+            if (o == i) return fail(WV_OVERLAPPING_ARRAYS, "Cotangents may not alias the state.");
+    KernelImpl& k = kernel->impl;
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        using wvamd::cx;
+        k.ensureAdjoint();
+        const size_t M = k.d.M;
+        const hipStream_t s = S(stream);
+        wvamd::DeviceBuffer y0, ybar;
+        y0.alloc(3 * M * sizeof(cx<double>));
+        ybar.alloc(3 * M * sizeof(cx<double>));
+        packTriple(Ap, Am, A0, y0.as<cx<double>>(), M, s);
+        packTriple(Lp, Lm, L0, ybar.as<cx<double>>(), M, s);
+        k.rk4Adjoint(y0.as<cx<double>>(), t, t0, h, nsteps, interval, ybar.as<cx<double>>(), s);
+        unpackTriple(ybar.as<cx<double>>(), Lp, Lm, L0, M, s);
+        hipCheck(hipStreamSynchronize(s), "adjoint");
         return WV_SUCCESS;
     });
 }

@@ -96,7 +96,8 @@ EXPORTED_SYMBOLS = [
     "wv_transform_to_spatial_domain_with_f_all_derivatives_device",
     "wv_transform_to_spatial_domain_with_g_all_derivatives_device",
     "wv_rk4_advance_device", "wv_total_energy_device", "wv_set_stage_instrumentation", "wv_stage_timings",
-    "wv_reset_stage_timings", "wv_create_hydrostatic",
+    "wv_reset_stage_timings", "wv_create_hydrostatic", "wv_nonlinear_flux_vjp_device",
+    "wv_transform_wave_vortex_to_uvweta_vjp_device", "wv_rk4_step_vjp_device", "wv_rk4_adjoint_device",
 ]
 
 
@@ -157,6 +158,15 @@ def library() -> C.CDLL:
     lib.wv_stage_timings.restype = C.c_int
     lib.wv_stage_timings.argtypes = [vp, C.POINTER(StageTiming), C.c_int]
     lib.wv_reset_stage_timings.argtypes = [vp]
+    lib.wv_nonlinear_flux_vjp_device.restype = C.c_int
+    lib.wv_nonlinear_flux_vjp_device.argtypes = [vp, C.c_double, C.c_double] + [vp] * 9 + [vp]
+    lib.wv_transform_wave_vortex_to_uvweta_vjp_device.restype = C.c_int
+    lib.wv_transform_wave_vortex_to_uvweta_vjp_device.argtypes = [vp, vp, C.c_double, C.c_double, vp, vp, vp, vp]
+    lib.wv_rk4_step_vjp_device.restype = C.c_int
+    lib.wv_rk4_step_vjp_device.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp]
+    lib.wv_rk4_adjoint_device.restype = C.c_int
+    lib.wv_rk4_adjoint_device.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                          vp, vp, vp, vp]
     _lib = lib
     return lib
 
@@ -354,6 +364,42 @@ class WVTransformConstantStratificationKernel:
         return e.value
 
     # ----------------------------------------------------- instrumentation --
+    # --------------------------------------------------------------- adjoint --
+    # Cotangents follow <a, b> = Re sum conj(a) b; the gradient of a real cost
+    # with respect to a complex coefficient is dJ/dRe + i dJ/dIm.
+    def flux_vjp(self, Ap, Am, A0, Lp, Lm, L0, t=0.0, t0=0.0, out=None, stream=None):
+        """J(y)^T (Lp, Lm, L0) for the nonlinear flux at time t."""
+        if out is None:
+            out = self.empty_coefficients()
+        _check(library().wv_nonlinear_flux_vjp_device(
+            self._h, float(t), float(t0), *[self._coef(a) for a in (Ap, Am, A0, Lp, Lm, L0)],
+            *[self._coef(o) for o in out], self._stream(stream)))
+        return out
+
+    def transform_wave_vortex_to_uvweta_vjp(self, fields_bar, t=0.0, t0=0.0, out=None, stream=None):
+        """Transpose of transform_wave_vortex_to_uvweta: (4, Nz, Ny, Nx) field
+        cotangents -> coefficient cotangents."""
+        if out is None:
+            out = self.empty_coefficients()
+        shape = (4,) + self.spatial_shape
+        _check(library().wv_transform_wave_vortex_to_uvweta_vjp_device(
+            self._h, self._dev(fields_bar, shape, "torch.float64"), float(t), float(t0),
+            *[self._coef(o) for o in out], self._stream(stream)))
+        return out
+
+    def rk4_step_vjp(self, Ap, Am, A0, t, h, Lp, Lm, L0, t0=0.0, stream=None):
+        """In place: (Lp, Lm, L0) <- cotangent before one RK4 step from (Ap, Am, A0) at t."""
+        _check(library().wv_rk4_step_vjp_device(
+            self._h, *[self._coef(a) for a in (Ap, Am, A0)], float(t), float(t0), float(h),
+            *[self._coef(a) for a in (Lp, Lm, L0)], self._stream(stream)))
+
+    def rk4_adjoint(self, Ap, Am, A0, t, h, nsteps, interval, Lp, Lm, L0, t0=0.0, stream=None):
+        """In place: (Lp, Lm, L0) <- cotangent of the state at t, given that of
+        the state after nsteps RK4 steps; checkpoints every `interval` steps."""
+        _check(library().wv_rk4_adjoint_device(
+            self._h, *[self._coef(a) for a in (Ap, Am, A0)], float(t), float(t0), float(h), int(nsteps),
+            int(interval), *[self._coef(a) for a in (Lp, Lm, L0)], self._stream(stream)))
+
     def set_stage_instrumentation(self, enabled: bool):
         _check(library().wv_set_stage_instrumentation(self._h, int(bool(enabled))))
 
