@@ -71,9 +71,19 @@ struct Mfma<double> {
 // for the remaining factor), each butterfly an in-register DFT, with all index
 // arithmetic in shifts and masks; other N use a direct DFT.  A pass holds at
 // most 16 complex values per thread, so the caller must launch at least
-// nb*N/16 threads.  The caller synchronizes before the call; the function
+// nb*N/16 threads.  Every workgroup that runs these transforms has
+// kFftThreads threads.  The caller synchronizes before the call; the function
 // returns after a final barrier.
 constexpr int kFftRegs = 16;
+constexpr int kFftThreads = 256;
+
+// threadIdx.x of a kFftThreads-thread workgroup, with its range known to the
+// compiler (items of a pass that every thread holds need no guard)
+__device__ __forceinline__ int fftThread() {
+    const int t = static_cast<int>(threadIdx.x);
+    __builtin_assume(t >= 0 && t < kFftThreads);
+    return t;
+}
 
 __device__ __forceinline__ int ldsIndex(int n, int sn) { return sn == 1 ? n + (n >> 4) : n * sn; }
 // padded length of a unit-stride transform
@@ -82,7 +92,10 @@ __host__ __device__ constexpr int ldsLength(int N) { return N + (N >> 4); }
 template <int NB1>
 struct BatchRows {
     int nb, sb, sb2;
-    __device__ __forceinline__ int base(int b) const { return (b % NB1) * sb + (b / NB1) * sb2; }
+    __device__ __forceinline__ int base(int b) const {
+        const unsigned u = static_cast<unsigned>(b);
+        return static_cast<int>(u % NB1) * sb + static_cast<int>(u / NB1) * sb2;
+    }
     __device__ __forceinline__ static int at(int n) { return n + (n >> 4); }
     // item -> (transform, position) when each transform holds 2^log2n items
     __device__ __forceinline__ void split(int item, int log2n, int& b, int& j) const {
@@ -176,7 +189,7 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
     cx<T> v[IPT][R];
 #pragma unroll
     for (int it = 0; it < IPT; ++it) {
-        const int item = threadIdx.x + it * blockDim.x;
+        const int item = fftThread() + it * kFftThreads;
         if (item < items) {
             int b, jj;
             bt.split(item, log2NR, b, jj);
@@ -185,9 +198,12 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 cx<T> x = buf[bt.at(jj + r * NR) + boff];
-                if (r > 0 && k > 0) {
+                if (r > 0 && log2Ns > 0) {
+                    // k = 0 takes the unit twiddle by a select (no divergent
+                    // branch); its table read stays in range
                     cx<T> w = tw[(r << log2Ns) + k - 1];
                     if (SIGN > 0) w.im = -w.im;
+                    if (k == 0) w = cx<T>{T(1), T(0)};
                     x = cmul(x, w);
                 }
                 v[it][r] = x;
@@ -198,7 +214,7 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < IPT; ++it) {
-        const int item = threadIdx.x + it * blockDim.x;
+        const int item = fftThread() + it * kFftThreads;
         if (item < items) {
             int b, jj;
             bt.split(item, log2NR, b, jj);
@@ -217,7 +233,7 @@ __device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restri
     const int items = batchCount(bt) * N;
 #pragma unroll
     for (int it = 0; it < kFftRegs; ++it) {
-        const int item = threadIdx.x + it * blockDim.x;
+        const int item = fftThread() + it * kFftThreads;
         if (item < items) {
             int b, k;
             bt.splitN(item, N, b, k);
@@ -233,7 +249,7 @@ __device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restri
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kFftRegs; ++it) {
-        const int item = threadIdx.x + it * blockDim.x;
+        const int item = fftThread() + it * kFftThreads;
         if (item < items) {
             int b, k;
             bt.splitN(item, N, b, k);
@@ -243,9 +259,26 @@ __device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restri
     __syncthreads();
 }
 
-// log2N < 0 marks a length that is not a power of two.
-template <typename T, int SIGN, typename Bt>
+// Radix-16 passes from sub-transform length 2^L2S up to 2^L2N.
+template <typename T, int SIGN, int L2N, int L2S, typename Bt>
+__device__ __forceinline__ void radix16Passes(cx<T>* buf, const Bt& bt, const cx<T>* __restrict__ tw) {
+    if constexpr (L2S < L2N) {
+        stockhamPass<T, 16, SIGN>(buf, L2N, L2S, bt, tw);
+        radix16Passes<T, SIGN, L2N, L2S + 4>(buf, bt, tw);
+    }
+}
+
+// log2N < 0 marks a length that is not a power of two.  L2N > 0 fixes the
+// length 2^L2N at compile time (N and log2N are then ignored): every pass
+// and every index shift is a constant.
+template <typename T, int SIGN, int L2N = 0, typename Bt>
 __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
+    if constexpr (L2N > 0) {
+        constexpr int lead = L2N & 3;
+        if constexpr (lead != 0) stockhamPass<T, 1 << lead, SIGN>(buf, L2N, 0, bt, tw);
+        radix16Passes<T, SIGN, L2N, lead>(buf, bt, tw);
+        return;
+    }
     if (log2N < 0) {
         directDft<T, SIGN>(buf, N, bt, tw);
         return;
@@ -264,7 +297,7 @@ __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& b
 // Copies the N-entry twiddle table into LDS (block-wide; caller synchronizes).
 template <typename T>
 __device__ __forceinline__ void loadTwiddles(cx<T>* dst, const cx<T>* __restrict__ src, int N) {
-    for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+    for (int i = fftThread(); i < N; i += kFftThreads) dst[i] = src[i];
 }
 
 }  // namespace wvamd

@@ -347,11 +347,16 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
 // Complex FFT along y of [Ny, KT] tiles: in/out layout [ch][z][iy][nkxP].
 // rowLimit[tile] (if given) is the largest |ky| retained in the kx tile: input
 // rows r < iy < Ny - r are known zeros (skipIn) or output rows there are not
-// needed (skipOut).
-template <typename T>
-__global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T>* __restrict__ out, int Ny,
-                                              int log2Ny, int nkxP, int log2KT, int Nz, const cx<T>* __restrict__ twY,
-                                              int sign, const int* __restrict__ rowLimit, int skipIn, int skipOut) {
+// needed (skipOut).  L2N > 0 fixes Ny = 2^L2N and the tile width 2^L2KT at
+// compile time (the runtime Ny, log2Ny and log2KT are then ignored).
+template <typename T, int L2N, int L2KT>
+__global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                                                      int NyArg, int log2NyArg, int nkxP, int log2KTArg, int Nz,
+                                                      const cx<T>* __restrict__ twY, int sign,
+                                                      const int* __restrict__ rowLimit, int skipIn, int skipOut) {
+    const int Ny = L2N > 0 ? 1 << L2N : NyArg;
+    const int log2Ny = L2N > 0 ? L2N : log2NyArg;
+    const int log2KT = L2N > 0 ? L2KT : log2KTArg;
     const int KT = 1 << log2KT;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T
         cx<T> v[kMaxPerThread];
 #pragma unroll
         for (int i = 0; i < kMaxPerThread; ++i) {
-            const int e = threadIdx.x + i * blockDim.x;
+            const int e = fftThread() + i * kFftThreads;
             const int ec = e < nel ? e : nel - 1;  // clamped: loads are unconditional
             const int kk = ec & (KT - 1), iy = ec >> log2KT;
             // rows known to be zero read row 0 instead (loaded by other lanes)
@@ -377,7 +382,7 @@ __global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T
         __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
 #pragma unroll
         for (int i = 0; i < kMaxPerThread; ++i) {
-            const int e = threadIdx.x + i * blockDim.x;
+            const int e = fftThread() + i * kFftThreads;
             const int iy = e >> log2KT;
             const bool zero = skipIn && iy > lim && iy < Ny - lim;
             if (e < nel) tile[e] = zero ? cx<T>{T(0), T(0)} : v[i];
@@ -385,10 +390,10 @@ __global__ __launch_bounds__(256) void k_yfft(const cx<T>* __restrict__ in, cx<T
     }
     __syncthreads();
     if (sign > 0)
-        fftLds<T, +1>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
+        fftLds<T, +1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
     else
-        fftLds<T, -1>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
-    for (int e = threadIdx.x; e < nel; e += blockDim.x) {
+        fftLds<T, -1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
+    for (int e = fftThread(); e < nel; e += kFftThreads) {
         const int kk = e & (KT - 1), iy = e >> log2KT;
         if (skipOut && iy > lim && iy < Ny - lim) continue;
         out[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk] = tile[e];
@@ -426,6 +431,16 @@ template <typename T>
 __host__ __device__ constexpr int xfusedStride(int Nx) { return ldsLength(Nx); }
 template <typename T>
 __host__ __device__ constexpr int xfusedRowStride(int Nx, int pairs) { return pairs * xfusedStride<T>(Nx); }
+// LDS bytes of a k_xfused block of RB rows (twiddles + transforms)
+template <typename T>
+__host__ __device__ constexpr size_t xfusedLdsBytes(int Nx, int pairs, int RB) {
+    return (static_cast<size_t>(Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(Nx, pairs)) * sizeof(cx<T>);
+}
+// rows per k_xfused block: two while four blocks still fit a CU's 160 KB of LDS
+template <typename T>
+__host__ __device__ constexpr int xfusedRows(int Nx, int pairs) {
+    return xfusedLdsBytes<T>(Nx, pairs, 2) <= 40 * 1024 ? 2 : 1;
+}
 
 // Spatial channels of the flux products, packed two per complex x transform.
 // Pair p takes half-spectrum channel srcA[p] (real part of the transform) and
@@ -464,14 +479,17 @@ struct XMap<3> {
 
 // Per (y, z) row: c2r of the spatial channels along x (two real transforms
 // per complex FFT), the NT advection products and r2c of the products.
-// kxv[k] = the x wavenumber of half-spectrum column k.
-template <typename T, int NT, int RB>
-__global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P, int Nx,
-                                                int log2Nx, int Ny, int Nz, int nkx, int nkxP,
-                                                const cx<T>* __restrict__ twX, const T* __restrict__ kxv,
-                                                T etaScale) {
+// kxv[k] = the x wavenumber of half-spectrum column k.  L2N > 0 fixes
+// Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored).
+template <typename T, int NT, int RB, int L2N>
+__global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P,
+                                                        int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
+                                                        const cx<T>* __restrict__ twX, const T* __restrict__ kxv,
+                                                        T etaScale) {
     using XM = XMap<NT>;
     constexpr int PAIRS = XM::pairs;
+    const int Nx = L2N > 0 ? 1 << L2N : NxArg;
+    const int log2Nx = L2N > 0 ? L2N : log2NxArg;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NxP = xfusedStride<T>(Nx);
     const int RS = xfusedRowStride<T>(Nx, PAIRS);
@@ -546,8 +564,8 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
         for (int k = nkx + lane; k <= Nx - nkx; k += 64) z[ldsIndex(k, 1)] = cx<T>{T(0), T(0)};
     }
     __syncthreads();
-    fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<PAIRS>{RB * PAIRS, NxP, RS}, tw);
-    for (int e = threadIdx.x; e < RB * Nx; e += blockDim.x) {
+    fftLds<T, +1, L2N>(buf, Nx, log2Nx, BatchRows<PAIRS>{RB * PAIRS, NxP, RS}, tw);
+    for (int e = fftThread(); e < RB * Nx; e += kFftThreads) {
         int rr, x;
         divmodN(e, Nx, log2Nx, rr, x);
         const int xl = ldsIndex(x, 1);
@@ -571,9 +589,9 @@ __global__ __launch_bounds__(256) void k_xfused(const cx<T>* __restrict__ G, cx<
     }
     __syncthreads();
     // forward transforms: pairs 0 and 1 of every row
-    fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
+    fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
     // wave w unpacks (row, pair) rq = w, w + nwaves, ...; its lanes walk k
-    const int nwaves = blockDim.x >> 6;
+    constexpr int nwaves = kFftThreads / 64;
     for (int rq = waveIndex(); rq < RB * 2; rq += nwaves)
     for (int k = threadIdx.x & 63; k < nkx; k += 64) {
         const int q = rq & 1, rr = rq >> 1;
@@ -1321,49 +1339,64 @@ struct KernelImpl {
         const size_t lds = (static_cast<size_t>(KT) + 1) * d.c.Ny * sizeof(cx<T>);
         const double frac = (skipIn || skipOut) ? 0.5 * (1.0 + retainedRowFraction) : 1.0;
         const double bytes = 2.0 * frac * nch * d.c.Nz * d.c.Ny * d.nkxP * sizeof(cx<T>);
+        // Ny = 64 ... 1024 with their default tile widths run kernels
+        // compiled for that length; others the runtime-length kernel.
+        using Fn = void (*)(const cx<T>*, cx<T>*, int, int, int, int, int, const cx<T>*, int, const int*, int, int);
+        const int l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
+        Fn fn = k_yfft<T, 0, 0>;
+        if (l2y == 6 && l2t == 4) fn = k_yfft<T, 6, 4>;
+        if (l2y == 7 && l2t == 4) fn = k_yfft<T, 7, 4>;
+        if (l2y == 8 && l2t == 4) fn = k_yfft<T, 8, 4>;
+        if (l2y == 9 && l2t == 3) fn = k_yfft<T, 9, 3>;
+        if (l2y == 10 && l2t == 2) fn = k_yfft<T, 10, 2>;
         timed(name, bytes, s, [&] {
-            hipLaunchKernelGGL(k_yfft<T>, dim3(static_cast<unsigned>(d.nkxP / KT), static_cast<unsigned>(d.c.Nz), nch),
-                               dim3(256), lds, s, in.as<cx<T>>(), out.as<cx<T>>(), static_cast<int>(d.c.Ny),
-                               log2Exact(d.c.Ny), static_cast<int>(d.nkxP), log2Exact(KT), static_cast<int>(d.c.Nz),
-                               twY.as<cx<T>>(), sign, rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0);
+            hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(d.nkxP / KT), static_cast<unsigned>(d.c.Nz), nch),
+                               dim3(kFftThreads), lds, s, in.as<cx<T>>(), out.as<cx<T>>(), static_cast<int>(d.c.Ny),
+                               l2y, static_cast<int>(d.nkxP), l2t, static_cast<int>(d.c.Nz), twY.as<cx<T>>(), sign,
+                               rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0);
         });
     }
 
-    template <typename T, int RB>
-    size_t xfusedLds() const {
-        const int pairs = NT == 4 ? XMap<4>::pairs : XMap<3>::pairs;
-        return (static_cast<size_t>(d.c.Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(static_cast<int>(d.c.Nx), pairs)) *
-               sizeof(cx<T>);
+    template <typename T, int NTc, int RB, int L2N>
+    void launchXfusedKernel(hipStream_t s) {
+        constexpr int pairs = XMap<NTc>::pairs;
+        const int Nx = static_cast<int>(d.c.Nx), Ny = static_cast<int>(d.c.Ny), Nz = static_cast<int>(d.c.Nz);
+        const int nrows = Ny * Nz;
+        const size_t lds = xfusedLdsBytes<T>(Nx, pairs, RB);
+        if (lds > maxLdsPerBlock) throw Failure{WV_UNSUPPORTED_OPERATION, "x transforms exceed the LDS of a workgroup."};
+        // power of two nearest N0 (exact scaling of the eta product)
+        const T es = static_cast<T>(std::ldexp(1.0, static_cast<int>(std::lround(std::log2(d.c.N0 > 0 ? d.c.N0 : 1.0)))));
+        hipLaunchKernelGGL((k_xfused<T, NTc, RB, L2N>), dim3(static_cast<unsigned>((nrows + RB - 1) / RB)),
+                           dim3(kFftThreads), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, log2Exact(d.c.Nx), Ny, Nz,
+                           static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), kxv.as<T>(), es);
+    }
+
+    // Nx = 64 ... 512 run kernels compiled for that length; others the
+    // runtime-length kernel.
+    template <typename T, int NTc>
+    void launchXfusedFor(hipStream_t s) {
+        constexpr int pairs = XMap<NTc>::pairs;
+        switch (log2Exact(d.c.Nx)) {
+            case 6: return launchXfusedKernel<T, NTc, xfusedRows<T>(64, pairs), 6>(s);
+            case 7: return launchXfusedKernel<T, NTc, xfusedRows<T>(128, pairs), 7>(s);
+            case 8: return launchXfusedKernel<T, NTc, xfusedRows<T>(256, pairs), 8>(s);
+            case 9: return launchXfusedKernel<T, NTc, xfusedRows<T>(512, pairs), 9>(s);
+            default: break;
+        }
+        if (xfusedRows<T>(static_cast<int>(d.c.Nx), pairs) == 2)
+            launchXfusedKernel<T, NTc, 2, 0>(s);
+        else
+            launchXfusedKernel<T, NTc, 1, 0>(s);
     }
 
     template <typename T>
     void launchXfused(hipStream_t s) {
         const double bytes = (static_cast<double>(NH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>);
-        const int Nx = static_cast<int>(d.c.Nx), l2 = log2Exact(d.c.Nx), Ny = static_cast<int>(d.c.Ny),
-                  Nz = static_cast<int>(d.c.Nz), nkx = static_cast<int>(d.nkx), nkxP = static_cast<int>(d.nkxP);
-        const int nrows = Ny * Nz;
-        // two rows per block while four blocks still fit a CU's 160 KB of LDS
-        const bool two = xfusedLds<T, 2>() <= std::min<size_t>(maxLdsPerBlock, 40 * 1024);
-        const size_t lds = two ? xfusedLds<T, 2>() : xfusedLds<T, 1>();
-        const dim3 grid(static_cast<unsigned>(two ? (nrows + 1) / 2 : nrows));
         timed("xfused", bytes, s, [&] {
-            const cx<T>* g = G.as<cx<T>>();
-            cx<T>* pp = P.as<cx<T>>();
-            const cx<T>* tw = twX.as<cx<T>>();
-            const T* kv = kxv.as<T>();
-            // power of two nearest N0 (exact scaling of the eta product)
-            const T es = static_cast<T>(std::ldexp(1.0, static_cast<int>(std::lround(std::log2(d.c.N0 > 0 ? d.c.N0 : 1.0)))));
-            if (NT == 4) {
-                if (two)
-                    hipLaunchKernelGGL((k_xfused<T, 4, 2>), grid, dim3(256), lds, s, g, pp, Nx, l2, Ny, Nz, nkx, nkxP, tw, kv, es);
-                else
-                    hipLaunchKernelGGL((k_xfused<T, 4, 1>), grid, dim3(256), lds, s, g, pp, Nx, l2, Ny, Nz, nkx, nkxP, tw, kv, es);
-            } else {
-                if (two)
-                    hipLaunchKernelGGL((k_xfused<T, 3, 2>), grid, dim3(256), lds, s, g, pp, Nx, l2, Ny, Nz, nkx, nkxP, tw, kv, es);
-                else
-                    hipLaunchKernelGGL((k_xfused<T, 3, 1>), grid, dim3(256), lds, s, g, pp, Nx, l2, Ny, Nz, nkx, nkxP, tw, kv, es);
-            }
+            if (NT == 4)
+                launchXfusedFor<T, 4>(s);
+            else
+                launchXfusedFor<T, 3>(s);
         });
     }
 
