@@ -105,6 +105,11 @@ struct ProjTables {
 struct Epilogue {
     int mode;
     double scale;  // 0.5h (stages 1,2), h (stage 3), h/6 (stage 4)
+    // RK modes: also store the field spectra of the state the stage produces
+    // (the next stage's input; after stage 4 the new y) at elapsed time
+    // nextElapsed, the input of the next flux evaluation
+    int emitFields;
+    double nextElapsed;
     cx<double>* out0;
     cx<double>* out1;
     cx<double>* out2;
@@ -124,30 +129,57 @@ struct Epilogue {
 // Index of the calling wave in its workgroup, as a wave-uniform (scalar) value.
 __device__ __forceinline__ int waveIndex() { return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)); }
 
+// Workgroups go to the 8 XCDs in turn (block b to XCD b % 8), each XCD with
+// its own L2.  This renumbers them so that every XCD runs one contiguous range
+// of logical blocks, and blocks that share cache lines share an L2.
+__device__ __forceinline__ int xcdBlock(int b, int nblocks) {
+    constexpr int kXcds = 8;
+    const int x = b % kXcds, q = nblocks / kXcds, r = nblocks % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
 
-// Evolved field spectra per coefficient (WVCoefficientFormulas.hpp:15-33).
+
+// Coefficient tables of the evolved field spectra.
+struct FieldTables {
+    const double* omega;
+    const cx<double>* UAp;
+    const cx<double>* VAp;
+    const cx<double>* WAp;
+    const double* NAp;
+    const cx<double>* UA0;
+    const cx<double>* VA0;
+    const double* NA0;
+};
+
+// Evolved field spectra (u, v, w, eta) of coefficient idx
+// (WVCoefficientFormulas.hpp:15-33), stored to FV[f*M + idx].
 template <typename T>
-__global__ __launch_bounds__(256) void k_fields(const cx<double>* __restrict__ Ap, const cx<double>* __restrict__ Am,
-                                                const cx<double>* __restrict__ A0, const double* __restrict__ omega,
-                                                const cx<double>* __restrict__ UAp, const cx<double>* __restrict__ VAp,
-                                                const cx<double>* __restrict__ WAp, const double* __restrict__ NAp,
-                                                const cx<double>* __restrict__ UA0, const cx<double>* __restrict__ VA0,
-                                                const double* __restrict__ NA0, double elapsed, int M,
-                                                cx<T>* __restrict__ FV) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= M) return;
+__device__ __forceinline__ void storeFieldSpectra(const FieldTables& ft, int idx, int M, cx<double> Ap,
+                                                  cx<double> Am, cx<double> A0, double omega, double elapsed,
+                                                  cx<T>* __restrict__ FV) {
+    const cx<double> UAp = ft.UAp[idx], VAp = ft.VAp[idx], WAp = ft.WAp[idx], UA0 = ft.UA0[idx], VA0 = ft.VA0[idx];
+    const double NAp = ft.NAp[idx], NA0 = ft.NA0[idx];
     double sn, cs;
-    sincos(omega[idx] * elapsed, &sn, &cs);
+    sincos(omega * elapsed, &sn, &cs);
     const cx<double> p{cs, sn};
-    const cx<double> ap = cmul(Ap[idx], p), am = cmul(Am[idx], cconj(p)), a0 = A0[idx];
-    const cx<double> u = cadd(cadd(cmul(UAp[idx], ap), cmul(cconj(UAp[idx]), am)), cmul(UA0[idx], a0));
-    const cx<double> v = cadd(cadd(cmul(VAp[idx], ap), cmul(cconj(VAp[idx]), am)), cmul(VA0[idx], a0));
-    const cx<double> w = cadd(cmul(WAp[idx], ap), cmul(WAp[idx], am));
-    const cx<double> e = cadd(cadd(cscale(ap, NAp[idx]), cscale(am, -NAp[idx])), cscale(a0, NA0[idx]));
+    const cx<double> ap = cmul(Ap, p), am = cmul(Am, cconj(p)), a0 = A0;
+    const cx<double> u = cadd(cadd(cmul(UAp, ap), cmul(cconj(UAp), am)), cmul(UA0, a0));
+    const cx<double> v = cadd(cadd(cmul(VAp, ap), cmul(cconj(VAp), am)), cmul(VA0, a0));
+    const cx<double> w = cadd(cmul(WAp, ap), cmul(WAp, am));
+    const cx<double> e = cadd(cadd(cscale(ap, NAp), cscale(am, -NAp)), cscale(a0, NA0));
     FV[idx] = {T(u.re), T(u.im)};
     FV[M + idx] = {T(v.re), T(v.im)};
     FV[2 * M + idx] = {T(w.re), T(w.im)};
     FV[3 * M + idx] = {T(e.re), T(e.im)};
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fields(const cx<double>* __restrict__ Ap, const cx<double>* __restrict__ Am,
+                                                const cx<double>* __restrict__ A0, FieldTables ft, double elapsed,
+                                                int M, cx<T>* __restrict__ FV) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M) return;
+    storeFieldSpectra(ft, idx, M, Ap[idx], Am[idx], A0[idx], ft.omega[idx], elapsed, FV);
 }
 
 // F- or G-family field spectrum Apm*waveScale + A0*geostrophicScale per
@@ -174,14 +206,28 @@ __host__ __device__ constexpr int synthJStride(int Js) {
     return sizeof(T) == 4 ? Js + ((4 - Js % 32) % 32 + 32) % 32 : Js + ((2 - Js % 16) % 16 + 16) % 16;
 }
 
+// Value of lane l ^ 1 (DPP quad permutation 1,0,3,2).
+__device__ __forceinline__ float pairSwap(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double pairSwap(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
 // Spectral assembly + vertical synthesis (MFMA).  One workgroup owns 8
 // consecutive columns: it stages their evolved field spectra (all j < Nj) and
 // m_j in LDS once, then its waves loop over (group, z-chunk) tasks, each
-// computing ZT tiles of 16 z levels for the group's channels:
+// computing ZT tiles of 16 z levels for NQ channel slots of the group:
 //   H[out][z][iy][kx] = sum_j syn_fam[z][j] * value_ch(j, column).
 // Complex values are two real GEMM columns; lane l owns GEMM column l%16
-// (column (l%16)/2, real or imaginary part) and k-row l/16.
-template <typename T, int ZT>
+// (column (l%16)/2, real or imaginary part) and k-row l/16.  The contraction
+// length Js is a multiple of 16, so the step loop runs in whole batches with
+// no per-step conditions; channel slots beyond the group's channels and z
+// tiles beyond the last carry zero coefficients or are not stored.
+template <typename T, int ZT, int NQ, int KS>
 __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
                                                int ncol, int M, SynthMats<T> mats, int Zs, int Js, int Nz,
                                                int Nj, int Ny, int nkxP, const T* __restrict__ mj, SynthPlan plan,
@@ -198,7 +244,7 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
     const int lane = threadIdx.x & 63;
     const int waveInBlock = waveIndex();
     const int nwaves = blockDim.x >> 6;
-    const int cg = blockIdx.x;
+    const int cg = xcdBlock(blockIdx.x, gridDim.x);
     const int ztiles = Zs / 16;
     const int nzc = (ztiles + ZT - 1) / ZT;
     const int nsteps = Js / 4;
@@ -211,7 +257,7 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
     // stage spectra: loads contiguous in j for each column, issued kB at a
     // time from clamped addresses (unconditional loads)
     {
-        constexpr int kB = 4;
+        constexpr int kB = 8;
         const int tot = 4 * Js * 8;
         for (int e0 = 0; e0 < tot; e0 += kB * static_cast<int>(blockDim.x)) {
             cx<T> v[kB];
@@ -246,19 +292,23 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
     const T rowSign = (cd.flags & kColConj) ? T(-1) : T(1);
     const T partWeight = part ? ((cd.flags & kColSelfConj) ? T(0) : rowSign) : T(1);
     const T ks = rowSign * T(cd.k), ls = rowSign * T(cd.l);
+    const bool store = colValid && !(plan.skipCompletion && (cd.flags & kColCompletion));
+    const int colOff = 2 * (cd.iy * nkxP + cd.kx) + part;  // word offset in a z plane
+    const int zStride = 2 * Ny * nkxP;                      // words per z plane
+    const size_t chStride = static_cast<size_t>(Nz) * zStride;
     // lane reads the real (part 0) or imaginary (part 1) word of fv[.][c][j]
     const T* fvw = reinterpret_cast<const T*>(fv) + 2 * c * Jp + part;
     for (int task = waveInBlock; task < plan.ngroups * nzc; task += nwaves) {
         const int gi = task % plan.ngroups;
         const int zc = task / plan.ngroups;
         const int mat = plan.g[gi].mat, nch = plan.g[gi].nch;
-        int fieldOff[4];
-        T coefV[4], coefZ[4];
+        int fieldOff[NQ];
+        T coefV[NQ], coefZ[NQ];
         // value channels: valueScale; spectral z-derivative channels: -/+ m_j / 2
         const T zsign = mat == 1 ? T(-0.5) : T(0.5);
         const T vscale = T(plan.valueScale);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             const bool live = q < nch;
             const bool dz = live && plan.g[gi].ch[q].dz;
             fieldOff[q] = live ? plan.g[gi].ch[q].field * 16 * Jp : 0;
@@ -267,14 +317,15 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
         }
         const T* A = mats.m[mat];
         const int zt0 = zc * ZT;
-        typename MF::acc acc[4][ZT];
+        typename MF::acc acc[NQ][ZT];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < NQ; ++q)
 #pragma unroll
             for (int r = 0; r < ZT; ++r) acc[q][r] = MF::zero();
         // A operands in register batches of kS steps: two named batches used in
-        // turn (no register copies, so each wait covers only the batch in use)
-        constexpr int kS = 4;
+        // turn (no register copies, so each wait covers only the batch in use);
+        // the step count is a multiple of 2 kS
+        constexpr int kS = KS;
         const T* Ar[ZT];
 #pragma unroll
         for (int r = 0; r < ZT; ++r) {
@@ -285,58 +336,60 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
 #pragma unroll
             for (int r = 0; r < ZT; ++r)
 #pragma unroll
-                for (int u = 0; u < kS; ++u) dst[r][u] = Ar[r][static_cast<size_t>(min(s0 + u, nsteps - 1)) * 64];
+                for (int u = 0; u < kS; ++u) dst[r][u] = Ar[r][(s0 + u) * 64];
         };
         auto run = [&](const T (&a)[ZT][kS], int s0) {
+            T b[kS][NQ];
 #pragma unroll
             for (int u = 0; u < kS; ++u) {
-                const int st = s0 + u;
-                if (st < nsteps) {
-                    const int j = 4 * st + kq;
-                    const T mjj = mjs[j];
-                    const T* fj = fvw + 2 * j;
+                const int j = 4 * (s0 + u) + kq;
+                const T mjj = mjs[j];
+                const T* fj = fvw + 2 * j;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        if (q < nch) {
-                            const T b = fj[fieldOff[q]] * (coefV[q] + coefZ[q] * mjj);
-#pragma unroll
-                            for (int r = 0; r < ZT; ++r)
-                                if (zt0 + r < ztiles) acc[q][r] = MF::run(a[r][u], b, acc[q][r]);
-                        }
-                    }
-                }
+                for (int q = 0; q < NQ; ++q) b[u][q] = fj[fieldOff[q]] * (coefV[q] + coefZ[q] * mjj);
             }
+#pragma unroll
+            for (int u = 0; u < kS; ++u)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                    for (int r = 0; r < ZT; ++r) acc[q][r] = MF::run(a[r][u], b[u][q], acc[q][r]);
         };
         T a0[ZT][kS], a1[ZT][kS];
         load(a0, 0);
-        for (int s0 = 0; s0 < nsteps; s0 += 2 * kS) {
+        int s0 = 0;
+        for (; s0 + 2 * kS < nsteps; s0 += 2 * kS) {
             load(a1, s0 + kS);
             run(a0, s0);
             load(a0, s0 + 2 * kS);
             run(a1, s0 + kS);
         }
+        load(a1, s0 + kS);
+        run(a0, s0);
+        run(a1, s0 + kS);
         // outputs: the value and i k / i l times it (the partner lane n^1
         // holds the other part of the same complex entry)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NQ; ++q) {
             if (q >= nch) continue;
             const int oV = plan.g[gi].ch[q].outV, oX = plan.g[gi].ch[q].outX, oY = plan.g[gi].ch[q].outY;
+            T* hV = reinterpret_cast<T*>(H) + (oV >= 0 ? oV : 0) * chStride + colOff;
+            T* hX = reinterpret_cast<T*>(H) + (oX >= 0 ? oX : 0) * chStride + colOff;
+            T* hY = reinterpret_cast<T*>(H) + (oY >= 0 ? oY : 0) * chStride + colOff;
 #pragma unroll
             for (int r = 0; r < ZT; ++r) {
                 const int zt = zt0 + r;
-                if (zt >= ztiles) continue;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const T v = acc[q][r][i];
-                    const T other = __shfl_xor(v, 1);
+                    const T other = pairSwap(v);
                     const T rot = part ? other : -other;  // part of i * (re + i im)
                     const int z = 16 * zt + MF::row(kq, i);
-                    if (z < Nz && colValid && !(plan.skipCompletion && (cd.flags & kColCompletion))) {
-                        const size_t o = (static_cast<size_t>(z) * Ny + cd.iy) * nkxP + cd.kx;
-                        const size_t chs = static_cast<size_t>(Nz) * Ny * nkxP;
-                        if (oV >= 0) reinterpret_cast<T*>(&H[oV * chs + o])[part] = v;
-                        if (oX >= 0) reinterpret_cast<T*>(&H[oX * chs + o])[part] = ks * rot;
-                        if (oY >= 0) reinterpret_cast<T*>(&H[oY * chs + o])[part] = ls * rot;
+                    if (zt < ztiles && z < Nz && store) {
+                        const int o = z * zStride;
+                        if (oV >= 0) hV[o] = v;
+                        if (oX >= 0) hX[o] = ks * rot;
+                        if (oY >= 0) hY[o] = ls * rot;
                     }
                 }
             }
@@ -635,23 +688,38 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
     }
 }
 
-// Vertical analysis (MFMA) + wave-vortex projection + flux store or RK4 stage.
+// Projection targets of the NT transformed products: u, v, w, eta
+// (nonhydrostatic) or u, v, eta (hydrostatic).
+template <int NT>
+__host__ __device__ constexpr int projTarget(int t) { return NT == 4 ? t : (t == 2 ? 3 : t); }
+
+// LDS bytes of k_proj: the staged columns, later the transposed results.
+template <typename T>
+constexpr size_t projLdsBytes(int NT, int Za, int Ja) {
+    const size_t staged = static_cast<size_t>(NT) * Za * 8 * sizeof(cx<T>);
+    const size_t results = static_cast<size_t>(Ja / 16) * 4 * 16 * 17 * sizeof(T);
+    return staged > results ? staged : results;
+}
+
+// Vertical analysis (MFMA) of the NT transformed products: the spectral
+// value of every target at (j, mode), stored as Tg[t][j + Nj*mode] (the
+// storage row's conjugation undone; the 1/(Nx Ny) scaling and the
+// wave-vortex projection follow in k_stage).
 // One workgroup owns 8 consecutive columns and has one wave per 16-row tile
 // of j; each target's Q columns (all z) are staged in LDS once and shared by
 // the waves.
-template <typename T, int NT>
+template <typename T, int NT, int KS>
 __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, const ColumnDev* __restrict__ cols,
                                                int ncol, const T* __restrict__ anaC, const T* __restrict__ anaS,
-                                               int Ja, int Za, int Nz, int Nj, int Ny, int nkxP,
-                                               int targetCode, int hydro, ProjTables tb, double elapsed,
-                                               double hscale, Epilogue ep) {
+                                               int Ja, int Za, int Nz, int Nj, int Ny, int nkxP, int M,
+                                               cx<T>* __restrict__ Tg) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [NT][Za][8]
     __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
     const int jt = waveIndex();
-    const int cg = blockIdx.x;
+    const int cg = xcdBlock(blockIdx.x, gridDim.x);
     if (threadIdx.x < 8) {
         const int col = cg * 8 + threadIdx.x;
         if (col < ncol) scol[threadIdx.x] = cols[col];
@@ -696,7 +764,7 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = MF::zero();
     const T* qw = reinterpret_cast<const T*>(qs) + 2 * (n >> 1) + part;  // [t][z][c] words
-    constexpr int kS = 8;
+    constexpr int kS = KS;  // the step count Za / 4 is a multiple of 2 kS
 #pragma unroll
     for (int fam = 0; fam < 2; ++fam) {
         const T* Aj = (fam ? anaS : anaC) + static_cast<size_t>(jt) * nsteps * 64 + lane;
@@ -704,132 +772,167 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
         // before the MFMAs cover only the batch in use)
         auto load = [&](T (&dst)[kS], int s0) {
 #pragma unroll
-            for (int u = 0; u < kS; ++u) dst[u] = Aj[static_cast<size_t>(min(s0 + u, nsteps - 1)) * 64];
+            for (int u = 0; u < kS; ++u) dst[u] = Aj[(s0 + u) * 64];
         };
         auto run = [&](const T (&a)[kS], int s0) {
+            T b[kS][NT];
 #pragma unroll
             for (int u = 0; u < kS; ++u) {
-                const int st = s0 + u;
-                if (st < nsteps) {
-                    const int zrow = 4 * st + kq;
+                const int zrow = 4 * (s0 + u) + kq;
 #pragma unroll
-                    for (int t = 0; t < NT; ++t) {
-                        const int target = (targetCode >> (4 * t)) & 0xF;
-                        if ((target >= 2) == (fam == 1))
-                            acc[t] = MF::run(a[u], qw[2 * ((t * Za + zrow) * 8)], acc[t]);
-                    }
-                }
+                for (int t = 0; t < NT; ++t)
+                    if ((projTarget<NT>(t) >= 2) == (fam == 1)) b[u][t] = qw[2 * ((t * Za + zrow) * 8)];
             }
+#pragma unroll
+            for (int u = 0; u < kS; ++u)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    if ((projTarget<NT>(t) >= 2) == (fam == 1)) acc[t] = MF::run(a[u], b[u][t], acc[t]);
         };
         T a0[kS], a1[kS];
         load(a0, 0);
-        for (int s0 = 0; s0 < nsteps; s0 += 2 * kS) {
+        int s0 = 0;
+        for (; s0 + 2 * kS < nsteps; s0 += 2 * kS) {
             load(a1, s0 + kS);
             run(a0, s0);
             load(a0, s0 + 2 * kS);
             run(a1, s0 + kS);
         }
+        load(a1, s0 + kS);
+        run(a0, s0);
+        run(a1, s0 + kS);
     }
-    const ColumnDev cd = scol[n >> 1];
-    const bool colValid = cd.wv >= 0;
-    // complex coefficient c_t(j, column) for rows j = 16 jt + row(kq, i); the
-    // even lane of a pair keeps items 0,1 and the odd lane items 2,3
-    double cre[4][2], cim[4][2];
+    // The analysis results C_t[j][n] go through LDS (the staged columns are
+    // no longer needed) so that the stores' lanes walk j: lane (c, jj) takes
+    // rows j = 16 jt + 8 s + jj of column c.
+    constexpr int kResStride = 17;  // words per row of a wave's [t][16 j][16 n] block
+    __syncthreads();  // every wave has finished reading qs
+    T* res = reinterpret_cast<T*>(smem) + jt * (4 * 16 * kResStride);
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const double mine = static_cast<double>(acc[t][i]);
-            const double other = __shfl_xor(mine, 1);
-            const double re = part ? other : mine, im = part ? mine : other;
-            if ((i >> 1) == part) {
-                cre[t][i & 1] = re;
-                cim[t][i & 1] = im;
-            }
-        }
-    }
-    if (!colValid || (cd.flags & kColCompletion)) return;
+        for (int i = 0; i < 4; ++i) res[(t * 16 + MF::row(kq, i)) * kResStride + n] = acc[t][i];
+    __syncthreads();
+    const int ec = lane >> 3, jj = lane & 7;
+    const ColumnDev cd = scol[ec];
+    if (cd.wv < 0 || (cd.flags & kColCompletion)) return;
     const bool conj = (cd.flags & kColConj) != 0;
-    const bool inertial = cd.wv == 0;
 #pragma unroll
     for (int ii = 0; ii < 2; ++ii) {
-        const int i = 2 * part + ii;
-        const int j = 16 * jt + MF::row(kq, i);
+        const int jl = 8 * ii + jj;
+        const int j = 16 * jt + jl;
         if (j >= Nj) continue;
         const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
-        double sn, cs;
-        sincos(tb.omega[idx] * elapsed, &sn, &cs);
-        const cx<double> p{cs, sn};
-        const double a0v = tb.A0Vort[idx], a0b = tb.A0Buoy[idx], na0 = tb.NA0[idx], apmn = tb.ApmN[idx];
-        cx<double> Fp{0.0, 0.0}, Fm{0.0, 0.0}, F0{0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int target = (targetCode >> (4 * t)) & 0xF;
-            cx<double> value{cre[t][ii], conj ? -cim[t][ii] : cim[t][ii]};
-            value = cscale(value, hscale);
-            cx<double> a0{0.0, 0.0};
-            if (target == 0) a0 = cscale(cmuli(value, -cd.l), a0v);
-            if (target == 1) a0 = cscale(cmuli(value, cd.k), a0v);
-            if (target == 3) a0 = cscale(value, a0b);
-            const cx<double> buoyValue = csub(target == 3 ? value : cx<double>{0.0, 0.0}, cscale(a0, na0));
-            const cx<double> buoy = cscale(buoyValue, apmn);
-            cx<double> wave{0.0, 0.0};
-            if (hydro) {
-                if (target == 0) wave = cmul(cmuli(value, cd.k), tb.ApmD[idx]);
-                if (target == 1) wave = cmul(cmuli(value, cd.l), tb.ApmD[idx]);
-            } else {
-                if (target == 0) wave = cscale(value, cd.cosA * tb.ApmDScaled[idx]);
-                if (target == 1) wave = cscale(value, cd.sinA * tb.ApmDScaled[idx]);
-                if (target == 2) wave = cmuli(value, (cd.Kh / 2.0) * tb.apmW[j]);
-            }
-            cx<double> cAp = cadd(wave, buoy), cAm = csub(wave, buoy);
-            if (inertial) {
-                cx<double> a{0.0, 0.0};
-                if (target == 0) a = cscale(value, tb.inertialScale[j]);
-                if (target == 1) a = cscale(cmuli(value, -1.0), tb.inertialScale[j]);
-                cAp = a;
-                cAm = cconj(a);
-            }
-            Fp = cadd(Fp, cmul(cAp, cconj(p)));
-            Fm = cadd(Fm, cmul(cAm, p));
-            F0 = cadd(F0, a0);
+            const T* rt = res + (t * 16 + jl) * kResStride + 2 * ec;
+            Tg[t * static_cast<size_t>(M) + idx] = cx<T>{rt[0], conj ? -rt[1] : rt[1]};
         }
-        const cx<double> k3[3] = {Fp, Fm, F0};
-        cx<double>* outs[3] = {ep.out0, ep.out1, ep.out2};
-        cx<double>* ys[3] = {ep.y0, ep.y1, ep.y2};
-        cx<double>* sts[3] = {ep.st0, ep.st1, ep.st2};
-        cx<double>* ws[3] = {ep.w0, ep.w1, ep.w2};
-        // RK4 stage arithmetic of WVFixedStepRK4::step with scaledSum
-        // (value + scale*increment, WVRungeKutta.cpp:15-19), rounded per
-        // operation as on the CPU.
-        auto axpy = [](cx<double> y, cx<double> x, double s) {
-            return cx<double>{__dadd_rn(y.re, __dmul_rn(s, x.re)), __dadd_rn(y.im, __dmul_rn(s, x.im))};
-        };
+    }
+}
+
+// Per-mode wavenumbers of the projection.
+struct ModeDev {
+    double k, l, Kh, cosA, sinA;
+};
+
+// Wave-vortex projection of the analysed products (projectFluxTargetMode,
+// ...Kernel.cpp) and the flux store or RK4 stage of WVFixedStepRK4::step,
+// one coefficient (j, mode) per thread; in RK modes with emitFields set it
+// also forms the field spectra of the state it produces (the input of the
+// next flux evaluation).
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, const ModeDev* __restrict__ modes,
+                                               int Nj, int M, ProjTables tb, FieldTables ft, double elapsed,
+                                               double hscale, Epilogue ep, cx<T>* __restrict__ FV) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M) return;
+    const int wv = idx / Nj, j = idx - wv * Nj;
+    constexpr bool hydroT = NT == 3;  // hydrostatic: ApmD, else ApmDScaled
+    const ModeDev md = modes[wv];
+    const bool inertial = wv == 0;
+    const double om = tb.omega[idx], a0v = tb.A0Vort[idx], a0b = tb.A0Buoy[idx], na0 = tb.NA0[idx],
+                 apmn = tb.ApmN[idx];
+    cx<double> apmd{0.0, 0.0};
+    double dsc = 0.0;
+    if constexpr (hydroT) apmd = tb.ApmD[idx];
+    else dsc = tb.ApmDScaled[idx];
+    cx<T> tv[NT];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const cx<double> kv = k3[a];
-            switch (ep.mode) {
-                case kFluxOut: outs[a][idx] = kv; break;
-                case kRk1:
-                    ws[a][idx] = kv;
-                    sts[a][idx] = axpy(ys[a][idx], kv, ep.scale);
-                    break;
-                case kRk2:
-                    ws[a][idx] = axpy(ws[a][idx], kv, 2.0);
-                    sts[a][idx] = axpy(ys[a][idx], kv, ep.scale);
-                    break;
-                case kRk3:
-                    ws[a][idx] = axpy(ws[a][idx], kv, 2.0);
-                    sts[a][idx] = axpy(ys[a][idx], kv, ep.scale);
-                    break;
-                default: {
-                    const cx<double> w = axpy(ws[a][idx], kv, 1.0);
-                    ys[a][idx] = axpy(ys[a][idx], w, ep.scale);
-                    break;
-                }
+    for (int t = 0; t < NT; ++t) tv[t] = Tg[t * static_cast<size_t>(M) + idx];
+    double sn, cs;
+    sincos(om * elapsed, &sn, &cs);
+    const cx<double> p{cs, sn};
+    cx<double> Fp{0.0, 0.0}, Fm{0.0, 0.0}, F0{0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int target = projTarget<NT>(t);
+        const cx<double> value = cscale(cx<double>{static_cast<double>(tv[t].re), static_cast<double>(tv[t].im)}, hscale);
+        cx<double> a0{0.0, 0.0};
+        if (target == 0) a0 = cscale(cmuli(value, -md.l), a0v);
+        if (target == 1) a0 = cscale(cmuli(value, md.k), a0v);
+        if (target == 3) a0 = cscale(value, a0b);
+        const cx<double> buoyValue = csub(target == 3 ? value : cx<double>{0.0, 0.0}, cscale(a0, na0));
+        const cx<double> buoy = cscale(buoyValue, apmn);
+        cx<double> wave{0.0, 0.0};
+        if constexpr (hydroT) {
+            if (target == 0) wave = cmul(cmuli(value, md.k), apmd);
+            if (target == 1) wave = cmul(cmuli(value, md.l), apmd);
+        } else {
+            if (target == 0) wave = cscale(value, md.cosA * dsc);
+            if (target == 1) wave = cscale(value, md.sinA * dsc);
+            if (target == 2) wave = cmuli(value, (md.Kh / 2.0) * tb.apmW[j]);
+        }
+        cx<double> cAp = cadd(wave, buoy), cAm = csub(wave, buoy);
+        if (inertial) {
+            cx<double> a{0.0, 0.0};
+            if (target == 0) a = cscale(value, tb.inertialScale[j]);
+            if (target == 1) a = cscale(cmuli(value, -1.0), tb.inertialScale[j]);
+            cAp = a;
+            cAm = cconj(a);
+        }
+        Fp = cadd(Fp, cmul(cAp, cconj(p)));
+        Fm = cadd(Fm, cmul(cAm, p));
+        F0 = cadd(F0, a0);
+    }
+    const cx<double> k3[3] = {Fp, Fm, F0};
+    cx<double>* outs[3] = {ep.out0, ep.out1, ep.out2};
+    cx<double>* ys[3] = {ep.y0, ep.y1, ep.y2};
+    cx<double>* sts[3] = {ep.st0, ep.st1, ep.st2};
+    cx<double>* ws[3] = {ep.w0, ep.w1, ep.w2};
+    // RK4 stage arithmetic of WVFixedStepRK4::step with scaledSum
+    // (value + scale*increment, WVRungeKutta.cpp:15-19), rounded per
+    // operation as on the CPU.
+    auto axpy = [](cx<double> y, cx<double> x, double s) {
+        return cx<double>{__dadd_rn(y.re, __dmul_rn(s, x.re)), __dadd_rn(y.im, __dmul_rn(s, x.im))};
+    };
+    cx<double> next[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const cx<double> kv = k3[a];
+        switch (ep.mode) {
+            case kFluxOut: outs[a][idx] = kv; break;
+            case kRk1:
+                ws[a][idx] = kv;
+                next[a] = axpy(ys[a][idx], kv, ep.scale);
+                sts[a][idx] = next[a];
+                break;
+            case kRk2:
+            case kRk3:
+                ws[a][idx] = axpy(ws[a][idx], kv, 2.0);
+                next[a] = axpy(ys[a][idx], kv, ep.scale);
+                sts[a][idx] = next[a];
+                break;
+            default: {
+                const cx<double> w = axpy(ws[a][idx], kv, 1.0);
+                next[a] = axpy(ys[a][idx], w, ep.scale);
+                ys[a][idx] = next[a];
+                break;
             }
         }
     }
+    if (ep.mode != kFluxOut && ep.emitFields)
+        storeFieldSpectra(ft, idx, M, next[0], next[1], next[2], om, ep.nextElapsed, FV);
 }
 
 // r2c along x of nf real FP64 fields [Nx,Ny,Nz,nf] into P[c][z][y][kx < nkx]
@@ -1064,7 +1167,7 @@ struct KernelImpl {
     DeviceBuffer cols, omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled, ApmD, ApmTE, A0TE,
         apmW, inertialScale, fWaveScale, gWaveScale, Fg, Gg;
     DeviceBuffer synC, synS, anaC, anaS, synGz, synFh, synEz, twX, twY, mj, kxv;
-    DeviceBuffer FV, H, G, P, Q, stage, weighted, partial, scalar;
+    DeviceBuffer FV, H, G, P, Q, Tg, modeTab, stage, weighted, partial, scalar;
     DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
     double retainedRowFraction = 1.0;
     size_t maxLdsPerBlock = 65536;
@@ -1082,7 +1185,7 @@ struct KernelImpl {
         size_t s = 0;
         for (const DeviceBuffer* b : {&rowLimit, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &synGz, &synFh, &synEz, &twX, &twY, &mj, &kxv, &FV, &H, &G, &P, &Q, &stage, &weighted, &partial, &scalar})
+                                      &anaS, &synGz, &synFh, &synEz, &twX, &twY, &mj, &kxv, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
         return s;
     }
@@ -1197,6 +1300,11 @@ struct KernelImpl {
         const size_t yLds = (static_cast<size_t>(KT) + 1) * cfg.Ny * elem;
         if (xLds > static_cast<size_t>(maxLds) || yLds > static_cast<size_t>(maxLds))
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid needs more LDS per workgroup than the device provides."};
+        const size_t pLds = prec == WV_PRECISION_FP64
+                                ? projLdsBytes<double>(NT, static_cast<int>(d.Za), static_cast<int>(d.Ja))
+                                : projLdsBytes<float>(NT, static_cast<int>(d.Za), static_cast<int>(d.Ja));
+        if (d.Ja > 256 || pLds > static_cast<size_t>(maxLds))
+            throw Failure{WV_UNSUPPORTED_OPERATION, "Vertical resolution too large for the in-LDS analysis (Nj <= 256)."};
         // columns
         ncol = d.columns.size();
         std::vector<ColumnDev> hc(ncol);
@@ -1249,6 +1357,15 @@ struct KernelImpl {
         G.alloc(static_cast<size_t>(NH) * chField * tsize);
         P.alloc(static_cast<size_t>(NT) * chField * tsize);
         Q.alloc(static_cast<size_t>(NT) * chField * tsize);
+        Tg.alloc(static_cast<size_t>(NT) * d.M * tsize);
+        {
+            std::vector<ModeDev> hm(d.Nkl);
+            for (size_t i = 0; i < d.Nkl; ++i) {
+                const FourierMode& fm = d.modes[i];
+                hm[i] = {fm.k, fm.l, fm.Kh, fm.cosAlpha, fm.sinAlpha};
+            }
+            upload(modeTab, hm);
+        }
         hipCheck(hipMemset(H.p, 0, H.bytes), "hipMemset");
         hipCheck(hipMemset(P.p, 0, P.bytes), "hipMemset");
         stage.alloc(3 * d.M * sizeof(cx<double>));
@@ -1290,6 +1407,16 @@ struct KernelImpl {
                              d.stratified ? synFh.as<T>() : nullptr, d.stratified ? synEz.as<T>() : nullptr}};
     }
 
+    FieldTables fieldTables() const {
+        return FieldTables{omega.as<double>(), UAp.as<cx<double>>(), VAp.as<cx<double>>(), WAp.as<cx<double>>(),
+                           NAp.as<double>(),   UA0.as<cx<double>>(), VA0.as<cx<double>>(), NA0.as<double>()};
+    }
+    ProjTables projTables() const {
+        return ProjTables{omega.as<double>(), A0Vort.as<double>(),     A0Buoy.as<double>(),
+                          NA0.as<double>(),   ApmN.as<double>(),       ApmDScaled.as<double>(),
+                          ApmD.as<cx<double>>(), apmW.as<double>(),    inertialScale.as<double>()};
+    }
+
     template <typename T>
     void launchFields(const double* Ap, const double* Am, const double* A0, double elapsed, hipStream_t s) {
         const int M = static_cast<int>(d.M);
@@ -1298,9 +1425,7 @@ struct KernelImpl {
         timed("fields", 152.0 * M + 4.0 * M * sizeof(cx<T>), s, [&] {
             hipLaunchKernelGGL(k_fields<T>, dim3((M + 255) / 256), dim3(256), 0, s,
                                reinterpret_cast<const cx<double>*>(Ap), reinterpret_cast<const cx<double>*>(Am),
-                               reinterpret_cast<const cx<double>*>(A0), omega.as<double>(), UAp.as<cx<double>>(),
-                               VAp.as<cx<double>>(), WAp.as<cx<double>>(), NAp.as<double>(), UA0.as<cx<double>>(),
-                               VA0.as<cx<double>>(), NA0.as<double>(), elapsed, M, FV.as<cx<T>>());
+                               reinterpret_cast<const cx<double>*>(A0), fieldTables(), elapsed, M, FV.as<cx<T>>());
         });
     }
 
@@ -1310,6 +1435,8 @@ struct KernelImpl {
                      const DeviceBuffer* out = nullptr, const SynthMats<T>* mats = nullptr,
                      const char* name = "synth") {
         constexpr int ZT = sizeof(T) == 8 ? 2 : 3;  // z tiles per task (FP64 accumulators are twice the size)
+        int maxch = 0;
+        for (int g = 0; g < sp.ngroups; ++g) maxch = std::max(maxch, sp.g[g].nch);
         const int ncg = static_cast<int>((ncol + 7) / 8);
         int nout = 0;
         for (int g = 0; g < sp.ngroups; ++g)
@@ -1322,7 +1449,10 @@ struct KernelImpl {
         int waves = 4;
         while (tasks % waves) --waves;
         timed(name, bytes, s, [&] {
-            hipLaunchKernelGGL((k_synth<T, ZT>), dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s,
+            const int nsteps = static_cast<int>(d.Js / 4);
+            auto kernel = maxch <= 2 ? (nsteps % 8 == 0 ? k_synth<T, ZT, 2, 4> : k_synth<T, ZT, 2, 2>)
+                                     : (nsteps % 8 == 0 ? k_synth<T, ZT, 4, 4> : k_synth<T, ZT, 4, 2>);
+            hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s,
                                in ? in->as<cx<T>>() : FV.as<cx<T>>(), cols.as<ColumnDev>(), static_cast<int>(ncol),
                                static_cast<int>(d.M), mats ? *mats : synthMats<T>(), static_cast<int>(d.Zs),
                                static_cast<int>(d.Js), static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj),
@@ -1400,53 +1530,78 @@ struct KernelImpl {
         });
     }
 
+    // Vertical analysis of the transformed products into Tg.
     template <typename T>
-    void launchProj(double elapsed, const Epilogue& ep, hipStream_t s) {
+    void launchProj(hipStream_t s) {
         const int ncg = static_cast<int>((ncol + 7) / 8);
-        ProjTables tb{omega.as<double>(), A0Vort.as<double>(), A0Buoy.as<double>(), NA0.as<double>(),
-                      ApmN.as<double>(), ApmDScaled.as<double>(), ApmD.as<cx<double>>(), apmW.as<double>(),
-                      inertialScale.as<double>()};
-        const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
-        // Q columns of every target, the projection tables (48 B per
-        // coefficient) and the epilogue's coefficient arrays: flux out 48 B;
-        // RK stage 1 reads y, writes w and the stage (144 B); stages 2, 3 read
-        // w, y and write w, stage (192 B); stage 4 reads w, y and writes y (144 B)
-        const double epBytes = ep.mode == kFluxOut ? 48.0 : (ep.mode == kRk2 || ep.mode == kRk3) ? 192.0 : 144.0;
-        const double bytes = static_cast<double>(NT) * d.c.Nz * ncol * sizeof(cx<T>) + d.M * (48.0 + epBytes);
+        const double bytes = static_cast<double>(NT) * (static_cast<double>(d.c.Nz) * ncol + d.M) * sizeof(cx<T>);
         timed("proj", bytes, s, [&] {
             const int njt = static_cast<int>(d.Ja / 16);
             const dim3 grid(static_cast<unsigned>(ncg)), block(static_cast<unsigned>(64 * njt));
-            const size_t lds = static_cast<size_t>(NT) * d.Za * 8 * sizeof(cx<T>);
+            const size_t lds = projLdsBytes<T>(NT, static_cast<int>(d.Za), static_cast<int>(d.Ja));
             const int ncolI = static_cast<int>(ncol), Ja = static_cast<int>(d.Ja), Za = static_cast<int>(d.Za),
                       Nz = static_cast<int>(d.c.Nz), Nj = static_cast<int>(d.c.Nj), Ny = static_cast<int>(d.c.Ny),
-                      nkxP = static_cast<int>(d.nkxP), hydro = d.c.isHydrostatic ? 1 : 0;
-            if (NT == 4)
-                hipLaunchKernelGGL((k_proj<T, 4>), grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
-                                   anaC.as<T>(), anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, targetCode, hydro, tb,
-                                   elapsed, hscale, ep);
-            else
-                hipLaunchKernelGGL((k_proj<T, 3>), grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI,
-                                   anaC.as<T>(), anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, targetCode, hydro, tb,
-                                   elapsed, hscale, ep);
+                      nkxP = static_cast<int>(d.nkxP), M = static_cast<int>(d.M);
+            // batch length: 4, 3 or 2 steps, whichever divides the step count in pairs
+            const int nsteps = Za / 4;
+            const int ks = nsteps % 8 == 0 ? 4 : nsteps % 6 == 0 ? 3 : 2;
+            using Fn = void (*)(const cx<T>*, const ColumnDev*, int, const T*, const T*, int, int, int, int, int, int,
+                                int, cx<T>*);
+            Fn fn = NT == 4 ? (ks == 4 ? k_proj<T, 4, 4> : ks == 3 ? k_proj<T, 4, 3> : k_proj<T, 4, 2>)
+                            : (ks == 4 ? k_proj<T, 3, 4> : ks == 3 ? k_proj<T, 3, 3> : k_proj<T, 3, 2>);
+            hipLaunchKernelGGL(fn, grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI, anaC.as<T>(),
+                               anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, M, Tg.as<cx<T>>());
+        });
+    }
+
+    // Projection + flux store or RK stage (+ the next flux input's spectra).
+    template <typename T>
+    void launchStage(double elapsed, const Epilogue& ep, hipStream_t s) {
+        const int M = static_cast<int>(d.M);
+        const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
+        // per coefficient: the NT analysed values, the projection tables
+        // (48 B) and the epilogue's coefficient arrays: flux out 48 B; RK
+        // stage 1 reads y, writes w and the stage (144 B); stages 2, 3 read
+        // w, y and write w, stage (192 B); stage 4 reads w, y and writes y
+        // (144 B); with the next input's spectra the field tables (96 B) and
+        // four spectra
+        const double epBytes = ep.mode == kFluxOut ? 48.0 : (ep.mode == kRk2 || ep.mode == kRk3) ? 192.0 : 144.0;
+        const double fieldBytes = (ep.mode != kFluxOut && ep.emitFields) ? 96.0 + 4.0 * sizeof(cx<T>) : 0.0;
+        const double bytes = static_cast<double>(M) * (NT * sizeof(cx<T>) + 48.0 + epBytes + fieldBytes);
+        // the kernel reads y and w in every mode
+        Epilogue e = ep;
+        if (e.mode == kFluxOut) {
+            e.y0 = e.w0 = e.out0;
+            e.y1 = e.w1 = e.out1;
+            e.y2 = e.w2 = e.out2;
+        }
+        timed("stage", bytes, s, [&] {
+            auto fn = NT == 4 ? k_stage<T, 4> : k_stage<T, 3>;
+            hipLaunchKernelGGL(fn, dim3((M + 255) / 256), dim3(256), 0, s, Tg.as<cx<T>>(), modeTab.as<ModeDev>(),
+                               static_cast<int>(d.c.Nj), M, projTables(), fieldTables(), elapsed, hscale, e,
+                               FV.as<cx<T>>());
         });
     }
 
     template <typename T>
+    // fieldsReady: FV already holds the spectra of (Ap, Am, A0) at t (the
+    // previous RK stage formed them).
     void rhs(const double* Ap, const double* Am, const double* A0, double t, double t0, const Epilogue& ep,
-             hipStream_t s) {
+             hipStream_t s, bool fieldsReady = false) {
         const double elapsed = t - t0;
-        launchFields<T>(Ap, Am, A0, elapsed, s);
+        if (!fieldsReady) launchFields<T>(Ap, Am, A0, elapsed, s);
         launchSynth<T>(fluxPlan(), s);
         launchYfft<T>(H, G, NH, +1, s, "yfft_inv", true, false);
         launchXfused<T>(s);
         launchYfft<T>(P, Q, NT, -1, s, "yfft_fwd", false, true);
-        launchProj<T>(elapsed, ep, s);
+        launchProj<T>(s);
+        launchStage<T>(elapsed, ep, s);
     }
 
     void rhsAny(const double* Ap, const double* Am, const double* A0, double t, double t0, const Epilogue& ep,
-                hipStream_t s) {
-        if (precision == WV_PRECISION_FP64) rhs<double>(Ap, Am, A0, t, t0, ep, s);
-        else rhs<float>(Ap, Am, A0, t, t0, ep, s);
+                hipStream_t s, bool fieldsReady = false) {
+        if (precision == WV_PRECISION_FP64) rhs<double>(Ap, Am, A0, t, t0, ep, s, fieldsReady);
+        else rhs<float>(Ap, Am, A0, t, t0, ep, s, fieldsReady);
     }
 
     template <typename T>
@@ -1798,33 +1953,53 @@ struct KernelImpl {
     }
 
     // One forward RK4 step in place (the arithmetic of wv_rk4_advance_device).
-    void rk4Step(cx<double>* y, double t, double t0, double h, hipStream_t s) {
+    // nsteps classic RK4 steps of the state y[0..2] in place, *t advanced.
+    // Each stage's epilogue forms the next flux evaluation's field spectra
+    // (with the same times as WVFixedStepRK4::step: t + h/2, t + h/2, t + h),
+    // so only the first evaluation of the call starts from the coefficients.
+    void rk4Advance(cx<double>* const y[3], double* t, double t0, double h, int nsteps, hipStream_t s) {
         const size_t M = d.M;
         cx<double>* st = stage.as<cx<double>>();
         cx<double>* w = weighted.as<cx<double>>();
         Epilogue ep{};
-        ep.y0 = y;
-        ep.y1 = y + M;
-        ep.y2 = y + 2 * M;
+        ep.y0 = y[0];
+        ep.y1 = y[1];
+        ep.y2 = y[2];
         ep.st0 = st;
         ep.st1 = st + M;
         ep.st2 = st + 2 * M;
         ep.w0 = w;
         ep.w1 = w + M;
         ep.w2 = w + 2 * M;
-        const double* yd = reinterpret_cast<const double*>(y);
+        ep.emitFields = 1;
         const double* sd = reinterpret_cast<const double*>(st);
-        ep.mode = kRk1;
-        ep.scale = 0.5 * h;
-        rhsAny(yd, yd + 2 * M, yd + 4 * M, t, t0, ep, s);
-        ep.mode = kRk2;
-        rhsAny(sd, sd + 2 * M, sd + 4 * M, t + 0.5 * h, t0, ep, s);
-        ep.mode = kRk3;
-        ep.scale = h;
-        rhsAny(sd, sd + 2 * M, sd + 4 * M, t + 0.5 * h, t0, ep, s);
-        ep.mode = kRk4;
-        ep.scale = h / 6.0;
-        rhsAny(sd, sd + 2 * M, sd + 4 * M, t + h, t0, ep, s);
+        const double* y0 = reinterpret_cast<const double*>(y[0]);
+        const double* y1 = reinterpret_cast<const double*>(y[1]);
+        const double* y2 = reinterpret_cast<const double*>(y[2]);
+        for (int n = 0; n < nsteps; ++n) {
+            const double ti = *t;
+            ep.mode = kRk1;
+            ep.scale = 0.5 * h;
+            ep.nextElapsed = (ti + 0.5 * h) - t0;
+            rhsAny(y0, y1, y2, ti, t0, ep, s, n > 0);
+            ep.mode = kRk2;
+            rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + 0.5 * h, t0, ep, s, true);
+            ep.mode = kRk3;
+            ep.scale = h;
+            ep.nextElapsed = (ti + h) - t0;
+            rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + 0.5 * h, t0, ep, s, true);
+            ep.mode = kRk4;
+            ep.scale = h / 6.0;
+            ep.emitFields = n + 1 < nsteps ? 1 : 0;  // the next step's first evaluation
+            rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + h, t0, ep, s, true);
+            ep.emitFields = 1;
+            *t = ti + h;
+        }
+    }
+
+    void rk4Step(cx<double>* y, double t, double t0, double h, hipStream_t s) {
+        cx<double>* const yy[3] = {y, y + d.M, y + 2 * d.M};
+        rk4Advance(yy, &t, t0, h, 1, s);
     }
 
     // Transpose of nsteps RK4 steps from y0 at time t: ybar (cotangent of the
@@ -2202,37 +2377,9 @@ wv_status wv_rk4_advance_device(wv_kernel* kernel, double* Ap, double* Am, doubl
     return guarded([&] {
         hipCheck(hipSetDevice(k.device), "hipSetDevice");
         using wvamd::cx;
-        const size_t M = k.d.M;
-        cx<double>* y[3] = {reinterpret_cast<cx<double>*>(Ap), reinterpret_cast<cx<double>*>(Am),
-                            reinterpret_cast<cx<double>*>(A0)};
-        cx<double>* st = k.stage.as<cx<double>>();
-        cx<double>* w = k.weighted.as<cx<double>>();
-        wvamd::Epilogue ep{};
-        ep.y0 = y[0];
-        ep.y1 = y[1];
-        ep.y2 = y[2];
-        ep.st0 = st;
-        ep.st1 = st + M;
-        ep.st2 = st + 2 * M;
-        ep.w0 = w;
-        ep.w1 = w + M;
-        ep.w2 = w + 2 * M;
-        const double* sd = reinterpret_cast<const double*>(st);
-        for (int n = 0; n < nsteps; ++n) {
-            const double ti = *t;
-            ep.mode = wvamd::kRk1;
-            ep.scale = 0.5 * h;
-            k.rhsAny(Ap, Am, A0, ti, t0, ep, S(stream));
-            ep.mode = wvamd::kRk2;
-            k.rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + 0.5 * h, t0, ep, S(stream));
-            ep.mode = wvamd::kRk3;
-            ep.scale = h;
-            k.rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + 0.5 * h, t0, ep, S(stream));
-            ep.mode = wvamd::kRk4;
-            ep.scale = h / 6.0;
-            k.rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + h, t0, ep, S(stream));
-            *t = ti + h;
-        }
+        cx<double>* const y[3] = {reinterpret_cast<cx<double>*>(Ap), reinterpret_cast<cx<double>*>(Am),
+                                  reinterpret_cast<cx<double>*>(A0)};
+        k.rk4Advance(y, t, t0, h, nsteps, S(stream));
         return WV_SUCCESS;
     });
 }
