@@ -128,7 +128,6 @@ __global__ __launch_bounds__(256) void k_xadj(const cx<T>* __restrict__ G, const
                 b = sb >= 0 ? G[sb * chStride + rowOff + k] : cx<T>{T(0), T(0)};
                 const T kk = kxv[k];
                 if (XM::ikA[t]) a = cmuli(a, kk);
-                if (XM::ikB[t]) b = cmuli(b, kk);
                 if (k == 0) a.im = b.im = T(0);
             } else {
                 const int ta = 2 * (t - PAIRS), tb = ta + 1;

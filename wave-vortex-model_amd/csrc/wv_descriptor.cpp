@@ -328,9 +328,10 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile, const Stratif
     // Power-of-two n: the Stockham pass twiddles of the LDS FFT, pass (Ns, R)
     // entry (r, k) at r*Ns + k - 1 holding exp(-2 pi i r k / (Ns R)), so that
     // consecutive k (consecutive threads) read consecutive entries; the pass
-    // order is the FFT's (radix 2^(log2 n mod 4) first, then radix 16).
+    // order is the FFT's (radix 2^(log2 n mod lr) first, then radix 2^lr;
+    // lr = 4, or 2 for the radix-4 sequence).
     // Other n: the plain table exp(-2 pi i q / n) of the direct DFT.
-    auto twiddles = [](size_t n, std::vector<double>& out) {
+    auto twiddles = [](size_t n, std::vector<double>& out, int lr) {
         out.assign(2 * n, 0.0);
         auto put = [&](size_t idx, size_t q) {
             out[2 * idx] = cosPiRatio(2 * q, n);
@@ -349,11 +350,12 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile, const Stratif
                     if (r * Ns + k >= 1) put(r * Ns + k - 1, (r * k * (n / (Ns * R))) % n);
             Ns *= R;
         };
-        if (l2 % 4) pass(size_t{1} << (l2 % 4));
-        while (Ns < n) pass(16);
+        if (l2 % lr) pass(size_t{1} << (l2 % lr));
+        while (Ns < n) pass(size_t{1} << lr);
     };
-    twiddles(c.Nx, twX);
-    twiddles(c.Ny, twY);
+    twiddles(c.Nx, twX, 4);
+    twiddles(c.Nx, twX4, 2);
+    twiddles(c.Ny, twY, 4);
     return {};
 }
 

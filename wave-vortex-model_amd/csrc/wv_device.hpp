@@ -8,7 +8,7 @@
 namespace wvamd {
 
 template <typename T>
-struct cx {
+struct alignas(2 * sizeof(T)) cx {  // aligned as a pair: one 8- or 16-byte access
     T re, im;
 };
 
@@ -64,8 +64,8 @@ struct Mfma<double> {
 //   BatchCols: nb = 2^log2nb interleaved transforms, element n of transform b
 //     at n*nb + b; consecutive threads walk the transforms.
 // X_k = sum_n x_n exp(SIGN*2*pi*i*n*k/N).  Twiddles: for power-of-two N the
-// pass table, entry r*Ns + k - 1 = exp(-2*pi*i*r*k/(Ns*R)) for the pass that
-// combines sub-transforms of length Ns with radix R; otherwise
+// pass table, entry r*Ns + k - 1 = exp(-2*pi*i*r*k/(Ns*R)) (k = 0..Ns-1) for
+// the pass that combines sub-transforms of length Ns with radix R; otherwise
 // tw[q] = exp(-2*pi*i*q/N).
 // Power-of-two N runs Stockham autosort passes of radix 16 (then 8, 4 or 2
 // for the remaining factor), each butterfly an in-register DFT, with all index
@@ -199,11 +199,9 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
             for (int r = 0; r < R; ++r) {
                 cx<T> x = buf[bt.at(jj + r * NR) + boff];
                 if (r > 0 && log2Ns > 0) {
-                    // k = 0 takes the unit twiddle by a select (no divergent
-                    // branch); its table read stays in range
+                    // the table holds the unit twiddle at k = 0 too
                     cx<T> w = tw[(r << log2Ns) + k - 1];
                     if (SIGN > 0) w.im = -w.im;
-                    if (k == 0) w = cx<T>{T(1), T(0)};
                     x = cmul(x, w);
                 }
                 v[it][r] = x;
@@ -259,24 +257,26 @@ __device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restri
     __syncthreads();
 }
 
-// Radix-16 passes from sub-transform length 2^L2S up to 2^L2N.
-template <typename T, int SIGN, int L2N, int L2S, typename Bt>
-__device__ __forceinline__ void radix16Passes(cx<T>* buf, const Bt& bt, const cx<T>* __restrict__ tw) {
+// Radix-2^LR passes from sub-transform length 2^L2S up to 2^L2N.
+template <typename T, int SIGN, int L2N, int L2S, int LR, typename Bt>
+__device__ __forceinline__ void radixPasses(cx<T>* buf, const Bt& bt, const cx<T>* __restrict__ tw) {
     if constexpr (L2S < L2N) {
-        stockhamPass<T, 16, SIGN>(buf, L2N, L2S, bt, tw);
-        radix16Passes<T, SIGN, L2N, L2S + 4>(buf, bt, tw);
+        stockhamPass<T, 1 << LR, SIGN>(buf, L2N, L2S, bt, tw);
+        radixPasses<T, SIGN, L2N, L2S + LR, LR>(buf, bt, tw);
     }
 }
 
 // log2N < 0 marks a length that is not a power of two.  L2N > 0 fixes the
 // length 2^L2N at compile time (N and log2N are then ignored): every pass
-// and every index shift is a constant.
-template <typename T, int SIGN, int L2N = 0, typename Bt>
+// and every index shift is a constant, and the passes are of radix 2^LR
+// (LR = 4 or 2; the leading pass takes the remaining factor) with the
+// twiddle table of that pass sequence.
+template <typename T, int SIGN, int L2N = 0, int LR = 4, typename Bt>
 __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
     if constexpr (L2N > 0) {
-        constexpr int lead = L2N & 3;
+        constexpr int lead = L2N % LR;
         if constexpr (lead != 0) stockhamPass<T, 1 << lead, SIGN>(buf, L2N, 0, bt, tw);
-        radix16Passes<T, SIGN, L2N, lead>(buf, bt, tw);
+        radixPasses<T, SIGN, L2N, lead, LR>(buf, bt, tw);
         return;
     }
     if (log2N < 0) {

@@ -484,10 +484,10 @@ template <typename T>
 __host__ __device__ constexpr int xfusedStride(int Nx) { return ldsLength(Nx); }
 template <typename T>
 __host__ __device__ constexpr int xfusedRowStride(int Nx, int pairs) { return pairs * xfusedStride<T>(Nx); }
-// LDS bytes of a k_xfused block of RB rows (twiddles + transforms)
+// LDS bytes of a k_xfused block of RB rows (two twiddle tables + transforms)
 template <typename T>
 __host__ __device__ constexpr size_t xfusedLdsBytes(int Nx, int pairs, int RB) {
-    return (static_cast<size_t>(Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(Nx, pairs)) * sizeof(cx<T>);
+    return (2 * static_cast<size_t>(Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(Nx, pairs)) * sizeof(cx<T>);
 }
 // rows per k_xfused block: two while four blocks still fit a CU's 160 KB of LDS
 template <typename T>
@@ -496,9 +496,9 @@ __host__ __device__ constexpr int xfusedRows(int Nx, int pairs) {
 }
 
 // Spatial channels of the flux products, packed two per complex x transform.
-// Pair p takes half-spectrum channel srcA[p] (real part of the transform) and
-// srcB[p] (imaginary part, -1 = none), each multiplied by i k when ik*[p] is
-// set.  A complex FFT's rounding error scales with the larger of its two
+// Pair p takes half-spectrum channel srcA[p] (real part of the transform),
+// multiplied by i k when ikA[p] is set, and srcB[p] (imaginary part, -1 =
+// none).  A complex FFT's rounding error scales with the larger of its two
 // signals, so each pair holds quantities of one kind and size (the reference
 // transforms every channel on its own).  Products: F_t = -(U d/dx T +
 // V d/dy T + W d/dz T) with the spatial channel numbers (2p + part) of
@@ -515,7 +515,6 @@ struct XMap<4> {
     static constexpr int srcA[8] = {0, 0, 1, 2, 3, 8, 2, 11};
     static constexpr int ikA[8] = {0, 1, 1, 1, 1, 0, 0, 0};
     static constexpr int srcB[8] = {1, 4, 5, 6, 7, 9, 10, -1};
-    static constexpr int ikB[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     static constexpr int uvw[3] = {0, 1, 12};
     static constexpr int prod[4][3] = {{2, 3, 10}, {4, 5, 11}, {6, 7, 13}, {8, 9, 14}};
 };
@@ -525,7 +524,6 @@ struct XMap<3> {
     static constexpr int srcA[7] = {0, 0, 1, 3, 7, 2, 9};
     static constexpr int ikA[7] = {0, 1, 1, 1, 0, 0, 0};
     static constexpr int srcB[7] = {1, 4, 5, 6, 8, -1, -1};
-    static constexpr int ikB[7] = {0, 0, 0, 0, 0, 0, 0};
     static constexpr int uvw[3] = {0, 1, 10};
     static constexpr int prod[3][3] = {{2, 3, 8}, {4, 5, 9}, {6, 7, 12}};
 };
@@ -533,11 +531,14 @@ struct XMap<3> {
 // Per (y, z) row: c2r of the spatial channels along x (two real transforms
 // per complex FFT), the NT advection products and r2c of the products.
 // kxv[k] = the x wavenumber of half-spectrum column k.  L2N > 0 fixes
-// Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored).
+// Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored);
+// the forward transforms then run radix-4 passes (twX4), so that all threads
+// share their four transforms.
 template <typename T, int NT, int RB, int L2N>
 __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P,
                                                         int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
-                                                        const cx<T>* __restrict__ twX, const T* __restrict__ kxv,
+                                                        const cx<T>* __restrict__ twX, const cx<T>* __restrict__ twX4,
+                                                        const T* __restrict__ kxv, const cx<T>* __restrict__ zeros,
                                                         T etaScale) {
     using XM = XMap<NT>;
     constexpr int PAIRS = XM::pairs;
@@ -547,11 +548,13 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     const int NxP = xfusedStride<T>(Nx);
     const int RS = xfusedRowStride<T>(Nx, PAIRS);
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
-    cx<T>* buf = tw + Nx;  // [RB][RS], row rr holds PAIRS transforms of stride NxP
+    cx<T>* tw4 = tw + Nx;
+    cx<T>* buf = tw4 + Nx;  // [RB][RS], row rr holds PAIRS transforms of stride NxP
     const int nrows = Ny * Nz;
     const int row0 = blockIdx.x * RB;  // row = z*Ny + y
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     loadTwiddles(tw, twX, Nx);
+    if (L2N > 0) loadTwiddles(tw4, twX4, Nx);
     // Transform t = rr*PAIRS + p packs spatial channels 2p (real part) and
     // 2p+1 (imaginary part) of row rr.  Wave w fills transforms w, w + 4, ...:
     // each retained entry k < nkx is read once and written at k and,
@@ -562,27 +565,25 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     const int lane = threadIdx.x & 63, wave = waveIndex();
     for (int kc = 0; kc < nkx; kc += 128) {
         cx<T> va[kTPW][2], vb[kTPW][2];
-        bool liveA[kTPW][2], liveB[kTPW][2];
 #pragma unroll
         for (int i = 0; i < kTPW; ++i) {
             const int t = wave + 4 * i;
             const int rr = t / PAIRS, p = t % PAIRS;
             const int row = row0 + rr;
-            const bool live = t < RB * PAIRS && row < nrows;
             const int sa = XM::srcA[p], sb = XM::srcB[p];
-            // addresses clamped into the arrays so that every load is issued
-            // unconditionally (no branch, no wait between them)
-            const size_t rowOff = static_cast<size_t>(row < nrows ? row : nrows - 1) * nkxP;
-            const cx<T>* A = G + sa * chStride + rowOff;
-            const cx<T>* B = G + (sb < 0 ? sa : sb) * chStride + rowOff;
+            // a missing channel or row reads the zero row, so the values need
+            // no select; addresses are clamped into the arrays so that every
+            // load is issued unconditionally (no branch, no wait between them)
+            const bool live = t < RB * PAIRS && row < nrows;
+            const size_t rowOff = static_cast<size_t>(row) * nkxP;
+            const cx<T>* A = live ? G + sa * chStride + rowOff : zeros;
+            const cx<T>* B = live && sb >= 0 ? G + sb * chStride + rowOff : zeros;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = kc + lane + 64 * h;
                 const int kcl = k < nkx ? k : nkx - 1;
                 va[i][h] = A[kcl];
                 vb[i][h] = B[kcl];
-                liveA[i][h] = live && k < nkx;
-                liveB[i][h] = live && k < nkx && sb >= 0;
             }
         }
         T kk[2];
@@ -597,18 +598,20 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
             const int t = wave + 4 * i;
             if (t >= RB * PAIRS) continue;
             const int p = t % PAIRS;
+            const bool ik = XM::ikA[p] != 0;
             cx<T>* z = buf + (t / PAIRS) * RS + p * NxP;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = kc + lane + 64 * h;
                 if (k >= nkx) continue;
-                cx<T> a = liveA[i][h] ? va[i][h] : cx<T>{T(0), T(0)};
-                cx<T> b = liveB[i][h] ? vb[i][h] : cx<T>{T(0), T(0)};
-                if (XM::ikA[p]) a = cmuli(a, kk[h]);
-                if (XM::ikB[p]) b = cmuli(b, kk[h]);
-                if (k == 0) a.im = b.im = T(0);
-                z[ldsIndex(k, 1)] = {a.re - b.im, a.im + b.re};
-                if (k > 0) z[ldsIndex(Nx - k, 1)] = {a.re + b.im, b.re - a.im};
+                cx<T> a = va[i][h];
+                const cx<T> b = vb[i][h];
+                if (ik) a = cmuli(a, kk[h]);
+                // k = 0: real parts only (the imaginary part of the kx = 0
+                // column is ignored by the c2r transform)
+                const T aim = (h == 0 && k == 0) ? T(0) : a.im, bim = (h == 0 && k == 0) ? T(0) : b.im;
+                z[ldsIndex(k, 1)] = {a.re - bim, aim + b.re};
+                if (k > 0) z[ldsIndex(Nx - k, 1)] = {a.re + bim, b.re - aim};
             }
         }
     }
@@ -642,7 +645,10 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     }
     __syncthreads();
     // forward transforms: pairs 0 and 1 of every row
-    fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
+    if constexpr (L2N > 0)
+        fftLds<T, -1, L2N, 2>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw4);
+    else
+        fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
     // wave w unpacks (row, pair) rq = w, w + nwaves, ...; its lanes walk k
     constexpr int nwaves = kFftThreads / 64;
     for (int rq = waveIndex(); rq < RB * 2; rq += nwaves)
@@ -1166,7 +1172,7 @@ struct KernelImpl {
 
     DeviceBuffer cols, omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled, ApmD, ApmTE, A0TE,
         apmW, inertialScale, fWaveScale, gWaveScale, Fg, Gg;
-    DeviceBuffer synC, synS, anaC, anaS, synGz, synFh, synEz, twX, twY, mj, kxv;
+    DeviceBuffer synC, synS, anaC, anaS, synGz, synFh, synEz, twX, twX4, twY, mj, kxv, zeroRow;
     DeviceBuffer FV, H, G, P, Q, Tg, modeTab, stage, weighted, partial, scalar;
     DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
     double retainedRowFraction = 1.0;
@@ -1185,7 +1191,7 @@ struct KernelImpl {
         size_t s = 0;
         for (const DeviceBuffer* b : {&rowLimit, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &synGz, &synFh, &synEz, &twX, &twY, &mj, &kxv, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &stage, &weighted, &partial, &scalar})
+                                      &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
         return s;
     }
@@ -1263,6 +1269,8 @@ struct KernelImpl {
             uploadAs<T>(synEz, mfmaOperand(d.synEz, d.Zs, d.Js));
         }
         uploadAs<T>(twX, d.twX);
+        uploadAs<T>(twX4, d.twX4);
+        uploadAs<T>(zeroRow, std::vector<double>(2 * d.nkxP, 0.0));  // a row of zero spectra
         uploadAs<T>(twY, d.twY);
         uploadAs<T>(mj, d.m);
         uploadAs<T>(kxv, d.kxv);
@@ -1498,7 +1506,8 @@ struct KernelImpl {
         const T es = static_cast<T>(std::ldexp(1.0, static_cast<int>(std::lround(std::log2(d.c.N0 > 0 ? d.c.N0 : 1.0)))));
         hipLaunchKernelGGL((k_xfused<T, NTc, RB, L2N>), dim3(static_cast<unsigned>((nrows + RB - 1) / RB)),
                            dim3(kFftThreads), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, log2Exact(d.c.Nx), Ny, Nz,
-                           static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), kxv.as<T>(), es);
+                           static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), twX4.as<cx<T>>(),
+                           kxv.as<T>(), zeroRow.as<cx<T>>(), es);
     }
 
     // Nx = 64 ... 512 run kernels compiled for that length; others the
