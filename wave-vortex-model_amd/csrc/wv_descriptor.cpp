@@ -289,8 +289,10 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile, const Stratif
     // (...Kernel.cpp:176-193): scale 1/(Nz-1), DST endpoints zeroed.
     const size_t L = Nz - 1;
     Zs = roundUp(Nz, 16);
-    Js = roundUp(Nj, 16);  // synthesis contraction: whole batches of 4 MFMA k-steps
-    Ja = roundUp(Nj, 16);
+    // contraction lengths of the MFMA synthesis and analysis: whole batches of
+    // 4 k-steps, also after splitting j by parity
+    Js = roundUp(Nj, 32);
+    Ja = roundUp(Nj, 32);
     Za = roundUp(Nz, 16);  // analysis contraction: whole batches of 4 MFMA k-steps
     synC.assign(Zs * Js, 0.0);
     synS.assign(Zs * Js, 0.0);

@@ -227,8 +227,16 @@ __device__ __forceinline__ double pairSwap(double v) {
 // length Js is a multiple of 16, so the step loop runs in whole batches with
 // no per-step conditions; channel slots beyond the group's channels and z
 // tiles beyond the last carry zero coefficients or are not stored.
-template <typename T, int ZT, int NQ, int KS>
-__global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
+//
+// FOLD (constant N, the DCT-I / DST-I of Nz - 1 = L intervals): row L - z of
+// those matrices is row z times (-1)^j (DCT-I) or -(-1)^j (DST-I), so with
+// E_z and O_z the sums over even and odd j the outputs are E + O at z and
+// E - O (DCT-I) or O - E (DST-I) at L - z, for z <= L/2 only: half the MFMA
+// work.  mats then holds the even-j parts of the two matrices (rows z <=
+// L/2, Zs of them) in m[0], m[1] and the odd-j parts in m[2], m[3]; a task
+// is one z tile of one group (ZT is 1).
+template <typename T, int ZT, int NQ, int KS, bool FOLD>
+__global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
                                                int ncol, int M, SynthMats<T> mats, int Zs, int Js, int Nz,
                                                int Nj, int Ny, int nkxP, const T* __restrict__ mj, SynthPlan plan,
                                                cx<T>* __restrict__ H) {
@@ -247,7 +255,7 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
     const int cg = xcdBlock(blockIdx.x, gridDim.x);
     const int ztiles = Zs / 16;
     const int nzc = (ztiles + ZT - 1) / ZT;
-    const int nsteps = Js / 4;
+    const int nsteps = FOLD ? Js / 8 : Js / 4;  // folded: Js/2 even (odd) j per sum
     if (threadIdx.x < 8) {
         const int col = cg * 8 + threadIdx.x;
         if (col < ncol) scol[threadIdx.x] = cols[col];
@@ -298,6 +306,103 @@ __global__ __launch_bounds__(256) void k_synth(const cx<T>* __restrict__ FV, con
     const size_t chStride = static_cast<size_t>(Nz) * zStride;
     // lane reads the real (part 0) or imaginary (part 1) word of fv[.][c][j]
     const T* fvw = reinterpret_cast<const T*>(fv) + 2 * c * Jp + part;
+    if constexpr (FOLD) {
+        const int L = Nz - 1, half = L / 2;
+        for (int task = waveInBlock; task < plan.ngroups * ztiles; task += nwaves) {
+            const int gi = task % plan.ngroups;
+            const int zt = task / plan.ngroups;
+            const int mat = plan.g[gi].mat, nch = plan.g[gi].nch;
+            int fieldOff[NQ];
+            T coefV[NQ], coefZ[NQ];
+            const T zsign = mat == 1 ? T(-0.5) : T(0.5);
+            const T vscale = T(plan.valueScale);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const bool live = q < nch;
+                const bool dz = live && plan.g[gi].ch[q].dz;
+                fieldOff[q] = live ? plan.g[gi].ch[q].field * 16 * Jp : 0;
+                coefV[q] = live && !dz ? vscale * partWeight : T(0);
+                coefZ[q] = dz ? zsign * partWeight : T(0);
+            }
+            const T* AE = mats.m[mat] + static_cast<size_t>(zt) * nsteps * 64 + lane;
+            const T* AO = mats.m[mat + 2] + static_cast<size_t>(zt) * nsteps * 64 + lane;
+            typename MF::acc accE[NQ], accO[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) accE[q] = accO[q] = MF::zero();
+            constexpr int kS = KS;
+            auto load = [&](T (&dE)[kS], T (&dO)[kS], int s0) {
+#pragma unroll
+                for (int u = 0; u < kS; ++u) {
+                    dE[u] = AE[(s0 + u) * 64];
+                    dO[u] = AO[(s0 + u) * 64];
+                }
+            };
+            auto run = [&](const T (&aE)[kS], const T (&aO)[kS], int s0) {
+                T bE[kS][NQ], bO[kS][NQ];
+#pragma unroll
+                for (int u = 0; u < kS; ++u) {
+                    const int je = 2 * (4 * (s0 + u) + kq);  // even j; the odd one is je + 1
+                    const T mE = mjs[je], mO = mjs[je + 1];
+                    const T* fj = fvw + 2 * je;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        bE[u][q] = fj[fieldOff[q]] * (coefV[q] + coefZ[q] * mE);
+                        bO[u][q] = fj[fieldOff[q] + 2] * (coefV[q] + coefZ[q] * mO);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kS; ++u)
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        accE[q] = MF::run(aE[u], bE[u][q], accE[q]);
+                        accO[q] = MF::run(aO[u], bO[u][q], accO[q]);
+                    }
+            };
+            T e0[kS], o0[kS], e1[kS], o1[kS];
+            load(e0, o0, 0);
+            int s0 = 0;
+            for (; s0 + 2 * kS < nsteps; s0 += 2 * kS) {
+                load(e1, o1, s0 + kS);
+                run(e0, o0, s0);
+                load(e0, o0, s0 + 2 * kS);
+                run(e1, o1, s0 + kS);
+            }
+            load(e1, o1, s0 + kS);
+            run(e0, o0, s0);
+            run(e1, o1, s0 + kS);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                if (q >= nch) continue;
+                const int oV = plan.g[gi].ch[q].outV, oX = plan.g[gi].ch[q].outX, oY = plan.g[gi].ch[q].outY;
+                T* hV = reinterpret_cast<T*>(H) + (oV >= 0 ? oV : 0) * chStride + colOff;
+                T* hX = reinterpret_cast<T*>(H) + (oX >= 0 ? oX : 0) * chStride + colOff;
+                T* hY = reinterpret_cast<T*>(H) + (oY >= 0 ? oY : 0) * chStride + colOff;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int z = 16 * zt + MF::row(kq, i);
+                    const T e = accE[q][i], o = accO[q][i];
+                    const T v1 = e + o, v2 = mat == 0 ? e - o : o - e;  // at z and at L - z
+                    // partner values first, with every lane active (a DPP
+                    // read from an inactive lane returns nothing)
+                    const T w1 = pairSwap(v1), w2 = pairSwap(v2);
+                    const T r1 = part ? w1 : -w1;  // part of i * value
+                    const T r2 = part ? w2 : -w2;
+                    if (z <= half && store) {
+                        const int o1 = z * zStride, o2 = (L - z) * zStride;
+                        if (oV >= 0) hV[o1] = v1;
+                        if (oX >= 0) hX[o1] = ks * r1;
+                        if (oY >= 0) hY[o1] = ls * r1;
+                        if (L - z != z) {
+                            if (oV >= 0) hV[o2] = v2;
+                            if (oX >= 0) hX[o2] = ks * r2;
+                            if (oY >= 0) hY[o2] = ls * r2;
+                        }
+                    }
+                }
+            }
+        }
+        return;
+    }
     for (int task = waveInBlock; task < plan.ngroups * nzc; task += nwaves) {
         const int gi = task % plan.ngroups;
         const int zc = task / plan.ngroups;
@@ -699,11 +804,12 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
 template <int NT>
 __host__ __device__ constexpr int projTarget(int t) { return NT == 4 ? t : (t == 2 ? 3 : t); }
 
-// LDS bytes of k_proj: the staged columns, later the transposed results.
+// LDS bytes of k_proj: the staged columns (Zq rows per target), later the
+// transposed results.
 template <typename T>
-constexpr size_t projLdsBytes(int NT, int Za, int Ja) {
-    const size_t staged = static_cast<size_t>(NT) * Za * 8 * sizeof(cx<T>);
-    const size_t results = static_cast<size_t>(Ja / 16) * 4 * 16 * 17 * sizeof(T);
+constexpr size_t projLdsBytes(int NT, int Zq, int Ja) {
+    const size_t staged = static_cast<size_t>(NT) * Zq * 8 * sizeof(cx<T>);
+    const size_t results = static_cast<size_t>(Ja) * 4 * 17 * sizeof(T);
     return staged > results ? staged : results;
 }
 
@@ -713,15 +819,21 @@ constexpr size_t projLdsBytes(int NT, int Za, int Ja) {
 // wave-vortex projection follow in k_stage).
 // One workgroup owns 8 consecutive columns and has one wave per 16-row tile
 // of j; each target's Q columns (all z) are staged in LDS once and shared by
-// the waves.
-template <typename T, int NT, int KS>
+// the waves.  ana.m[0] and m[1] are the cosine- and sine-family analysis
+// matrices ([Ja x Za]).
+// FOLD (constant N): column L - z of the DCT-I / DST-I analysis (L = Nz - 1)
+// is column z times (-1)^j or -(-1)^j, so rows of even j contract the sums
+// s_z = q_z + q_{L-z} (DCT-I) or the differences d_z = q_z - q_{L-z}
+// (DST-I) over z <= L/2, rows of odd j the other: half the MFMA work.  Za is
+// then the folded row count, ana.m[0], m[1] hold the even-j rows and m[2],
+// m[3] the odd-j rows, and the first Ja/32 waves take even j.
+template <typename T, int NT, int KS, bool FOLD>
 __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, const ColumnDev* __restrict__ cols,
-                                               int ncol, const T* __restrict__ anaC, const T* __restrict__ anaS,
-                                               int Ja, int Za, int Nz, int Nj, int Ny, int nkxP, int M,
-                                               cx<T>* __restrict__ Tg) {
+                                               int ncol, SynthMats<T> ana, int Ja, int Za, int Nz, int Nj, int Ny,
+                                               int nkxP, int M, cx<T>* __restrict__ Tg) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [NT][Za][8]
+    cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [NT][Za][8], folded [NT][2][Za][8] (sums, differences)
     __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
     const int jt = waveIndex();
@@ -735,15 +847,16 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     const int part = n & 1;
     const size_t tStride = static_cast<size_t>(Nz) * Ny * nkxP;
     const int nsteps = Za / 4;
+    const int L = Nz - 1;
     __syncthreads();  // scol ready
-    // stage the columns of every target, qs[t][z][c]; a thread issues kB
-    // loads before storing them
+    // stage the columns of every target; a thread issues all loads of kB
+    // entries (two each when folded) before storing them
     {
         constexpr int kB = 4;
         const int tot = NT * Za * 8;
         for (int e0 = 0; e0 < tot; e0 += kB * blockDim.x) {
-            cx<T> v[kB];
-            bool ok[kB];
+            cx<T> v[kB], w[kB];
+            bool ok[kB], pair[kB];
 #pragma unroll
             for (int i = 0; i < kB; ++i) {
                 const int e = e0 + threadIdx.x + i * blockDim.x;
@@ -751,15 +864,35 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
                 const int c = ec & 7, zt = ec >> 3;
                 const int t = zt / Za, z = zt - t * Za;
                 const ColumnDev& cc = scol[c];
+                const size_t colBase = t * tStride + static_cast<size_t>(cc.iy) * nkxP + cc.kx;
+                const size_t plane = static_cast<size_t>(Ny) * nkxP;
                 const int zc = z < Nz ? z : Nz - 1;
-                v[i] = Q[t * tStride + (static_cast<size_t>(zc) * Ny + cc.iy) * nkxP + cc.kx];
-                ok[i] = e < tot && cc.wv >= 0 && z < Nz;
+                v[i] = Q[colBase + zc * plane];
+                if constexpr (FOLD) {
+                    const int zm = L - zc >= 0 ? L - zc : 0;
+                    w[i] = Q[colBase + zm * plane];
+                    ok[i] = e < tot && cc.wv >= 0 && 2 * z <= L;
+                    pair[i] = z < L - z;
+                } else {
+                    ok[i] = e < tot && cc.wv >= 0 && z < Nz;
+                }
             }
             __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
 #pragma unroll
             for (int i = 0; i < kB; ++i) {
                 const int e = e0 + threadIdx.x + i * blockDim.x;
-                if (e < tot) qs[e] = ok[i] ? v[i] : cx<T>{T(0), T(0)};
+                if (e >= tot) continue;
+                const cx<T> zero{T(0), T(0)};
+                if constexpr (FOLD) {
+                    const int zt = e >> 3, t = zt / Za;
+                    const int r = e - t * Za * 8;  // (z, c) within the target
+                    const cx<T> sv = !ok[i] ? zero : pair[i] ? cadd(v[i], w[i]) : v[i];
+                    const cx<T> dv = !ok[i] || !pair[i] ? zero : csub(v[i], w[i]);
+                    qs[(2 * t) * Za * 8 + r] = sv;
+                    qs[(2 * t + 1) * Za * 8 + r] = dv;
+                } else {
+                    qs[e] = ok[i] ? v[i] : zero;
+                }
             }
         }
     }
@@ -770,10 +903,16 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = MF::zero();
     const T* qw = reinterpret_cast<const T*>(qs) + 2 * (n >> 1) + part;  // [t][z][c] words
+    const int nEven = Ja / 32;
+    const int P = FOLD && jt >= nEven ? 1 : 0;        // folded: parity of this wave's j
+    const int jh = FOLD ? jt - P * nEven : jt;        // row tile within the parity
     constexpr int kS = KS;  // the step count Za / 4 is a multiple of 2 kS
 #pragma unroll
     for (int fam = 0; fam < 2; ++fam) {
-        const T* Aj = (fam ? anaS : anaC) + static_cast<size_t>(jt) * nsteps * 64 + lane;
+        const T* Aj = ana.m[fam + 2 * P] + static_cast<size_t>(jh) * nsteps * 64 + lane;
+        // staged block read by this family: the columns themselves, or
+        // (folded) the sums or differences, by family and parity of j
+        const int par = FOLD ? (fam ^ P) : 0;
         // two named batches used in turn (no register copies, so the waits
         // before the MFMAs cover only the batch in use)
         auto load = [&](T (&dst)[kS], int s0) {
@@ -787,7 +926,8 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
                 const int zrow = 4 * (s0 + u) + kq;
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
-                    if ((projTarget<NT>(t) >= 2) == (fam == 1)) b[u][t] = qw[2 * ((t * Za + zrow) * 8)];
+                    if ((projTarget<NT>(t) >= 2) == (fam == 1))
+                        b[u][t] = FOLD ? qw[2 * (((2 * t + par) * Za + zrow) * 8)] : qw[2 * ((t * Za + zrow) * 8)];
             }
 #pragma unroll
             for (int u = 0; u < kS; ++u)
@@ -808,16 +948,20 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
         run(a0, s0);
         run(a1, s0 + kS);
     }
-    // The analysis results C_t[j][n] go through LDS (the staged columns are
-    // no longer needed) so that the stores' lanes walk j: lane (c, jj) takes
-    // rows j = 16 jt + 8 s + jj of column c.
-    constexpr int kResStride = 17;  // words per row of a wave's [t][16 j][16 n] block
+    // The analysis results go through LDS as res[t][j][n] (the staged columns
+    // are no longer needed) so that the stores' lanes walk j: lane (c, jj)
+    // takes rows j = 16 jt + 8 s + jj of column c.
+    constexpr int kResStride = 17;  // words per row j
     __syncthreads();  // every wave has finished reading qs
-    T* res = reinterpret_cast<T*>(smem) + jt * (4 * 16 * kResStride);
+    T* res = reinterpret_cast<T*>(smem);
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) res[(t * 16 + MF::row(kq, i)) * kResStride + n] = acc[t][i];
+        for (int i = 0; i < 4; ++i) {
+            const int r = 16 * jh + MF::row(kq, i);
+            const int j = FOLD ? 2 * r + P : r;
+            res[(t * Ja + j) * kResStride + n] = acc[t][i];
+        }
     __syncthreads();
     const int ec = lane >> 3, jj = lane & 7;
     const ColumnDev cd = scol[ec];
@@ -825,13 +969,12 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     const bool conj = (cd.flags & kColConj) != 0;
 #pragma unroll
     for (int ii = 0; ii < 2; ++ii) {
-        const int jl = 8 * ii + jj;
-        const int j = 16 * jt + jl;
+        const int j = 16 * jt + 8 * ii + jj;
         if (j >= Nj) continue;
         const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const T* rt = res + (t * 16 + jl) * kResStride + 2 * ec;
+            const T* rt = res + (t * Ja + j) * kResStride + 2 * ec;
             Tg[t * static_cast<size_t>(M) + idx] = cx<T>{rt[0], conj ? -rt[1] : rt[1]};
         }
     }
@@ -1173,6 +1316,11 @@ struct KernelImpl {
     DeviceBuffer cols, omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled, ApmD, ApmTE, A0TE,
         apmW, inertialScale, fWaveScale, gWaveScale, Fg, Gg;
     DeviceBuffer synC, synS, anaC, anaS, synGz, synFh, synEz, twX, twX4, twY, mj, kxv, zeroRow;
+    // even / odd j parts of the DCT-I, DST-I synthesis (rows z <= (Nz-1)/2) and
+    // of the analysis (columns z <= (Nz-1)/2): constant N
+    DeviceBuffer synCE, synSE, synCO, synSO, anaCE, anaCO, anaSE, anaSO;
+    bool fold = false;
+    size_t Zh = 0;  // folded z rows (multiple of 16)
     DeviceBuffer FV, H, G, P, Q, Tg, modeTab, stage, weighted, partial, scalar;
     DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
     double retainedRowFraction = 1.0;
@@ -1191,7 +1339,7 @@ struct KernelImpl {
         size_t s = 0;
         for (const DeviceBuffer* b : {&rowLimit, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &stage, &weighted, &partial, &scalar})
+                                      &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
         return s;
     }
@@ -1268,6 +1416,31 @@ struct KernelImpl {
             uploadAs<T>(synFh, mfmaOperand(d.synFh, d.Zs, d.Js));
             uploadAs<T>(synEz, mfmaOperand(d.synEz, d.Zs, d.Js));
         }
+        if (fold) {
+            // rows z <= half of the synthesis, split by the parity of j
+            const size_t half = (d.c.Nz - 1) / 2, Jh = d.Js / 2, Jah = d.Ja / 2;
+            Zh = (half + 1 + 15) / 16 * 16;
+            auto synPart = [&](const std::vector<double>& m, size_t par) {
+                std::vector<double> out(Zh * Jh, 0.0);
+                for (size_t z = 0; z <= half; ++z)
+                    for (size_t jh = 0; jh < Jh; ++jh) out[z * Jh + jh] = m[z * d.Js + 2 * jh + par];
+                return mfmaOperand(out, Zh, Jh);
+            };
+            auto anaPart = [&](const std::vector<double>& m, size_t par) {
+                std::vector<double> out(Jah * Zh, 0.0);
+                for (size_t jh = 0; jh < Jah; ++jh)
+                    for (size_t z = 0; z <= half; ++z) out[jh * Zh + z] = m[(2 * jh + par) * d.Za + z];
+                return mfmaOperand(out, Jah, Zh);
+            };
+            uploadAs<T>(synCE, synPart(d.synC, 0));
+            uploadAs<T>(synSE, synPart(d.synS, 0));
+            uploadAs<T>(synCO, synPart(d.synC, 1));
+            uploadAs<T>(synSO, synPart(d.synS, 1));
+            uploadAs<T>(anaCE, anaPart(d.anaC, 0));
+            uploadAs<T>(anaCO, anaPart(d.anaC, 1));
+            uploadAs<T>(anaSE, anaPart(d.anaS, 0));
+            uploadAs<T>(anaSO, anaPart(d.anaS, 1));
+        }
         uploadAs<T>(twX, d.twX);
         uploadAs<T>(twX4, d.twX4);
         uploadAs<T>(zeroRow, std::vector<double>(2 * d.nkxP, 0.0));  // a row of zero spectra
@@ -1308,9 +1481,10 @@ struct KernelImpl {
         const size_t yLds = (static_cast<size_t>(KT) + 1) * cfg.Ny * elem;
         if (xLds > static_cast<size_t>(maxLds) || yLds > static_cast<size_t>(maxLds))
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid needs more LDS per workgroup than the device provides."};
-        const size_t pLds = prec == WV_PRECISION_FP64
-                                ? projLdsBytes<double>(NT, static_cast<int>(d.Za), static_cast<int>(d.Ja))
-                                : projLdsBytes<float>(NT, static_cast<int>(d.Za), static_cast<int>(d.Ja));
+        // staged rows: Za, or 2 x ((Nz-1)/2 + 1) rounded up to 16 when folded
+        const int Zq = static_cast<int>(std::max(d.Za, 2 * (((d.c.Nz - 1) / 2 + 1 + 15) / 16 * 16)));
+        const size_t pLds = prec == WV_PRECISION_FP64 ? projLdsBytes<double>(NT, Zq, static_cast<int>(d.Ja))
+                                                      : projLdsBytes<float>(NT, Zq, static_cast<int>(d.Ja));
         if (d.Ja > 256 || pLds > static_cast<size_t>(maxLds))
             throw Failure{WV_UNSUPPORTED_OPERATION, "Vertical resolution too large for the in-LDS analysis (Nj <= 256)."};
         // columns
@@ -1356,6 +1530,7 @@ struct KernelImpl {
         upload(gWaveScale, d.gWaveScale);
         upload(Fg, d.Fg);
         upload(Gg, d.Gg);
+        fold = !d.stratified;
         if (precision == WV_PRECISION_FP64) setupTyped<double>();
         else setupTyped<float>();
         const size_t plane = cfg.Ny * d.nkxP;
@@ -1452,20 +1627,32 @@ struct KernelImpl {
                 nout += (sp.g[g].ch[q].outV >= 0) + (sp.g[g].ch[q].outX >= 0) + (sp.g[g].ch[q].outY >= 0);
         const double bytes = static_cast<double>(nout) * d.c.Nz * ncol * sizeof(cx<T>) + 4.0 * d.M * sizeof(cx<T>);
         const size_t lds = 4 * 8 * static_cast<size_t>(synthJStride<T>(static_cast<int>(d.Js))) * sizeof(cx<T>) + d.Js * sizeof(T);
-        // waves per workgroup: the largest of 4, 3, 2, 1 that divides the tasks
-        const int tasks = sp.ngroups * static_cast<int>((d.Zs / 16 + ZT - 1) / ZT);
-        int waves = 4;
+        // the folded DCT-I / DST-I path serves the constant-N matrices
+        const bool folded = fold && mats == nullptr;
+        const int ztiles = static_cast<int>((folded ? Zh : d.Zs) / 16);
+        // waves per workgroup: the largest of 8 ... 1 that divides the tasks
+        const int tasks = folded ? sp.ngroups * ztiles : sp.ngroups * ((ztiles + ZT - 1) / ZT);
+        int waves = folded ? 8 : 4;
         while (tasks % waves) --waves;
+        const int nsteps = static_cast<int>(folded ? d.Js / 8 : d.Js / 4);
+        const bool ks4 = nsteps % 8 == 0;
+        using Fn = void (*)(const cx<T>*, const ColumnDev*, int, int, SynthMats<T>, int, int, int, int, int, int,
+                            const T*, SynthPlan, cx<T>*);
+        Fn fn;
+        if (folded)
+            fn = maxch <= 2 ? (ks4 ? k_synth<T, 1, 2, 4, true> : k_synth<T, 1, 2, 2, true>)
+                            : (ks4 ? k_synth<T, 1, 4, 4, true> : k_synth<T, 1, 4, 2, true>);
+        else
+            fn = maxch <= 2 ? (ks4 ? k_synth<T, ZT, 2, 4, false> : k_synth<T, ZT, 2, 2, false>)
+                            : (ks4 ? k_synth<T, ZT, 4, 4, false> : k_synth<T, ZT, 4, 2, false>);
+        const SynthMats<T> m = folded ? SynthMats<T>{{synCE.as<T>(), synSE.as<T>(), synCO.as<T>(), synSO.as<T>(), nullptr}}
+                                      : (mats ? *mats : synthMats<T>());
         timed(name, bytes, s, [&] {
-            const int nsteps = static_cast<int>(d.Js / 4);
-            auto kernel = maxch <= 2 ? (nsteps % 8 == 0 ? k_synth<T, ZT, 2, 4> : k_synth<T, ZT, 2, 2>)
-                                     : (nsteps % 8 == 0 ? k_synth<T, ZT, 4, 4> : k_synth<T, ZT, 4, 2>);
-            hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s,
+            hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s,
                                in ? in->as<cx<T>>() : FV.as<cx<T>>(), cols.as<ColumnDev>(), static_cast<int>(ncol),
-                               static_cast<int>(d.M), mats ? *mats : synthMats<T>(), static_cast<int>(d.Zs),
-                               static_cast<int>(d.Js), static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj),
-                               static_cast<int>(d.c.Ny), static_cast<int>(d.nkxP), mj.as<T>(), sp,
-                               out ? out->as<cx<T>>() : H.as<cx<T>>());
+                               static_cast<int>(d.M), m, ztiles * 16, static_cast<int>(d.Js),
+                               static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
+                               static_cast<int>(d.nkxP), mj.as<T>(), sp, out ? out->as<cx<T>>() : H.as<cx<T>>());
         });
     }
 
@@ -1547,19 +1734,27 @@ struct KernelImpl {
         timed("proj", bytes, s, [&] {
             const int njt = static_cast<int>(d.Ja / 16);
             const dim3 grid(static_cast<unsigned>(ncg)), block(static_cast<unsigned>(64 * njt));
-            const size_t lds = projLdsBytes<T>(NT, static_cast<int>(d.Za), static_cast<int>(d.Ja));
-            const int ncolI = static_cast<int>(ncol), Ja = static_cast<int>(d.Ja), Za = static_cast<int>(d.Za),
-                      Nz = static_cast<int>(d.c.Nz), Nj = static_cast<int>(d.c.Nj), Ny = static_cast<int>(d.c.Ny),
-                      nkxP = static_cast<int>(d.nkxP), M = static_cast<int>(d.M);
+            const int Zq = static_cast<int>(fold ? Zh : d.Za);  // contraction rows
+            const size_t lds = projLdsBytes<T>(NT, fold ? 2 * Zq : Zq, static_cast<int>(d.Ja));
+            const int ncolI = static_cast<int>(ncol), Ja = static_cast<int>(d.Ja), Nz = static_cast<int>(d.c.Nz),
+                      Nj = static_cast<int>(d.c.Nj), Ny = static_cast<int>(d.c.Ny), nkxP = static_cast<int>(d.nkxP),
+                      M = static_cast<int>(d.M);
             // batch length: 4, 3 or 2 steps, whichever divides the step count in pairs
-            const int nsteps = Za / 4;
+            const int nsteps = Zq / 4;
             const int ks = nsteps % 8 == 0 ? 4 : nsteps % 6 == 0 ? 3 : 2;
-            using Fn = void (*)(const cx<T>*, const ColumnDev*, int, const T*, const T*, int, int, int, int, int, int,
-                                int, cx<T>*);
-            Fn fn = NT == 4 ? (ks == 4 ? k_proj<T, 4, 4> : ks == 3 ? k_proj<T, 4, 3> : k_proj<T, 4, 2>)
-                            : (ks == 4 ? k_proj<T, 3, 4> : ks == 3 ? k_proj<T, 3, 3> : k_proj<T, 3, 2>);
-            hipLaunchKernelGGL(fn, grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI, anaC.as<T>(),
-                               anaS.as<T>(), Ja, Za, Nz, Nj, Ny, nkxP, M, Tg.as<cx<T>>());
+            using Fn = void (*)(const cx<T>*, const ColumnDev*, int, SynthMats<T>, int, int, int, int, int, int, int,
+                                cx<T>*);
+            Fn fn;
+            if (fold)
+                fn = NT == 4 ? (ks == 4 ? k_proj<T, 4, 4, true> : ks == 3 ? k_proj<T, 4, 3, true> : k_proj<T, 4, 2, true>)
+                             : (ks == 4 ? k_proj<T, 3, 4, true> : ks == 3 ? k_proj<T, 3, 3, true> : k_proj<T, 3, 2, true>);
+            else
+                fn = NT == 4 ? (ks == 4 ? k_proj<T, 4, 4, false> : ks == 3 ? k_proj<T, 4, 3, false> : k_proj<T, 4, 2, false>)
+                             : (ks == 4 ? k_proj<T, 3, 4, false> : ks == 3 ? k_proj<T, 3, 3, false> : k_proj<T, 3, 2, false>);
+            const SynthMats<T> ana = fold ? SynthMats<T>{{anaCE.as<T>(), anaSE.as<T>(), anaCO.as<T>(), anaSO.as<T>(), nullptr}}
+                                          : SynthMats<T>{{anaC.as<T>(), anaS.as<T>(), nullptr, nullptr, nullptr}};
+            hipLaunchKernelGGL(fn, grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI, ana, Ja, Zq, Nz, Nj,
+                               Ny, nkxP, M, Tg.as<cx<T>>());
         });
     }
 
