@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_rk4_256x256x129_mixed.md")
 # stage name -> kernel of the mixed-precision RHS
 STAGE_KERNEL = {"fields": "k_fields<float", "synth": "k_synth<float", "xfused": "k_xfused<float",
-                "proj": "k_proj<float"}
+                "proj": "k_proj<float", "stage": "k_stage<float"}
 
 
 def step_model_bytes(Nx, Ny, Nz, M, b_spec, b_real):
