@@ -507,11 +507,21 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
 // rows r < iy < Ny - r are known zeros (skipIn) or output rows there are not
 // needed (skipOut).  L2N > 0 fixes Ny = 2^L2N and the tile width 2^L2KT at
 // compile time (the runtime Ny, log2Ny and log2KT are then ignored).
-template <typename T, int L2N, int L2KT>
+// Channel plan of the inverse y transform with y derivatives: block z takes
+// input channel src[z] to output channel src[z] and, when der[z] >= 0, also
+// i l times it to output channel der[z] (l = lrow[iy], the y wavenumber of
+// storage row iy), from the same loaded tile.
+struct YDerivPlan {
+    int src[16];
+    int der[16];
+};
+
+template <typename T, int L2N, int L2KT, bool DERIV>
 __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
                                                       int NyArg, int log2NyArg, int nkxP, int log2KTArg, int Nz,
                                                       const cx<T>* __restrict__ twY, int sign,
-                                                      const int* __restrict__ rowLimit, int skipIn, int skipOut) {
+                                                      const int* __restrict__ rowLimit, int skipIn, int skipOut,
+                                                      YDerivPlan yp, const T* __restrict__ lrow) {
     const int Ny = L2N > 0 ? 1 << L2N : NyArg;
     const int log2Ny = L2N > 0 ? L2N : log2NyArg;
     const int log2KT = L2N > 0 ? L2KT : log2KTArg;
@@ -521,40 +531,56 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
     cx<T>* tile = tw + Ny;
     const int kx0 = blockIdx.x * KT;
     const int lim = rowLimit ? rowLimit[blockIdx.x] : Ny;
-    const size_t plane = (static_cast<size_t>(blockIdx.z) * Nz + blockIdx.y) * static_cast<size_t>(Ny) * nkxP;
+    const int ch = DERIV ? yp.src[blockIdx.z] : static_cast<int>(blockIdx.z);
+    const size_t planeStride = static_cast<size_t>(Ny) * nkxP;
+    const size_t plane = (static_cast<size_t>(ch) * Nz + blockIdx.y) * planeStride;
     const int nel = Ny * KT;
     loadTwiddles(tw, twY, Ny);
-    {
-        // all of a thread's loads are issued before its LDS stores
-        constexpr int kMaxPerThread = 16;  // nel <= 16 * blockDim (checked at setup)
-        cx<T> v[kMaxPerThread];
+    // all of a thread's loads are issued before its LDS stores
+    constexpr int kMaxPerThread = 16;  // nel <= 16 * blockDim (checked at setup)
+    cx<T> v[kMaxPerThread];
 #pragma unroll
-        for (int i = 0; i < kMaxPerThread; ++i) {
-            const int e = fftThread() + i * kFftThreads;
-            const int ec = e < nel ? e : nel - 1;  // clamped: loads are unconditional
-            const int kk = ec & (KT - 1), iy = ec >> log2KT;
-            // rows known to be zero read row 0 instead (loaded by other lanes)
-            const bool zero = skipIn && iy > lim && iy < Ny - lim;
-            v[i] = in[plane + static_cast<size_t>(zero ? 0 : iy) * nkxP + kx0 + kk];
-        }
-        __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
+    for (int i = 0; i < kMaxPerThread; ++i) {
+        const int e = fftThread() + i * kFftThreads;
+        const int ec = e < nel ? e : nel - 1;  // clamped: loads are unconditional
+        const int kk = ec & (KT - 1), iy = ec >> log2KT;
+        // rows known to be zero read row 0 instead (loaded by other lanes)
+        const bool zero = skipIn && iy > lim && iy < Ny - lim;
+        v[i] = in[plane + static_cast<size_t>(zero ? 0 : iy) * nkxP + kx0 + kk];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
+    auto fill = [&](bool deriv) {
 #pragma unroll
         for (int i = 0; i < kMaxPerThread; ++i) {
             const int e = fftThread() + i * kFftThreads;
             const int iy = e >> log2KT;
             const bool zero = skipIn && iy > lim && iy < Ny - lim;
-            if (e < nel) tile[e] = zero ? cx<T>{T(0), T(0)} : v[i];
+            cx<T> x = v[i];
+            if (deriv) {
+                const T l = lrow[iy < Ny ? iy : Ny - 1];
+                x = cx<T>{l * -x.im, l * x.re};  // i l x
+            }
+            if (e < nel) tile[e] = zero ? cx<T>{T(0), T(0)} : x;
         }
-    }
-    __syncthreads();
-    if (sign > 0)
-        fftLds<T, +1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
-    else
-        fftLds<T, -1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
-    for (int e = fftThread(); e < nel; e += kFftThreads) {
-        const int kk = e & (KT - 1), iy = e >> log2KT;
-        if (skipOut && iy > lim && iy < Ny - lim) continue;
-        out[plane + static_cast<size_t>(iy) * nkxP + kx0 + kk] = tile[e];
+    };
+    auto transform = [&](size_t outPlane) {
+        __syncthreads();
+        if (sign > 0)
+            fftLds<T, +1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
+        else
+            fftLds<T, -1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
+        for (int e = fftThread(); e < nel; e += kFftThreads) {
+            const int kk = e & (KT - 1), iy = e >> log2KT;
+            if (skipOut && iy > lim && iy < Ny - lim) continue;
+            out[outPlane + static_cast<size_t>(iy) * nkxP + kx0 + kk] = tile[e];
+        }
+    };
+    fill(false);
+    transform(plane);
+    if (DERIV && yp.der[blockIdx.z] >= 0) {
+        __syncthreads();  // the tile is refilled
+        fill(true);
+        transform((static_cast<size_t>(yp.der[blockIdx.z]) * Nz + blockIdx.y) * planeStride);
     }
 }
 
@@ -1323,6 +1349,8 @@ struct KernelImpl {
     size_t Zh = 0;  // folded z rows (multiple of 16)
     DeviceBuffer FV, H, G, P, Q, Tg, modeTab, stage, weighted, partial, scalar;
     DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
+    DeviceBuffer lrow;      // per storage row iy: its y wavenumber (rowSign * l of its columns)
+    bool lrowValid = false;
     double retainedRowFraction = 1.0;
     size_t maxLdsPerBlock = 65536;
 
@@ -1337,7 +1365,7 @@ struct KernelImpl {
 
     size_t persistentBytes() const {
         size_t s = 0;
-        for (const DeviceBuffer* b : {&rowLimit, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
+        for (const DeviceBuffer* b : {&rowLimit, &lrow, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
                                       &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
@@ -1445,6 +1473,23 @@ struct KernelImpl {
         uploadAs<T>(twX4, d.twX4);
         uploadAs<T>(zeroRow, std::vector<double>(2 * d.nkxP, 0.0));  // a row of zero spectra
         uploadAs<T>(twY, d.twY);
+        {
+            // y wavenumber of each storage row, as the synthesis forms it
+            // (rowSign * l of every column of the row); valid when all
+            // columns of a row agree
+            std::vector<double> l(d.c.Ny, 0.0);
+            std::vector<char> seen(d.c.Ny, 0);
+            lrowValid = true;
+            for (const Column& c : d.columns) {
+                const double sgn = (c.flags & kColConj) ? -1.0 : 1.0;
+                const double v = sgn * d.modes[static_cast<size_t>(c.wv)].l;
+                const size_t iy = static_cast<size_t>(c.iy);
+                if (seen[iy] && l[iy] != v) lrowValid = false;
+                l[iy] = v;
+                seen[iy] = 1;
+            }
+            uploadAs<T>(lrow, l);
+        }
         uploadAs<T>(mj, d.m);
         uploadAs<T>(kxv, d.kxv);
         tsize = csize<T>();
@@ -1584,6 +1629,26 @@ struct KernelImpl {
         return sp;
     }
 
+    // The flux plan without the y-derivative outputs, which the inverse y
+    // transform forms instead (plan in *yp; false when the storage rows'
+    // y wavenumbers are not one per row).
+    bool fluxPlanLean(SynthPlan& sp, YDerivPlan& yp) const {
+        if (!lrowValid) return false;
+        sp = fluxPlan();
+        yp = YDerivPlan{};
+        int n = 0;
+        for (int g = 0; g < sp.ngroups; ++g)
+            for (int q = 0; q < sp.g[g].nch; ++q) {
+                SynthChannel& c = sp.g[g].ch[q];
+                if (c.outX >= 0 || c.outV < 0) return false;
+                yp.src[n] = c.outV;
+                yp.der[n] = c.outY;
+                ++n;
+                c.outY = -1;
+            }
+        return n <= 16;
+    }
+
     template <typename T>
     SynthMats<T> synthMats() const {
         return SynthMats<T>{{synC.as<T>(), synS.as<T>(), d.stratified ? synGz.as<T>() : nullptr,
@@ -1657,28 +1722,36 @@ struct KernelImpl {
     }
 
     // skipIn: input rows outside the tile's retained ky band are zeros;
-    // skipOut: output rows outside it are not needed.
+    // skipOut: output rows outside it are not needed.  yp (inverse flux
+    // transform): the channel plan with y derivatives; nch is then its
+    // number of input channels.
     template <typename T>
     void launchYfft(const DeviceBuffer& in, DeviceBuffer& out, int nch, int sign, hipStream_t s, const char* name,
-                    bool skipIn, bool skipOut) {
+                    bool skipIn, bool skipOut, const YDerivPlan* yp = nullptr) {
         const size_t lds = (static_cast<size_t>(KT) + 1) * d.c.Ny * sizeof(cx<T>);
         const double frac = (skipIn || skipOut) ? 0.5 * (1.0 + retainedRowFraction) : 1.0;
-        const double bytes = 2.0 * frac * nch * d.c.Nz * d.c.Ny * d.nkxP * sizeof(cx<T>);
+        int nout = nch;
+        if (yp)
+            for (int c = 0; c < nch; ++c) nout += yp->der[c] >= 0;
+        const double bytes = frac * (nch + nout) * static_cast<double>(d.c.Nz) * d.c.Ny * d.nkxP * sizeof(cx<T>);
         // Ny = 64 ... 1024 with their default tile widths run kernels
         // compiled for that length; others the runtime-length kernel.
-        using Fn = void (*)(const cx<T>*, cx<T>*, int, int, int, int, int, const cx<T>*, int, const int*, int, int);
+        using Fn = void (*)(const cx<T>*, cx<T>*, int, int, int, int, int, const cx<T>*, int, const int*, int, int,
+                            YDerivPlan, const T*);
         const int l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
-        Fn fn = k_yfft<T, 0, 0>;
-        if (l2y == 6 && l2t == 4) fn = k_yfft<T, 6, 4>;
-        if (l2y == 7 && l2t == 4) fn = k_yfft<T, 7, 4>;
-        if (l2y == 8 && l2t == 4) fn = k_yfft<T, 8, 4>;
-        if (l2y == 9 && l2t == 3) fn = k_yfft<T, 9, 3>;
-        if (l2y == 10 && l2t == 2) fn = k_yfft<T, 10, 2>;
+        const bool dv = yp != nullptr;
+        Fn fn = dv ? k_yfft<T, 0, 0, true> : k_yfft<T, 0, 0, false>;
+        if (l2y == 6 && l2t == 4) fn = dv ? k_yfft<T, 6, 4, true> : k_yfft<T, 6, 4, false>;
+        if (l2y == 7 && l2t == 4) fn = dv ? k_yfft<T, 7, 4, true> : k_yfft<T, 7, 4, false>;
+        if (l2y == 8 && l2t == 4) fn = dv ? k_yfft<T, 8, 4, true> : k_yfft<T, 8, 4, false>;
+        if (l2y == 9 && l2t == 3) fn = dv ? k_yfft<T, 9, 3, true> : k_yfft<T, 9, 3, false>;
+        if (l2y == 10 && l2t == 2) fn = dv ? k_yfft<T, 10, 2, true> : k_yfft<T, 10, 2, false>;
+        const YDerivPlan plan = yp ? *yp : YDerivPlan{};
         timed(name, bytes, s, [&] {
             hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(d.nkxP / KT), static_cast<unsigned>(d.c.Nz), nch),
                                dim3(kFftThreads), lds, s, in.as<cx<T>>(), out.as<cx<T>>(), static_cast<int>(d.c.Ny),
                                l2y, static_cast<int>(d.nkxP), l2t, static_cast<int>(d.c.Nz), twY.as<cx<T>>(), sign,
-                               rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0);
+                               rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0, plan, lrow.as<T>());
         });
     }
 
@@ -1794,8 +1867,17 @@ struct KernelImpl {
              hipStream_t s, bool fieldsReady = false) {
         const double elapsed = t - t0;
         if (!fieldsReady) launchFields<T>(Ap, Am, A0, elapsed, s);
-        launchSynth<T>(fluxPlan(), s);
-        launchYfft<T>(H, G, NH, +1, s, "yfft_inv", true, false);
+        SynthPlan lean;
+        YDerivPlan yp;
+        if (fluxPlanLean(lean, yp)) {
+            int nsrc = 0;
+            for (int g = 0; g < lean.ngroups; ++g) nsrc += lean.g[g].nch;
+            launchSynth<T>(lean, s);
+            launchYfft<T>(H, G, nsrc, +1, s, "yfft_inv", true, false, &yp);
+        } else {
+            launchSynth<T>(fluxPlan(), s);
+            launchYfft<T>(H, G, NH, +1, s, "yfft_inv", true, false);
+        }
         launchXfused<T>(s);
         launchYfft<T>(P, Q, NT, -1, s, "yfft_fwd", false, true);
         launchProj<T>(s);
