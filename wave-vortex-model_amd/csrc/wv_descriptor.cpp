@@ -106,7 +106,10 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile, const Stratif
 
     // Vertical structure -- WVKernelTypes.cpp:169-212.
     f = 2.0 * c.rotationRate * std::sin(c.latitude * kPi / 180.0);
-    const double N02 = c.N0 * c.N0, f2 = f * f;
+    N02 = c.N0 * c.N0;
+    f2 = f * f;
+    waveNorm = std::sqrt(2.0 * kModalGravity / (c.Lz * (N02 - f2)));
+    projPrefactor = std::sqrt(c.g * c.Lz / (2.0 * (N02 - f2)));
     if (!c.isHydrostatic && N02 <= f2)
         return "Nonhydrostatic constant stratification requires N0 squared greater than f squared.";
     if (stratified) {

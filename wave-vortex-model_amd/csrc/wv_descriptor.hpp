@@ -53,6 +53,10 @@ struct Descriptor {
     size_t nkxP = 0;  // nkx padded to the y-FFT tile width
     std::vector<double> kxv;  // x wavenumber of half-spectrum column kx < nkx
     double f = 0;
+    // N0^2, f^2 and the square roots the per-coefficient tables share:
+    // waveNorm = sqrt(2 g_modal / (Lz (N0^2 - f^2))), projPrefactor =
+    // sqrt(g Lz / (2 (N0^2 - f^2)))
+    double N02 = 0, f2 = 0, waveNorm = 0, projPrefactor = 0;
 
     std::vector<FourierMode> modes;
     std::vector<Column> columns;      // sorted by (iy, kx)

@@ -154,11 +154,10 @@ struct FieldTables {
 // Evolved field spectra (u, v, w, eta) of coefficient idx
 // (WVCoefficientFormulas.hpp:15-33), stored to FV[f*M + idx].
 template <typename T>
-__device__ __forceinline__ void storeFieldSpectra(const FieldTables& ft, int idx, int M, cx<double> Ap,
-                                                  cx<double> Am, cx<double> A0, double omega, double elapsed,
-                                                  cx<T>* __restrict__ FV) {
-    const cx<double> UAp = ft.UAp[idx], VAp = ft.VAp[idx], WAp = ft.WAp[idx], UA0 = ft.UA0[idx], VA0 = ft.VA0[idx];
-    const double NAp = ft.NAp[idx], NA0 = ft.NA0[idx];
+__device__ __forceinline__ void storeFieldSpectra(cx<double> UAp, cx<double> VAp, cx<double> WAp, double NAp,
+                                                  cx<double> UA0, cx<double> VA0, double NA0, int idx, int M,
+                                                  cx<double> Ap, cx<double> Am, cx<double> A0, double omega,
+                                                  double elapsed, cx<T>* __restrict__ FV) {
     double sn, cs;
     sincos(omega * elapsed, &sn, &cs);
     const cx<double> p{cs, sn};
@@ -179,7 +178,8 @@ __global__ __launch_bounds__(256) void k_fields(const cx<double>* __restrict__ A
                                                 int M, cx<T>* __restrict__ FV) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= M) return;
-    storeFieldSpectra(ft, idx, M, Ap[idx], Am[idx], A0[idx], ft.omega[idx], elapsed, FV);
+    storeFieldSpectra(ft.UAp[idx], ft.VAp[idx], ft.WAp[idx], ft.NAp[idx], ft.UA0[idx], ft.VA0[idx], ft.NA0[idx], idx,
+                      M, Ap[idx], Am[idx], A0[idx], ft.omega[idx], elapsed, FV);
 }
 
 // F- or G-family field spectrum Apm*waveScale + A0*geostrophicScale per
@@ -518,8 +518,8 @@ struct YDerivPlan {
 
 template <typename T, int L2N, int L2KT, bool DERIV>
 __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
-                                                      int NyArg, int log2NyArg, int nkxP, int log2KTArg, int Nz,
-                                                      const cx<T>* __restrict__ twY, int sign,
+                                                      int NyArg, int log2NyArg, int nkx, int nkxP, int log2KTArg,
+                                                      int Nz, const cx<T>* __restrict__ twY, int sign,
                                                       const int* __restrict__ rowLimit, int skipIn, int skipOut,
                                                       YDerivPlan yp, const T* __restrict__ lrow) {
     const int Ny = L2N > 0 ? 1 << L2N : NyArg;
@@ -569,9 +569,10 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
             fftLds<T, +1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
         else
             fftLds<T, -1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
+        // columns kx >= nkx (the tile's padding) are not written
         for (int e = fftThread(); e < nel; e += kFftThreads) {
             const int kk = e & (KT - 1), iy = e >> log2KT;
-            if (skipOut && iy > lim && iy < Ny - lim) continue;
+            if ((skipOut && iy > lim && iy < Ny - lim) || kx0 + kk >= nkx) continue;
             out[outPlane + static_cast<size_t>(iy) * nkxP + kx0 + kk] = tile[e];
         }
     };
@@ -1011,6 +1012,84 @@ struct ModeDev {
     double k, l, Kh, cosA, sinA;
 };
 
+// Inputs of the per-coefficient tables: constants and per-j arrays.
+struct CoefConsts {
+    double f, f2, N02, g, Lz;
+    double S;   // sqrt(2 g_modal / (Lz (N0^2 - f^2))): nonhydrostatic wave normalization
+    double P0;  // sqrt(g Lz / (2 (N0^2 - f^2))): wave-projection prefactor (constant N)
+    int hydro, stratified;
+    const double* m;    // m_j
+    const double* h0;   // equivalent depths
+    const double* Fg;   // F, G normalizations
+    const double* Gg;
+    const double* gws;  // G wave scale
+    const double* apmW;
+    const double* inertialScale;
+};
+
+// The per-coefficient tables of coefficient (j, mode) -- the descriptor's
+// formulas (WVKernelTypes.cpp:214-284) -- evaluated with the same operations
+// in the same order, each rounded on its own (no contraction), so the values
+// equal the host tables bit for bit.
+struct CoefRow {
+    double om, NAp, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled;
+    cx<double> UAp, VAp, WAp, UA0, VA0, ApmD;
+};
+__device__ __forceinline__ CoefRow coefficients(const CoefConsts& c, int j, const ModeDev& md) {
+    auto mul = [](double a, double b) { return __dmul_rn(a, b); };
+    auto dv = [](double a, double b) { return __ddiv_rn(a, b); };
+    auto add = [](double a, double b) { return __dadd_rn(a, b); };
+    CoefRow r{};
+    const double Kh = md.Kh, Kh2 = mul(Kh, Kh);
+    const double Mj = c.m[j], h0 = c.h0[j], Fg = c.Fg[j], Gg = c.Gg[j];
+    const double sign = j % 2 == 0 ? 1.0 : -1.0;
+    const bool wave = Kh > 0.0 && j > 0, inertial = Kh == 0.0, geostrophic = Kh > 0.0, mda = Kh == 0.0 && j > 0;
+    const double N02mf2 = __dsub_rn(c.N02, c.f2);
+    double hpm = 1.0;
+    if (j > 0)
+        hpm = c.stratified ? h0 : c.hydro ? dv(c.N02, mul(mul(c.g, Mj), Mj)) : dv(N02mf2, mul(c.g, add(mul(Mj, Mj), Kh2)));
+    double Fw = Fg, Gw = Gg;
+    if (!c.hydro) {
+        Fw = j == 0 ? 2.0 : mul(mul(mul(sign, hpm), Mj), c.S);
+        Gw = j == 0 ? 1.0 : mul(sign, c.S);
+    }
+    const double om = __dsqrt_rn(add(mul(mul(c.g, hpm), Kh2), c.f2));
+    const double Fwg = dv(Fg, Fw), Gwg = dv(Gg, Gw);
+    const double fs = dv(Fg, Fwg);
+    r.om = om;
+    const double prefactor = c.stratified ? 0.0 : mul(sign, c.P0);
+    r.ApmDScaled = mul(dv(Mj, 2.0), prefactor);
+    if (wave) {
+        const double gs = c.gws[j];
+        const double fo = dv(c.f, om);
+        r.UAp = {mul(md.cosA, fs), mul(mul(-fo, md.sinA), fs)};
+        r.VAp = {mul(md.sinA, fs), mul(mul(fo, md.cosA), fs)};
+        r.WAp = {mul(0.0, gs), mul(mul(-Kh, hpm), gs)};
+        r.NAp = mul(dv(mul(-Kh, hpm), om), gs);
+        const double deltaScale = dv(mul(h0, Gwg), Fg);
+        const double twoKh = mul(mul(2.0, Kh), hpm);
+        r.ApmD = {mul(0.0, deltaScale), mul(dv(-1.0, twoKh), deltaScale)};
+        r.ApmN = dv(mul(dv(-om, twoKh), Gwg), Gg);
+    } else if (inertial) {
+        r.UAp = {fs, 0.0};
+        r.VAp = {0.0, fs};
+    }
+    if (geostrophic) {
+        const double lr2inv = j == 0 ? 0.0 : dv(c.f2, mul(c.g, h0));
+        const double den = add(Kh2, lr2inv);
+        r.UA0 = {mul(0.0, Fg), mul(dv(md.l, den), Fg)};
+        r.VA0 = {mul(0.0, Fg), mul(dv(-md.k, den), Fg)};
+        r.NA0 = mul(j == 0 ? 0.0 : dv(-dv(c.f, c.g), den), Gg);
+        r.A0Vort = dv(1.0, Fg);
+        r.A0Buoy = j == 0 ? 0.0 : dv(dv(-c.f, h0), Gg);
+    } else if (mda) {
+        r.NA0 = Gg;
+        r.A0Buoy = dv(1.0, Gg);
+        r.A0Vort = dv(dv(c.f2, mul(2.0, h0)), Fg);
+    }
+    return r;
+}
+
 // Wave-vortex projection of the analysed products (projectFluxTargetMode,
 // ...Kernel.cpp) and the flux store or RK4 stage of WVFixedStepRK4::step,
 // one coefficient (j, mode) per thread; in RK modes with emitFields set it
@@ -1018,20 +1097,19 @@ struct ModeDev {
 // next flux evaluation).
 template <typename T, int NT>
 __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, const ModeDev* __restrict__ modes,
-                                               int Nj, int M, ProjTables tb, FieldTables ft, double elapsed,
-                                               double hscale, Epilogue ep, cx<T>* __restrict__ FV) {
+                                               int Nj, int M, CoefConsts cc, double elapsed, double hscale,
+                                               Epilogue ep, cx<T>* __restrict__ FV) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= M) return;
     const int wv = idx / Nj, j = idx - wv * Nj;
     constexpr bool hydroT = NT == 3;  // hydrostatic: ApmD, else ApmDScaled
     const ModeDev md = modes[wv];
     const bool inertial = wv == 0;
-    const double om = tb.omega[idx], a0v = tb.A0Vort[idx], a0b = tb.A0Buoy[idx], na0 = tb.NA0[idx],
-                 apmn = tb.ApmN[idx];
-    cx<double> apmd{0.0, 0.0};
-    double dsc = 0.0;
-    if constexpr (hydroT) apmd = tb.ApmD[idx];
-    else dsc = tb.ApmDScaled[idx];
+    // the coefficient tables, formed here instead of read from memory
+    const CoefRow cr = coefficients(cc, j, md);
+    const double om = cr.om, a0v = cr.A0Vort, a0b = cr.A0Buoy, na0 = cr.NA0, apmn = cr.ApmN;
+    const cx<double> apmd = cr.ApmD;
+    const double dsc = cr.ApmDScaled;
     cx<T> tv[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) tv[t] = Tg[t * static_cast<size_t>(M) + idx];
@@ -1056,13 +1134,13 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
         } else {
             if (target == 0) wave = cscale(value, md.cosA * dsc);
             if (target == 1) wave = cscale(value, md.sinA * dsc);
-            if (target == 2) wave = cmuli(value, (md.Kh / 2.0) * tb.apmW[j]);
+            if (target == 2) wave = cmuli(value, (md.Kh / 2.0) * cc.apmW[j]);
         }
         cx<double> cAp = cadd(wave, buoy), cAm = csub(wave, buoy);
         if (inertial) {
             cx<double> a{0.0, 0.0};
-            if (target == 0) a = cscale(value, tb.inertialScale[j]);
-            if (target == 1) a = cscale(cmuli(value, -1.0), tb.inertialScale[j]);
+            if (target == 0) a = cscale(value, cc.inertialScale[j]);
+            if (target == 1) a = cscale(cmuli(value, -1.0), cc.inertialScale[j]);
             cAp = a;
             cAm = cconj(a);
         }
@@ -1107,7 +1185,8 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
         }
     }
     if (ep.mode != kFluxOut && ep.emitFields)
-        storeFieldSpectra(ft, idx, M, next[0], next[1], next[2], om, ep.nextElapsed, FV);
+        storeFieldSpectra(cr.UAp, cr.VAp, cr.WAp, cr.NAp, cr.UA0, cr.VA0, cr.NA0, idx, M, next[0], next[1], next[2], om,
+                          ep.nextElapsed, FV);
 }
 
 // r2c along x of nf real FP64 fields [Nx,Ny,Nz,nf] into P[c][z][y][kx < nkx]
@@ -1347,7 +1426,8 @@ struct KernelImpl {
     DeviceBuffer synCE, synSE, synCO, synSO, anaCE, anaCO, anaSE, anaSO;
     bool fold = false;
     size_t Zh = 0;  // folded z rows (multiple of 16)
-    DeviceBuffer FV, H, G, P, Q, Tg, modeTab, stage, weighted, partial, scalar;
+    DeviceBuffer FV, H, G, P, Q, Tg, modeTab, jTab, stage, weighted, partial, scalar;
+    CoefConsts coefConsts{};
     DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
     DeviceBuffer lrow;      // per storage row iy: its y wavenumber (rowSign * l of its columns)
     bool lrowValid = false;
@@ -1367,7 +1447,7 @@ struct KernelImpl {
         size_t s = 0;
         for (const DeviceBuffer* b : {&rowLimit, &lrow, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &stage, &weighted, &partial, &scalar})
+                                      &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &jTab, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
         return s;
     }
@@ -1593,6 +1673,24 @@ struct KernelImpl {
                 hm[i] = {fm.k, fm.l, fm.Kh, fm.cosAlpha, fm.sinAlpha};
             }
             upload(modeTab, hm);
+            // per-j inputs of the coefficient tables: m, h0, Fg, Gg, G wave
+            // scale, apmW, inertialScale
+            const size_t Nj = d.c.Nj;
+            std::vector<double> jt(7 * Nj);
+            for (size_t j = 0; j < Nj; ++j) {
+                jt[j] = d.m[j];
+                jt[Nj + j] = d.h0[j];
+                jt[2 * Nj + j] = d.Fg[j];
+                jt[3 * Nj + j] = d.Gg[j];
+                jt[4 * Nj + j] = d.gWaveScale[j];
+                jt[5 * Nj + j] = d.apmW[j];
+                jt[6 * Nj + j] = d.inertialScale[j];
+            }
+            upload(jTab, jt);
+            const double* jp = jTab.as<double>();
+            coefConsts = CoefConsts{d.f, d.f2, d.N02, cfg.g, cfg.Lz, d.waveNorm, d.projPrefactor,
+                                    cfg.isHydrostatic ? 1 : 0, d.stratified ? 1 : 0,
+                                    jp, jp + Nj, jp + 2 * Nj, jp + 3 * Nj, jp + 4 * Nj, jp + 5 * Nj, jp + 6 * Nj};
         }
         hipCheck(hipMemset(H.p, 0, H.bytes), "hipMemset");
         hipCheck(hipMemset(P.p, 0, P.bytes), "hipMemset");
@@ -1736,8 +1834,8 @@ struct KernelImpl {
         const double bytes = frac * (nch + nout) * static_cast<double>(d.c.Nz) * d.c.Ny * d.nkxP * sizeof(cx<T>);
         // Ny = 64 ... 1024 with their default tile widths run kernels
         // compiled for that length; others the runtime-length kernel.
-        using Fn = void (*)(const cx<T>*, cx<T>*, int, int, int, int, int, const cx<T>*, int, const int*, int, int,
-                            YDerivPlan, const T*);
+        using Fn = void (*)(const cx<T>*, cx<T>*, int, int, int, int, int, int, const cx<T>*, int, const int*, int,
+                            int, YDerivPlan, const T*);
         const int l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
         const bool dv = yp != nullptr;
         Fn fn = dv ? k_yfft<T, 0, 0, true> : k_yfft<T, 0, 0, false>;
@@ -1750,8 +1848,9 @@ struct KernelImpl {
         timed(name, bytes, s, [&] {
             hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(d.nkxP / KT), static_cast<unsigned>(d.c.Nz), nch),
                                dim3(kFftThreads), lds, s, in.as<cx<T>>(), out.as<cx<T>>(), static_cast<int>(d.c.Ny),
-                               l2y, static_cast<int>(d.nkxP), l2t, static_cast<int>(d.c.Nz), twY.as<cx<T>>(), sign,
-                               rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0, plan, lrow.as<T>());
+                               l2y, static_cast<int>(d.nkx), static_cast<int>(d.nkxP), l2t, static_cast<int>(d.c.Nz),
+                               twY.as<cx<T>>(), sign, rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0, plan,
+                               lrow.as<T>());
         });
     }
 
@@ -1843,8 +1942,8 @@ struct KernelImpl {
         // (144 B); with the next input's spectra the field tables (96 B) and
         // four spectra
         const double epBytes = ep.mode == kFluxOut ? 48.0 : (ep.mode == kRk2 || ep.mode == kRk3) ? 192.0 : 144.0;
-        const double fieldBytes = (ep.mode != kFluxOut && ep.emitFields) ? 96.0 + 4.0 * sizeof(cx<T>) : 0.0;
-        const double bytes = static_cast<double>(M) * (NT * sizeof(cx<T>) + 48.0 + epBytes + fieldBytes);
+        const double fieldBytes = (ep.mode != kFluxOut && ep.emitFields) ? 4.0 * sizeof(cx<T>) : 0.0;
+        const double bytes = static_cast<double>(M) * (NT * sizeof(cx<T>) + epBytes + fieldBytes);
         // the kernel reads y and w in every mode
         Epilogue e = ep;
         if (e.mode == kFluxOut) {
@@ -1855,8 +1954,7 @@ struct KernelImpl {
         timed("stage", bytes, s, [&] {
             auto fn = NT == 4 ? k_stage<T, 4> : k_stage<T, 3>;
             hipLaunchKernelGGL(fn, dim3((M + 255) / 256), dim3(256), 0, s, Tg.as<cx<T>>(), modeTab.as<ModeDev>(),
-                               static_cast<int>(d.c.Nj), M, projTables(), fieldTables(), elapsed, hscale, e,
-                               FV.as<cx<T>>());
+                               static_cast<int>(d.c.Nj), M, coefConsts, elapsed, hscale, e, FV.as<cx<T>>());
         });
     }
 
