@@ -262,29 +262,38 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
         else scol[threadIdx.x] = ColumnDev{0, 0, -1, 0, 0.0, 0.0, 0.0, 0.0, 0.0};
     }
     __syncthreads();
-    // stage spectra: loads contiguous in j for each column, issued kB at a
-    // time from clamped addresses (unconditional loads)
+    // stage spectra: wave w copies rows (field, column) r = w, w + nwaves, ...
+    // (kR rows at a time, all loads issued before the stores), its lanes
+    // walking j; rows of absent columns and entries j >= Nj are zeros
     {
-        constexpr int kB = 8;
-        const int tot = 4 * Js * 8;
-        for (int e0 = 0; e0 < tot; e0 += kB * static_cast<int>(blockDim.x)) {
-            cx<T> v[kB];
-            bool ok[kB];
+        constexpr int kR = 4;    // rows per batch
+        constexpr int kH = 4;    // 64-entry chunks of a row (Js <= 256)
+        const int nh = (Js + 63) >> 6;
+        for (int r0 = waveInBlock; r0 < 32; r0 += kR * nwaves) {
+            cx<T> v[kR][kH];
 #pragma unroll
-            for (int i = 0; i < kB; ++i) {
-                const int e = e0 + threadIdx.x + i * blockDim.x;
-                const int ec = e < tot ? e : tot - 1;
-                const int j = ec % Js, fc = ec / Js;
-                const int c = fc % 8, f = fc / 8;
-                const int wv = scol[c].wv;
-                v[i] = FV[static_cast<size_t>(f) * M + (j < Nj ? j : Nj - 1) + static_cast<size_t>(Nj) * (wv >= 0 ? wv : 0)];
-                ok[i] = wv >= 0 && j < Nj;
+            for (int i = 0; i < kR; ++i) {
+                const int r = r0 + i * nwaves;
+                const int rc = r < 32 ? r : 31;
+                const int wv = scol[rc & 7].wv;
+                const cx<T>* src = FV + static_cast<size_t>(rc >> 3) * M + static_cast<size_t>(Nj) * (wv >= 0 ? wv : 0);
+#pragma unroll
+                for (int h = 0; h < kH; ++h) {
+                    const int j = lane + 64 * h;
+                    if (h < nh) v[i][h] = src[j < Nj ? j : Nj - 1];
+                }
             }
             __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
 #pragma unroll
-            for (int i = 0; i < kB; ++i) {
-                const int e = e0 + threadIdx.x + i * blockDim.x;
-                if (e < tot) fv[(e / Js) * Jp + e % Js] = ok[i] ? v[i] : cx<T>{T(0), T(0)};
+            for (int i = 0; i < kR; ++i) {
+                const int r = r0 + i * nwaves;
+                if (r >= 32) continue;
+                const bool live = scol[r & 7].wv >= 0;
+#pragma unroll
+                for (int h = 0; h < kH; ++h) {
+                    const int j = lane + 64 * h;
+                    if (h < nh && j < Js) fv[r * Jp + j] = (live && j < Nj) ? v[i][h] : cx<T>{T(0), T(0)};
+                }
             }
         }
     }
@@ -876,49 +885,39 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     const int nsteps = Za / 4;
     const int L = Nz - 1;
     __syncthreads();  // scol ready
-    // stage the columns of every target; a thread issues all loads of kB
-    // entries (two each when folded) before storing them
+    // stage the columns of every target: lane (zr, c) of wave w takes column
+    // c of levels z = 8 g + zr, g = w, w + nwaves, ...; all loads of a batch
+    // (every target; two levels each when folded) precede its stores
     {
-        constexpr int kB = 4;
-        const int tot = NT * Za * 8;
-        for (int e0 = 0; e0 < tot; e0 += kB * blockDim.x) {
-            cx<T> v[kB], w[kB];
-            bool ok[kB], pair[kB];
+        const int nwaves = blockDim.x >> 6;
+        const int c = lane & 7, zr = lane >> 3;
+        const ColumnDev& cc = scol[c];
+        const size_t plane = static_cast<size_t>(Ny) * nkxP;
+        const size_t colBase = static_cast<size_t>(cc.iy) * nkxP + cc.kx;
+        const bool colLive = cc.wv >= 0;
+        for (int g = jt; 8 * g < Za; g += nwaves) {
+            const int z = 8 * g + zr;
+            const int zc = z < Nz ? z : Nz - 1;  // clamped: loads are unconditional
+            const int zm = L - zc >= 0 ? L - zc : 0;
+            cx<T> v[NT], w[NT];
 #pragma unroll
-            for (int i = 0; i < kB; ++i) {
-                const int e = e0 + threadIdx.x + i * blockDim.x;
-                const int ec = e < tot ? e : tot - 1;  // clamped: loads are unconditional
-                const int c = ec & 7, zt = ec >> 3;
-                const int t = zt / Za, z = zt - t * Za;
-                const ColumnDev& cc = scol[c];
-                const size_t colBase = t * tStride + static_cast<size_t>(cc.iy) * nkxP + cc.kx;
-                const size_t plane = static_cast<size_t>(Ny) * nkxP;
-                const int zc = z < Nz ? z : Nz - 1;
-                v[i] = Q[colBase + zc * plane];
-                if constexpr (FOLD) {
-                    const int zm = L - zc >= 0 ? L - zc : 0;
-                    w[i] = Q[colBase + zm * plane];
-                    ok[i] = e < tot && cc.wv >= 0 && 2 * z <= L;
-                    pair[i] = z < L - z;
-                } else {
-                    ok[i] = e < tot && cc.wv >= 0 && z < Nz;
-                }
+            for (int t = 0; t < NT; ++t) {
+                v[t] = Q[t * tStride + zc * plane + colBase];
+                if constexpr (FOLD) w[t] = Q[t * tStride + zm * plane + colBase];
             }
             __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
+            if (z >= Za) continue;
+            const cx<T> zero{T(0), T(0)};
 #pragma unroll
-            for (int i = 0; i < kB; ++i) {
-                const int e = e0 + threadIdx.x + i * blockDim.x;
-                if (e >= tot) continue;
-                const cx<T> zero{T(0), T(0)};
+            for (int t = 0; t < NT; ++t) {
                 if constexpr (FOLD) {
-                    const int zt = e >> 3, t = zt / Za;
-                    const int r = e - t * Za * 8;  // (z, c) within the target
-                    const cx<T> sv = !ok[i] ? zero : pair[i] ? cadd(v[i], w[i]) : v[i];
-                    const cx<T> dv = !ok[i] || !pair[i] ? zero : csub(v[i], w[i]);
-                    qs[(2 * t) * Za * 8 + r] = sv;
-                    qs[(2 * t + 1) * Za * 8 + r] = dv;
+                    const bool ok = colLive && 2 * z <= L, pair = z < L - z;
+                    const cx<T> sv = !ok ? zero : pair ? cadd(v[t], w[t]) : v[t];
+                    const cx<T> dv = !ok || !pair ? zero : csub(v[t], w[t]);
+                    qs[((2 * t) * Za + z) * 8 + c] = sv;
+                    qs[((2 * t + 1) * Za + z) * 8 + c] = dv;
                 } else {
-                    qs[e] = ok[i] ? v[i] : zero;
+                    qs[(t * Za + z) * 8 + c] = colLive && z < Nz ? v[t] : zero;
                 }
             }
         }
