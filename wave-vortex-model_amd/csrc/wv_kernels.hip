@@ -234,7 +234,7 @@ __device__ __forceinline__ double pairSwap(double v) {
 // E - O (DCT-I) or O - E (DST-I) at L - z, for z <= L/2 only: half the MFMA
 // work.  mats then holds the even-j parts of the two matrices (rows z <=
 // L/2, Zs of them) in m[0], m[1] and the odd-j parts in m[2], m[3]; a task
-// is one z tile of one group (ZT is 1).
+// is one z tile of NQ channel slots of one group (ZT is 1).
 template <typename T, int ZT, int NQ, int KS, bool FOLD>
 __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
                                                int ncol, int M, SynthMats<T> mats, int Zs, int Js, int Nz,
@@ -317,19 +317,25 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
     const T* fvw = reinterpret_cast<const T*>(fv) + 2 * c * Jp + part;
     if constexpr (FOLD) {
         const int L = Nz - 1, half = L / 2;
-        for (int task = waveInBlock; task < plan.ngroups * ztiles; task += nwaves) {
+        // channel slots of a group in parts of NQ
+        int maxch = 0;
+        for (int g = 0; g < plan.ngroups; ++g) maxch = max(maxch, plan.g[g].nch);
+        const int parts = (maxch + NQ - 1) / NQ;
+        for (int task = waveInBlock; task < plan.ngroups * parts * ztiles; task += nwaves) {
             const int gi = task % plan.ngroups;
-            const int zt = task / plan.ngroups;
+            const int pt = (task / plan.ngroups) % parts;
+            const int zt = task / (plan.ngroups * parts);
             const int mat = plan.g[gi].mat, nch = plan.g[gi].nch;
+            const int q0 = pt * NQ;  // first channel of this part
             int fieldOff[NQ];
             T coefV[NQ], coefZ[NQ];
             const T zsign = mat == 1 ? T(-0.5) : T(0.5);
             const T vscale = T(plan.valueScale);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                const bool live = q < nch;
-                const bool dz = live && plan.g[gi].ch[q].dz;
-                fieldOff[q] = live ? plan.g[gi].ch[q].field * 16 * Jp : 0;
+                const bool live = q0 + q < nch;
+                const bool dz = live && plan.g[gi].ch[q0 + q].dz;
+                fieldOff[q] = live ? plan.g[gi].ch[q0 + q].field * 16 * Jp : 0;
                 coefV[q] = live && !dz ? vscale * partWeight : T(0);
                 coefZ[q] = dz ? zsign * partWeight : T(0);
             }
@@ -381,8 +387,9 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
             run(e1, o1, s0 + kS);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                if (q >= nch) continue;
-                const int oV = plan.g[gi].ch[q].outV, oX = plan.g[gi].ch[q].outX, oY = plan.g[gi].ch[q].outY;
+                if (q0 + q >= nch) continue;
+                const SynthChannel& sc = plan.g[gi].ch[q0 + q];
+                const int oV = sc.outV, oX = sc.outX, oY = sc.outY;
                 T* hV = reinterpret_cast<T*>(H) + (oV >= 0 ? oV : 0) * chStride + colOff;
                 T* hX = reinterpret_cast<T*>(H) + (oX >= 0 ? oX : 0) * chStride + colOff;
                 T* hY = reinterpret_cast<T*>(H) + (oY >= 0 ? oY : 0) * chStride + colOff;
@@ -1793,7 +1800,10 @@ struct KernelImpl {
         const bool folded = fold && mats == nullptr;
         const int ztiles = static_cast<int>((folded ? Zh : d.Zs) / 16);
         // waves per workgroup: the largest of 8 ... 1 that divides the tasks
-        const int tasks = folded ? sp.ngroups * ztiles : sp.ngroups * ((ztiles + ZT - 1) / ZT);
+        // folded tasks take four channel slots (FP32) or two (FP64: half the
+        // accumulator registers, more resident waves)
+        constexpr int FQ = sizeof(T) == 8 ? 2 : 4;
+        const int tasks = folded ? sp.ngroups * ((maxch + FQ - 1) / FQ) * ztiles : sp.ngroups * ((ztiles + ZT - 1) / ZT);
         int waves = folded ? 8 : 4;
         while (tasks % waves) --waves;
         const int nsteps = static_cast<int>(folded ? d.Js / 8 : d.Js / 4);
@@ -1802,8 +1812,7 @@ struct KernelImpl {
                             const T*, SynthPlan, cx<T>*);
         Fn fn;
         if (folded)
-            fn = maxch <= 2 ? (ks4 ? k_synth<T, 1, 2, 4, true> : k_synth<T, 1, 2, 2, true>)
-                            : (ks4 ? k_synth<T, 1, 4, 4, true> : k_synth<T, 1, 4, 2, true>);
+            fn = ks4 ? k_synth<T, 1, FQ, 4, true> : k_synth<T, 1, FQ, 2, true>;
         else
             fn = maxch <= 2 ? (ks4 ? k_synth<T, ZT, 2, 4, false> : k_synth<T, ZT, 2, 2, false>)
                             : (ks4 ? k_synth<T, ZT, 4, 4, false> : k_synth<T, ZT, 4, 2, false>);
