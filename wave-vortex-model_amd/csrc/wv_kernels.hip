@@ -261,14 +261,23 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
         if (col < ncol) scol[threadIdx.x] = cols[col];
         else scol[threadIdx.x] = ColumnDev{0, 0, -1, 0, 0.0, 0.0, 0.0, 0.0, 0.0};
     }
+    // m_j for the z-derivative channels (Js <= 4 blockDim.x)
+    T mv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = threadIdx.x + k * blockDim.x;
+        mv[k] = mj[j < Nj ? j : Nj - 1];
+    }
     __syncthreads();
     // stage spectra: wave w copies rows (field, column) r = w, w + nwaves, ...
     // (kR rows at a time, all loads issued before the stores), its lanes
-    // walking j; rows of absent columns and entries j >= Nj are zeros
+    // walking j; rows of absent columns and entries j >= Nj are zeros.  The
+    // loads are unconditional (entries j >= Nj read entry Nj - 1, one line
+    // for the whole wave): a load under a branch would make every load wait
+    // for the ones before it.
     {
         constexpr int kR = 4;    // rows per batch
         constexpr int kH = 4;    // 64-entry chunks of a row (Js <= 256)
-        const int nh = (Js + 63) >> 6;
         for (int r0 = waveInBlock; r0 < 32; r0 += kR * nwaves) {
             cx<T> v[kR][kH];
 #pragma unroll
@@ -280,7 +289,7 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
 #pragma unroll
                 for (int h = 0; h < kH; ++h) {
                     const int j = lane + 64 * h;
-                    if (h < nh) v[i][h] = src[j < Nj ? j : Nj - 1];
+                    v[i][h] = src[j < Nj ? j : Nj - 1];
                 }
             }
             __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
@@ -292,12 +301,16 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
 #pragma unroll
                 for (int h = 0; h < kH; ++h) {
                     const int j = lane + 64 * h;
-                    if (h < nh && j < Js) fv[r * Jp + j] = (live && j < Nj) ? v[i][h] : cx<T>{T(0), T(0)};
+                    if (j < Js) fv[r * Jp + j] = (live && j < Nj) ? v[i][h] : cx<T>{T(0), T(0)};
                 }
             }
         }
     }
-    for (int j = threadIdx.x; j < Js; j += blockDim.x) mjs[j] = j < Nj ? mj[j] : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = threadIdx.x + k * blockDim.x;
+        if (j < Js) mjs[j] = j < Nj ? mv[k] : T(0);
+    }
     __syncthreads();
     const int n = lane & 15, kq = lane >> 4;
     const int c = n >> 1, part = n & 1;
@@ -321,7 +334,30 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
         int maxch = 0;
         for (int g = 0; g < plan.ngroups; ++g) maxch = max(maxch, plan.g[g].nch);
         const int parts = (maxch + NQ - 1) / NQ;
-        for (int task = waveInBlock; task < plan.ngroups * parts * ztiles; task += nwaves) {
+        const int ntasks = plan.ngroups * parts * ztiles;
+        // operand rows of task tk's even-j and odd-j matrices (lane offset included)
+        auto matRows = [&](int tk, const T*& pE, const T*& pO) {
+            const int zt = tk / (plan.ngroups * parts);
+            const int mat = plan.g[tk % plan.ngroups].mat;
+            pE = mats.m[mat] + static_cast<size_t>(zt) * nsteps * 64 + lane;
+            pO = mats.m[mat + 2] + static_cast<size_t>(zt) * nsteps * 64 + lane;
+        };
+        constexpr int kS = KS;
+        // A operands in register batches of kS steps, two named batches used
+        // in turn; the first batch of a wave's next task is loaded before the
+        // current task's stores, so that its wait does not include them (one
+        // counter covers a wave's loads and stores in issue order).
+        T e0[kS], o0[kS], e1[kS], o1[kS];
+        {
+            const T *pE, *pO;
+            matRows(waveInBlock < ntasks ? waveInBlock : ntasks - 1, pE, pO);
+#pragma unroll
+            for (int u = 0; u < kS; ++u) {
+                e0[u] = pE[u * 64];
+                o0[u] = pO[u * 64];
+            }
+        }
+        for (int task = waveInBlock; task < ntasks; task += nwaves) {
             const int gi = task % plan.ngroups;
             const int pt = (task / plan.ngroups) % parts;
             const int zt = task / (plan.ngroups * parts);
@@ -339,12 +375,11 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
                 coefV[q] = live && !dz ? vscale * partWeight : T(0);
                 coefZ[q] = dz ? zsign * partWeight : T(0);
             }
-            const T* AE = mats.m[mat] + static_cast<size_t>(zt) * nsteps * 64 + lane;
-            const T* AO = mats.m[mat + 2] + static_cast<size_t>(zt) * nsteps * 64 + lane;
+            const T *AE, *AO;
+            matRows(task, AE, AO);
             typename MF::acc accE[NQ], accO[NQ];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) accE[q] = accO[q] = MF::zero();
-            constexpr int kS = KS;
             auto load = [&](T (&dE)[kS], T (&dO)[kS], int s0) {
 #pragma unroll
                 for (int u = 0; u < kS; ++u) {
@@ -373,17 +408,33 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
                         accO[q] = MF::run(aO[u], bO[u][q], accO[q]);
                     }
             };
-            T e0[kS], o0[kS], e1[kS], o1[kS];
-            load(e0, o0, 0);
+            // (the scheduling barriers keep each batch's loads a whole batch
+            // of MFMA work ahead of their use; left alone, the compiler moves
+            // them down to just before it)
             int s0 = 0;
             for (; s0 + 2 * kS < nsteps; s0 += 2 * kS) {
                 load(e1, o1, s0 + kS);
+                __builtin_amdgcn_sched_barrier(0);
                 run(e0, o0, s0);
                 load(e0, o0, s0 + 2 * kS);
+                __builtin_amdgcn_sched_barrier(0);
                 run(e1, o1, s0 + kS);
             }
             load(e1, o1, s0 + kS);
+            __builtin_amdgcn_sched_barrier(0);
             run(e0, o0, s0);
+            {
+                // the next task's first batch (the last task reloads its own)
+                const int next = task + nwaves < ntasks ? task + nwaves : task;
+                const T *pE, *pO;
+                matRows(next, pE, pO);
+#pragma unroll
+                for (int u = 0; u < kS; ++u) {
+                    e0[u] = pE[u * 64];
+                    o0[u] = pO[u * 64];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
             run(e1, o1, s0 + kS);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
@@ -1799,13 +1850,16 @@ struct KernelImpl {
         // the folded DCT-I / DST-I path serves the constant-N matrices
         const bool folded = fold && mats == nullptr;
         const int ztiles = static_cast<int>((folded ? Zh : d.Zs) / 16);
-        // waves per workgroup: the largest of 8 ... 1 that divides the tasks
-        // folded tasks take four channel slots (FP32) or two (FP64: half the
-        // accumulator registers, more resident waves)
-        constexpr int FQ = sizeof(T) == 8 ? 2 : 4;
+        // Folded tasks take two channel slots (64 VGPRs in FP32: 8 waves per
+        // SIMD, so 6 workgroups per CU, as many as the LDS holds) and run on
+        // four waves, one per SIMD, which deal the tasks round-robin.  (A
+        // five-wave workgroup puts two of its waves on one SIMD, which then
+        // has twice the work of the others.)  Unfolded: the largest of 4 ... 1
+        // waves that divides the tasks.
+        constexpr int FQ = 2;
         const int tasks = folded ? sp.ngroups * ((maxch + FQ - 1) / FQ) * ztiles : sp.ngroups * ((ztiles + ZT - 1) / ZT);
-        int waves = folded ? 8 : 4;
-        while (tasks % waves) --waves;
+        int waves = folded ? std::min(4, tasks) : 4;
+        while (!folded && tasks % waves) --waves;
         const int nsteps = static_cast<int>(folded ? d.Js / 8 : d.Js / 4);
         const bool ks4 = nsteps % 8 == 0;
         using Fn = void (*)(const cx<T>*, const ColumnDev*, int, int, SynthMats<T>, int, int, int, int, int, int,

@@ -282,6 +282,8 @@ class WVTransformConstantStratificationKernel:
     def _stream(stream):
         if stream is None:
             return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+        if hasattr(stream, "cuda_stream"):  # a torch.cuda.Stream
+            return C.c_void_p(stream.cuda_stream)
         return C.c_void_p(stream)
 
     # ---------------------------------------------------------------- flux --
