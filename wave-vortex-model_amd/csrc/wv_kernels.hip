@@ -441,28 +441,43 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
                 if (q0 + q >= nch) continue;
                 const SynthChannel& sc = plan.g[gi].ch[q0 + q];
                 const int oV = sc.outV, oX = sc.outX, oY = sc.outY;
-                T* hV = reinterpret_cast<T*>(H) + (oV >= 0 ? oV : 0) * chStride + colOff;
-                T* hX = reinterpret_cast<T*>(H) + (oX >= 0 ? oX : 0) * chStride + colOff;
-                T* hY = reinterpret_cast<T*>(H) + (oY >= 0 ? oY : 0) * chStride + colOff;
+                // rows z = zr0 + i dzr and L - z of this lane, as byte offsets
+                // within a channel (uniform channel bases, 32-bit lane offsets)
+                const int zr0 = 16 * zt + MF::row(kq, 0);
+                constexpr int dzr = sizeof(T) == 4 ? 1 : 4;  // MF::row(kq, i + 1) - MF::row(kq, i)
+                const unsigned offZ = (static_cast<unsigned>(zr0) * zStride + colOff) * sizeof(T);
+                const unsigned offL = (static_cast<unsigned>(L - zr0) * zStride + colOff) * sizeof(T);
+                const unsigned stepB = static_cast<unsigned>(dzr * zStride) * sizeof(T);
+                char* const hb = reinterpret_cast<char*>(H);
+                const size_t chBytes = chStride * sizeof(T);
+                char* const hV = hb + (oV >= 0 ? oV : 0) * chBytes;
+                const bool xy = oX >= 0 || oY >= 0;  // the i k and i l outputs (uniform)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int z = 16 * zt + MF::row(kq, i);
+                    const int z = zr0 + i * dzr;
                     const T e = accE[q][i], o = accO[q][i];
                     const T v1 = e + o, v2 = mat == 0 ? e - o : o - e;  // at z and at L - z
-                    // partner values first, with every lane active (a DPP
-                    // read from an inactive lane returns nothing)
-                    const T w1 = pairSwap(v1), w2 = pairSwap(v2);
-                    const T r1 = part ? w1 : -w1;  // part of i * value
-                    const T r2 = part ? w2 : -w2;
-                    if (z <= half && store) {
-                        const int o1 = z * zStride, o2 = (L - z) * zStride;
-                        if (oV >= 0) hV[o1] = v1;
-                        if (oX >= 0) hX[o1] = ks * r1;
-                        if (oY >= 0) hY[o1] = ls * r1;
-                        if (L - z != z) {
-                            if (oV >= 0) hV[o2] = v2;
-                            if (oX >= 0) hX[o2] = ks * r2;
-                            if (oY >= 0) hY[o2] = ls * r2;
+                    const unsigned o1 = offZ + i * stepB, o2 = offL - i * stepB;
+                    const bool lo = z <= half && store, hi = lo && L - z != z;
+                    if (oV >= 0) {
+                        if (lo) *reinterpret_cast<T*>(hV + o1) = v1;
+                        if (hi) *reinterpret_cast<T*>(hV + o2) = v2;
+                    }
+                    if (xy) {
+                        // partner values with every lane active (a DPP read
+                        // from an inactive lane returns nothing)
+                        const T w1 = pairSwap(v1), w2 = pairSwap(v2);
+                        const T r1 = part ? w1 : -w1;  // part of i * value
+                        const T r2 = part ? w2 : -w2;
+                        char* const hX = hb + (oX >= 0 ? oX : 0) * chBytes;
+                        char* const hY = hb + (oY >= 0 ? oY : 0) * chBytes;
+                        if (oX >= 0) {
+                            if (lo) *reinterpret_cast<T*>(hX + o1) = ks * r1;
+                            if (hi) *reinterpret_cast<T*>(hX + o2) = ks * r2;
+                        }
+                        if (oY >= 0) {
+                            if (lo) *reinterpret_cast<T*>(hY + o1) = ls * r1;
+                            if (hi) *reinterpret_cast<T*>(hY + o2) = ls * r2;
                         }
                     }
                 }
@@ -527,16 +542,20 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
 #pragma unroll
                     for (int r = 0; r < ZT; ++r) acc[q][r] = MF::run(a[r][u], b[u][q], acc[q][r]);
         };
+        // (scheduling barriers keep each load a whole batch ahead of its use)
         T a0[ZT][kS], a1[ZT][kS];
         load(a0, 0);
         int s0 = 0;
         for (; s0 + 2 * kS < nsteps; s0 += 2 * kS) {
             load(a1, s0 + kS);
+            __builtin_amdgcn_sched_barrier(0);
             run(a0, s0);
             load(a0, s0 + 2 * kS);
+            __builtin_amdgcn_sched_barrier(0);
             run(a1, s0 + kS);
         }
         load(a1, s0 + kS);
+        __builtin_amdgcn_sched_barrier(0);
         run(a0, s0);
         run(a1, s0 + kS);
         // outputs: the value and i k / i l times it (the partner lane n^1
@@ -1019,16 +1038,20 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
                 for (int t = 0; t < NT; ++t)
                     if ((projTarget<NT>(t) >= 2) == (fam == 1)) acc[t] = MF::run(a[u], b[u][t], acc[t]);
         };
+        // (scheduling barriers keep each load a whole batch ahead of its use)
         T a0[kS], a1[kS];
         load(a0, 0);
         int s0 = 0;
         for (; s0 + 2 * kS < nsteps; s0 += 2 * kS) {
             load(a1, s0 + kS);
+            __builtin_amdgcn_sched_barrier(0);
             run(a0, s0);
             load(a0, s0 + 2 * kS);
+            __builtin_amdgcn_sched_barrier(0);
             run(a1, s0 + kS);
         }
         load(a1, s0 + kS);
+        __builtin_amdgcn_sched_barrier(0);
         run(a0, s0);
         run(a1, s0 + kS);
     }
@@ -1717,6 +1740,9 @@ struct KernelImpl {
         else setupTyped<float>();
         const size_t plane = cfg.Ny * d.nkxP;
         const size_t chField = cfg.Nz * plane;
+        // the synthesis addresses a channel with 32-bit byte offsets
+        if (chField * tsize >= (size_t(1) << 32))
+            throw Failure{WV_UNSUPPORTED_OPERATION, "Grid too large: one spatial channel exceeds 4 GiB."};
         FV.alloc(4 * d.M * tsize);
         H.alloc(static_cast<size_t>(NH) * chField * tsize);
         G.alloc(static_cast<size_t>(NH) * chField * tsize);
