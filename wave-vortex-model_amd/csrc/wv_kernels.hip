@@ -76,7 +76,8 @@ struct SynthGroup {
     SynthChannel ch[4];
 };
 constexpr int kMaxGroups = 8;
-constexpr int kSynthMats = 5;
+constexpr int kSynthMats = 5;      // distinct synthesis operators
+constexpr int kSynthMatSlots = 8;  // operands one synthesis launch can take
 struct SynthPlan {
     int ngroups;
     double valueScale;
@@ -85,7 +86,7 @@ struct SynthPlan {
 };
 template <typename T>
 struct SynthMats {
-    const T* m[kSynthMats];
+    const T* m[kSynthMatSlots];
 };
 
 enum EpilogueMode : int { kFluxOut = 0, kRk1 = 1, kRk2 = 2, kRk3 = 3, kRk4 = 4 };
@@ -233,8 +234,11 @@ __device__ __forceinline__ double pairSwap(double v) {
 // E_z and O_z the sums over even and odd j the outputs are E + O at z and
 // E - O (DCT-I) or O - E (DST-I) at L - z, for z <= L/2 only: half the MFMA
 // work.  mats then holds the even-j parts of the two matrices (rows z <=
-// L/2, Zs of them) in m[0], m[1] and the odd-j parts in m[2], m[3]; a task
-// is one z tile of NQ channel slots of one group (ZT is 1).
+// L/2, Zs of them) in m[0], m[1], the odd-j parts in m[2], m[3], and the
+// same with column j times +m_j / 2 (DCT-I) or -m_j / 2 (DST-I), the
+// operators of the z derivatives, in m[4] ... m[7].  A task is one z tile of
+// NQ channel slots of one group (ZT is 1), all values or all z derivatives,
+// so that the MFMA B operand is the staged spectrum itself.
 template <typename T, int ZT, int NQ, int KS, bool FOLD>
 __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
                                                int ncol, int M, SynthMats<T> mats, int Zs, int Js, int Nz,
@@ -261,12 +265,13 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
         if (col < ncol) scol[threadIdx.x] = cols[col];
         else scol[threadIdx.x] = ColumnDev{0, 0, -1, 0, 0.0, 0.0, 0.0, 0.0, 0.0};
     }
-    // m_j for the z-derivative channels (Js <= 4 blockDim.x)
+    // m_j for the z-derivative channels (Js <= 4 blockDim.x; the folded
+    // operators carry m_j themselves)
     T mv[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int j = threadIdx.x + k * blockDim.x;
-        mv[k] = mj[j < Nj ? j : Nj - 1];
+        mv[k] = FOLD ? T(0) : mj[j < Nj ? j : Nj - 1];
     }
     __syncthreads();
     // stage spectra: wave w copies rows (field, column) r = w, w + nwaves, ...
@@ -306,10 +311,12 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
             }
         }
     }
+    if constexpr (!FOLD) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int j = threadIdx.x + k * blockDim.x;
-        if (j < Js) mjs[j] = j < Nj ? mv[k] : T(0);
+        for (int k = 0; k < 4; ++k) {
+            const int j = threadIdx.x + k * blockDim.x;
+            if (j < Js) mjs[j] = j < Nj ? mv[k] : T(0);
+        }
     }
     __syncthreads();
     const int n = lane & 15, kq = lane >> 4;
@@ -335,12 +342,14 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
         for (int g = 0; g < plan.ngroups; ++g) maxch = max(maxch, plan.g[g].nch);
         const int parts = (maxch + NQ - 1) / NQ;
         const int ntasks = plan.ngroups * parts * ztiles;
-        // operand rows of task tk's even-j and odd-j matrices (lane offset included)
+        // operand rows of task tk's even-j and odd-j matrices (lane offset
+        // included); a part's channels are all values or all z derivatives
         auto matRows = [&](int tk, const T*& pE, const T*& pO) {
+            const int g = tk % plan.ngroups;
             const int zt = tk / (plan.ngroups * parts);
-            const int mat = plan.g[tk % plan.ngroups].mat;
-            pE = mats.m[mat] + static_cast<size_t>(zt) * nsteps * 64 + lane;
-            pO = mats.m[mat + 2] + static_cast<size_t>(zt) * nsteps * 64 + lane;
+            const int m = plan.g[g].mat + (plan.g[g].ch[((tk / plan.ngroups) % parts) * NQ].dz ? 4 : 0);
+            pE = mats.m[m] + static_cast<size_t>(zt) * nsteps * 64 + lane;
+            pO = mats.m[m + 2] + static_cast<size_t>(zt) * nsteps * 64 + lane;
         };
         constexpr int kS = KS;
         // A operands in register batches of kS steps, two named batches used
@@ -363,17 +372,16 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
             const int zt = task / (plan.ngroups * parts);
             const int mat = plan.g[gi].mat, nch = plan.g[gi].nch;
             const int q0 = pt * NQ;  // first channel of this part
+            // B is the staged spectrum itself; the channel's factor (the value
+            // scale, or 1 for z derivatives, whose -/+ m_j / 2 is in the
+            // operator) and the column's part weight scale the outputs
+            const bool dzPart = plan.g[gi].ch[q0].dz != 0;
+            const T oscale = (dzPart ? T(1) : T(plan.valueScale)) * partWeight;
             int fieldOff[NQ];
-            T coefV[NQ], coefZ[NQ];
-            const T zsign = mat == 1 ? T(-0.5) : T(0.5);
-            const T vscale = T(plan.valueScale);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const bool live = q0 + q < nch;
-                const bool dz = live && plan.g[gi].ch[q0 + q].dz;
                 fieldOff[q] = live ? plan.g[gi].ch[q0 + q].field * 16 * Jp : 0;
-                coefV[q] = live && !dz ? vscale * partWeight : T(0);
-                coefZ[q] = dz ? zsign * partWeight : T(0);
             }
             const T *AE, *AO;
             matRows(task, AE, AO);
@@ -392,12 +400,11 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
 #pragma unroll
                 for (int u = 0; u < kS; ++u) {
                     const int je = 2 * (4 * (s0 + u) + kq);  // even j; the odd one is je + 1
-                    const T mE = mjs[je], mO = mjs[je + 1];
                     const T* fj = fvw + 2 * je;
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
-                        bE[u][q] = fj[fieldOff[q]] * (coefV[q] + coefZ[q] * mE);
-                        bO[u][q] = fj[fieldOff[q] + 2] * (coefV[q] + coefZ[q] * mO);
+                        bE[u][q] = fj[fieldOff[q]];
+                        bO[u][q] = fj[fieldOff[q] + 2];
                     }
                 }
 #pragma unroll
@@ -456,7 +463,7 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
                 for (int i = 0; i < 4; ++i) {
                     const int z = zr0 + i * dzr;
                     const T e = accE[q][i], o = accO[q][i];
-                    const T v1 = e + o, v2 = mat == 0 ? e - o : o - e;  // at z and at L - z
+                    const T v1 = (e + o) * oscale, v2 = (mat == 0 ? e - o : o - e) * oscale;  // at z and at L - z
                     const unsigned o1 = offZ + i * stepB, o2 = offL - i * stepB;
                     const bool lo = z <= half && store, hi = lo && L - z != z;
                     if (oV >= 0) {
@@ -1504,6 +1511,7 @@ struct KernelImpl {
     // even / odd j parts of the DCT-I, DST-I synthesis (rows z <= (Nz-1)/2) and
     // of the analysis (columns z <= (Nz-1)/2): constant N
     DeviceBuffer synCE, synSE, synCO, synSO, anaCE, anaCO, anaSE, anaSO;
+    DeviceBuffer synCmE, synSmE, synCmO, synSmO;  // folded, columns times +/- m_j / 2
     bool fold = false;
     size_t Zh = 0;  // folded z rows (multiple of 16)
     DeviceBuffer FV, H, G, P, Q, Tg, modeTab, jTab, stage, weighted, partial, scalar;
@@ -1527,7 +1535,7 @@ struct KernelImpl {
         size_t s = 0;
         for (const DeviceBuffer* b : {&rowLimit, &lrow, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &jTab, &stage, &weighted, &partial, &scalar})
+                                      &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &synCmE, &synSmE, &synCmO, &synSmO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &jTab, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
         return s;
     }
@@ -1608,10 +1616,15 @@ struct KernelImpl {
             // rows z <= half of the synthesis, split by the parity of j
             const size_t half = (d.c.Nz - 1) / 2, Jh = d.Js / 2, Jah = d.Ja / 2;
             Zh = (half + 1 + 15) / 16 * 16;
-            auto synPart = [&](const std::vector<double>& m, size_t par) {
+            // (scale != 0: column j times scale m_j, the z-derivative operator)
+            auto synPart = [&](const std::vector<double>& m, size_t par, double scale = 0.0) {
                 std::vector<double> out(Zh * Jh, 0.0);
                 for (size_t z = 0; z <= half; ++z)
-                    for (size_t jh = 0; jh < Jh; ++jh) out[z * Jh + jh] = m[z * d.Js + 2 * jh + par];
+                    for (size_t jh = 0; jh < Jh; ++jh) {
+                        const size_t j = 2 * jh + par;
+                        const double a = m[z * d.Js + j];
+                        out[z * Jh + jh] = scale == 0.0 ? a : j < d.c.Nj ? a * (scale * d.m[j]) : 0.0;
+                    }
                 return mfmaOperand(out, Zh, Jh);
             };
             auto anaPart = [&](const std::vector<double>& m, size_t par) {
@@ -1624,6 +1637,10 @@ struct KernelImpl {
             uploadAs<T>(synSE, synPart(d.synS, 0));
             uploadAs<T>(synCO, synPart(d.synC, 1));
             uploadAs<T>(synSO, synPart(d.synS, 1));
+            uploadAs<T>(synCmE, synPart(d.synC, 0, 0.5));
+            uploadAs<T>(synSmE, synPart(d.synS, 0, -0.5));
+            uploadAs<T>(synCmO, synPart(d.synC, 1, 0.5));
+            uploadAs<T>(synSmO, synPart(d.synS, 1, -0.5));
             uploadAs<T>(anaCE, anaPart(d.anaC, 0));
             uploadAs<T>(anaCO, anaPart(d.anaC, 1));
             uploadAs<T>(anaSE, anaPart(d.anaS, 0));
@@ -1873,16 +1890,22 @@ struct KernelImpl {
                 nout += (sp.g[g].ch[q].outV >= 0) + (sp.g[g].ch[q].outX >= 0) + (sp.g[g].ch[q].outY >= 0);
         const double bytes = static_cast<double>(nout) * d.c.Nz * ncol * sizeof(cx<T>) + 4.0 * d.M * sizeof(cx<T>);
         const size_t lds = 4 * 8 * static_cast<size_t>(synthJStride<T>(static_cast<int>(d.Js))) * sizeof(cx<T>) + d.Js * sizeof(T);
-        // the folded DCT-I / DST-I path serves the constant-N matrices
-        const bool folded = fold && mats == nullptr;
+        constexpr int FQ = 2;
+        // the folded DCT-I / DST-I path serves the constant-N matrices, for
+        // plans whose parts of FQ channels are each all values or all z
+        // derivatives (one operator per task)
+        bool homogeneous = true;
+        for (int g = 0; g < sp.ngroups; ++g)
+            for (int q = 0; q < sp.g[g].nch; ++q)
+                homogeneous = homogeneous && sp.g[g].ch[q].dz == sp.g[g].ch[q - q % FQ].dz;
+        const bool folded = fold && mats == nullptr && homogeneous;
         const int ztiles = static_cast<int>((folded ? Zh : d.Zs) / 16);
-        // Folded tasks take two channel slots (64 VGPRs in FP32: 8 waves per
-        // SIMD, so 6 workgroups per CU, as many as the LDS holds) and run on
-        // four waves, one per SIMD, which deal the tasks round-robin.  (A
+        // Folded tasks take FQ = 2 channel slots (64 VGPRs in FP32: 8 waves
+        // per SIMD, so 6 workgroups per CU, as many as the LDS holds) and run
+        // on four waves, one per SIMD, which deal the tasks round-robin.  (A
         // five-wave workgroup puts two of its waves on one SIMD, which then
         // has twice the work of the others.)  Unfolded: the largest of 4 ... 1
         // waves that divides the tasks.
-        constexpr int FQ = 2;
         const int tasks = folded ? sp.ngroups * ((maxch + FQ - 1) / FQ) * ztiles : sp.ngroups * ((ztiles + ZT - 1) / ZT);
         int waves = folded ? std::min(4, tasks) : 4;
         while (!folded && tasks % waves) --waves;
@@ -1896,7 +1919,8 @@ struct KernelImpl {
         else
             fn = maxch <= 2 ? (ks4 ? k_synth<T, ZT, 2, 4, false> : k_synth<T, ZT, 2, 2, false>)
                             : (ks4 ? k_synth<T, ZT, 4, 4, false> : k_synth<T, ZT, 4, 2, false>);
-        const SynthMats<T> m = folded ? SynthMats<T>{{synCE.as<T>(), synSE.as<T>(), synCO.as<T>(), synSO.as<T>(), nullptr}}
+        const SynthMats<T> m = folded ? SynthMats<T>{{synCE.as<T>(), synSE.as<T>(), synCO.as<T>(), synSO.as<T>(),
+                                                      synCmE.as<T>(), synSmE.as<T>(), synCmO.as<T>(), synSmO.as<T>()}}
                                       : (mats ? *mats : synthMats<T>());
         timed(name, bytes, s, [&] {
             hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s,
