@@ -1914,8 +1914,13 @@ struct KernelImpl {
         using Fn = void (*)(const cx<T>*, const ColumnDev*, int, int, SynthMats<T>, int, int, int, int, int, int,
                             const T*, SynthPlan, cx<T>*);
         Fn fn;
+        // folded FP32: operand batches of up to 6 steps (a wave keeps a
+        // whole batch of loads in flight; the MFMA work of one small batch is
+        // much shorter than a load's latency).  FP64 keeps short batches (its
+        // registers bound the resident waves).
         if (folded)
-            fn = ks4 ? k_synth<T, 1, FQ, 4, true> : k_synth<T, 1, FQ, 2, true>;
+            fn = sizeof(T) == 4 && nsteps % 12 == 0 ? k_synth<T, 1, FQ, 6, true>
+                                  : ks4 ? k_synth<T, 1, FQ, 4, true> : k_synth<T, 1, FQ, 2, true>;
         else
             fn = maxch <= 2 ? (ks4 ? k_synth<T, ZT, 2, 4, false> : k_synth<T, ZT, 2, 2, false>)
                             : (ks4 ? k_synth<T, ZT, 4, 4, false> : k_synth<T, ZT, 4, 2, false>);
