@@ -8,10 +8,14 @@ half geostrophic energy, rms horizontal speed 5 cm/s); fixed dt = 0.5 CFL.
 A "step" is one classic RK4 step (four nonlinear-flux evaluations plus the
 stage combinations) with the state resident in HBM.
 
-Prints one JSON line.  value = mixed-precision RK4 steps/s; the FP64 rate,
-the mixed-vs-FP64 relative total-energy error after the timed steps, the
-dominant kernel's roofline (HIP events, live) and the CPU oracle baseline
-ride along.
+Prints one JSON line.  value = mixed-precision RK4 steps/s, the median of
+`--runs` timed repetitions of exactly `--steps` steps each (after `--warmup`
+untimed steps); the FP64 rate (same protocol), the mixed-vs-FP64 relative
+total-energy error after the timed steps and its maximum over every step of
+an untimed lockstep run (`--energy-steps`, 200 by default), the
+reference-style relative error max|delta| / max|FP64| of Ap, Am, A0 at the end
+of that run, the dominant kernel's roofline (HIP events, live) and the CPU
+oracle baseline ride along.
 """
 import argparse
 import json
@@ -60,16 +64,40 @@ def initial_state(k, seed=4001, jmax=64, urms=0.05):
     return (Ap, Am, A0), dt
 
 
-def timed_steps(k, state, dt, steps, warmup):
+def timed_steps(k, state, dt, steps, warmup, runs=1):
+    """Seconds of each of `runs` repetitions of exactly `steps` RK4 steps
+    (the state carries on from one repetition to the next), after `warmup`
+    untimed steps; returns (seconds per run, device state)."""
     import torch
     dev = [k.to_device(a) for a in state]
     t = k.rk4_advance(*dev, t=0.0, h=dt, nsteps=warmup) if warmup else 0.0
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    k.rk4_advance(*dev, t=t, h=dt, nsteps=steps)
+    secs = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        t = k.rk4_advance(*dev, t=t, h=dt, nsteps=steps)
+        torch.cuda.synchronize()
+        secs.append(time.perf_counter() - t0)
+    return secs, dev
+
+
+def lockstep_errors(k32, k64, state, dt, steps):
+    """Mixed and FP64 runs advanced one step at a time from the same state:
+    max over steps of |E_mixed - E_fp64| / E_fp64, and max|delta| / max|FP64|
+    of Ap, Am, A0 after the last step (the reference's relative-error form)."""
+    import torch
+    d32 = [k32.to_device(a) for a in state]
+    d64 = [k64.to_device(a) for a in state]
+    t32 = t64 = 0.0
+    worst = 0.0
+    for _ in range(steps):
+        t32 = k32.rk4_advance(*d32, t=t32, h=dt, nsteps=1)
+        t64 = k64.rk4_advance(*d64, t=t64, h=dt, nsteps=1)
+        e32, e64 = k64.total_energy(*d32), k64.total_energy(*d64)
+        worst = max(worst, abs(e32 - e64) / abs(e64))
+    rel = {name: float((a - b).abs().max() / b.abs().max()) for name, a, b in zip(("Ap", "Am", "A0"), d32, d64)}
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    return elapsed, dev
+    return worst, rel
 
 
 def stage_profile(k, state, dt, steps=2):
@@ -114,6 +142,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--runs", type=int, default=5, help="timed repetitions of --steps steps (median reported)")
+    ap.add_argument("--energy-steps", type=int, default=200, help="steps of the untimed lockstep error run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile", choices=["mixed", "fp64", "both"], default=None,
@@ -135,8 +165,8 @@ def main():
         state, dt = initial_state(kk)
         for prec in (("mixed", "fp64") if args.profile == "both" else (args.profile,)):
             k = kk if prec == "fp64" else wvamd.WVTransformConstantStratificationKernel(cfg, prec)
-            sec, _ = timed_steps(k, state, dt, args.steps, args.warmup)
-            print(f"{prec}: {args.steps / sec:.2f} steps/s", flush=True)
+            secs, _ = timed_steps(k, state, dt, args.steps, args.warmup)
+            print(f"{prec}: {args.steps / secs[0]:.2f} steps/s", flush=True)
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         torch.mul(a, 2.0, out=b)  # calibration kernel: reads and writes exactly 512 MiB
@@ -147,13 +177,19 @@ def main():
     k32 = wvamd.WVTransformConstantStratificationKernel(cfg, "mixed")
     state, dt = initial_state(k64)
 
-    sec32, d32 = timed_steps(k32, state, dt, args.steps, args.warmup)
-    sec64, d64 = timed_steps(k64, state, dt, args.steps, args.warmup)
+    runs = max(1, args.runs)
+    secs32, d32 = timed_steps(k32, state, dt, args.steps, args.warmup, runs)
+    secs64, d64 = timed_steps(k64, state, dt, args.steps, args.warmup, runs)
     e32 = k64.total_energy(*d32)
     e64 = k64.total_energy(*d64)
     energy_err = abs(e32 - e64) / abs(e64)
+    sec32 = float(np.median(secs32))
+    sec64 = float(np.median(secs64))
     sps32 = args.steps / sec32
     sps64 = args.steps / sec64
+    del d32, d64
+    worst_energy, coef_err = lockstep_errors(k32, k64, state, dt, args.energy_steps) if args.energy_steps > 0 \
+        else (None, None)
 
     host32 = host_flux_rate(k32, state)
     host64 = host_flux_rate(k64, state)
@@ -190,6 +226,11 @@ def main():
                    "Lx_m": 200e3, "Lz_m": 1300.0, "Nj": 85, "Nkl": k64.Nkl, "dt_s": dt},
         "fp64_steps_per_s": round(sps64, 3), "fp64_ms_per_step": round(1e3 * sec64 / args.steps, 4),
         "mixed_over_fp64": round(sps32 / sps64, 3), "energy_rel_err_mixed_vs_fp64": energy_err,
+        "runs": runs, "mixed_steps_per_s_runs": [round(args.steps / s, 2) for s in secs32],
+        "fp64_steps_per_s_runs": [round(args.steps / s, 2) for s in secs64],
+        "lockstep": {"steps": args.energy_steps, "energy_rel_err_max_over_steps": worst_energy,
+                     "coef_rel_err_max_abs": coef_err,
+                     "note": "APV not reported: it needs eta_true, which this path does not form"},
         "roofline": roofline, "reference_dataflow_roofline": step_model,
         "host_memory_flux_per_s": {"mixed": round(host32, 2), "fp64": round(host64, 2),
                                    "note": "PCIe-inclusive: state copied in and flux copied out per call"},
