@@ -241,9 +241,9 @@ __device__ __forceinline__ double pairSwap(double v) {
 // so that the MFMA B operand is the staged spectrum itself.
 template <typename T, int ZT, int NQ, int KS, bool FOLD>
 __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, const ColumnDev* __restrict__ cols,
-                                               int ncol, int M, SynthMats<T> mats, int Zs, int Js, int Nz,
-                                               int Nj, int Ny, int nkxP, const T* __restrict__ mj, SynthPlan plan,
-                                               cx<T>* __restrict__ H) {
+                                               int ncol, int M, SynthMats<T> mats, int Zs, int zmid, int Js,
+                                               int Nz, int Nj, int Ny, int nkxP, const T* __restrict__ mj,
+                                               SynthPlan plan, cx<T>* __restrict__ H) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
     // [4 fields][8 columns][Jp]: the staging stores run along j, and the
@@ -443,6 +443,42 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
             }
             __builtin_amdgcn_sched_barrier(0);
             run(e1, o1, s0 + kS);
+            if (zmid >= 0 && zt == 0) {
+                // The midpoint row z = zmid = L/2, alone in the tile after the
+                // last one contracted on MFMA (Nz = 16 m + 1): a dot product
+                // over j per lane (its k-row group kq), summed over the four
+                // groups, with that tile's row 0 as coefficients (lane 16 kq
+                // of each step).  At the midpoint z = L - z: one value, E + O.
+                const T* cE = AE + static_cast<size_t>(ztiles - zt) * nsteps * 64 + 16 * kq - lane;
+                const T* cO = AO + static_cast<size_t>(ztiles - zt) * nsteps * 64 + 16 * kq - lane;
+                T mid[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) mid[q] = T(0);
+                for (int s = 0; s < nsteps; s += 4) {  // nsteps is a multiple of 4
+                    T ce[4], co[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        ce[u] = cE[(s + u) * 64];
+                        co[u] = cO[(s + u) * 64];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const T* fj = fvw + 4 * (4 * (s + u) + kq);
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) mid[q] += ce[u] * fj[fieldOff[q]] + co[u] * fj[fieldOff[q] + 2];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    mid[q] += __shfl_xor(mid[q], 16);
+                    mid[q] += __shfl_xor(mid[q], 32);
+                    if (q0 + q >= nch) continue;
+                    const int oV = plan.g[gi].ch[q0 + q].outV;
+                    if (oV >= 0 && kq == 0 && store)
+                        reinterpret_cast<T*>(H)[oV * chStride + static_cast<size_t>(zmid) * zStride + colOff] =
+                            mid[q] * oscale;
+                }
+            }
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 if (q0 + q >= nch) continue;
@@ -949,8 +985,8 @@ constexpr size_t projLdsBytes(int NT, int Zq, int Ja) {
 // m[3] the odd-j rows, and the first Ja/32 waves take even j.
 template <typename T, int NT, int KS, bool FOLD>
 __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, const ColumnDev* __restrict__ cols,
-                                               int ncol, SynthMats<T> ana, int Ja, int Za, int Nz, int Nj, int Ny,
-                                               int nkxP, int M, cx<T>* __restrict__ Tg) {
+                                               int ncol, SynthMats<T> ana, int Ja, int Za, int Zk, int zmid, int Nz,
+                                               int Nj, int Ny, int nkxP, int M, cx<T>* __restrict__ Tg) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [NT][Za][8], folded [NT][2][Za][8] (sums, differences)
@@ -966,7 +1002,8 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     const int n = lane & 15, kq = lane >> 4;
     const int part = n & 1;
     const size_t tStride = static_cast<size_t>(Nz) * Ny * nkxP;
-    const int nsteps = Za / 4;
+    const int tsteps = Za / 4;  // operator steps per row tile (the table's layout)
+    const int nsteps = Zk / 4;  // steps contracted on MFMA: rows z < Zk
     const int L = Nz - 1;
     __syncthreads();  // scol ready
     // stage the columns of every target: lane (zr, c) of wave w takes column
@@ -1019,7 +1056,8 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     constexpr int kS = KS;  // the step count Za / 4 is a multiple of 2 kS
 #pragma unroll
     for (int fam = 0; fam < 2; ++fam) {
-        const T* Aj = ana.m[fam + 2 * P] + static_cast<size_t>(jh) * nsteps * 64 + lane;
+        const T* At = ana.m[fam + 2 * P] + static_cast<size_t>(jh) * tsteps * 64;  // this row tile
+        const T* Aj = At + lane;
         // staged block read by this family: the columns themselves, or
         // (folded) the sums or differences, by family and parity of j
         const int par = FOLD ? (fam ^ P) : 0;
@@ -1061,6 +1099,22 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
         __builtin_amdgcn_sched_barrier(0);
         run(a0, s0);
         run(a1, s0 + kS);
+        if (zmid >= 0) {
+            // the one staged row z = zmid beyond the MFMA tiles (the folded
+            // midpoint L/2): a rank-1 term, operator column zmid of this
+            // tile's rows times the staged value of the lane's column
+            T am[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) am[i] = At[(zmid >> 2) * 64 + 16 * (zmid & 3) + MF::row(kq, i)];
+            const int par = FOLD ? (fam ^ P) : 0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                if ((projTarget<NT>(t) >= 2) == (fam == 1)) {
+                    const T sm = FOLD ? qw[2 * (((2 * t + par) * Za + zmid) * 8)] : qw[2 * ((t * Za + zmid) * 8)];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[t][i] += am[i] * sm;
+                }
+        }
     }
     // The analysis results go through LDS as res[t][j][n] (the staged columns
     // are no longer needed) so that the stores' lanes walk j: lane (c, jj)
@@ -1899,7 +1953,13 @@ struct KernelImpl {
             for (int q = 0; q < sp.g[g].nch; ++q)
                 homogeneous = homogeneous && sp.g[g].ch[q].dz == sp.g[g].ch[q - q % FQ].dz;
         const bool folded = fold && mats == nullptr && homogeneous;
-        const int ztiles = static_cast<int>((folded ? Zh : d.Zs) / 16);
+        // folded, when the last 16-row tile holds only the midpoint row
+        // z = L/2 (Nz = 16 m + 1): that row is a dot product beside the MFMA
+        // tiles
+        const int half = static_cast<int>((d.c.Nz - 1) / 2);
+        const bool midRow = folded && (d.c.Nz - 1) % 2 == 0 && (half + 1) % 16 == 1;
+        const int ztiles = static_cast<int>((folded ? Zh : d.Zs) / 16) - (midRow ? 1 : 0);
+        const int zmid = midRow ? half : -1;
         // Folded tasks take FQ = 2 channel slots (64 VGPRs in FP32: 8 waves
         // per SIMD, so 6 workgroups per CU, as many as the LDS holds) and run
         // on four waves, one per SIMD, which deal the tasks round-robin.  (A
@@ -1912,7 +1972,7 @@ struct KernelImpl {
         const int nsteps = static_cast<int>(folded ? d.Js / 8 : d.Js / 4);
         const bool ks4 = nsteps % 8 == 0;
         using Fn = void (*)(const cx<T>*, const ColumnDev*, int, int, SynthMats<T>, int, int, int, int, int, int,
-                            const T*, SynthPlan, cx<T>*);
+                            int, const T*, SynthPlan, cx<T>*);
         Fn fn;
         // folded FP32: operand batches of up to 6 steps (a wave keeps a
         // whole batch of loads in flight; the MFMA work of one small batch is
@@ -1930,7 +1990,7 @@ struct KernelImpl {
         timed(name, bytes, s, [&] {
             hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s,
                                in ? in->as<cx<T>>() : FV.as<cx<T>>(), cols.as<ColumnDev>(), static_cast<int>(ncol),
-                               static_cast<int>(d.M), m, ztiles * 16, static_cast<int>(d.Js),
+                               static_cast<int>(d.M), m, ztiles * 16, zmid, static_cast<int>(d.Js),
                                static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
                                static_cast<int>(d.nkxP), mj.as<T>(), sp, out ? out->as<cx<T>>() : H.as<cx<T>>());
         });
@@ -2023,16 +2083,22 @@ struct KernelImpl {
         timed("proj", bytes, s, [&] {
             const int njt = static_cast<int>(d.Ja / 16);
             const dim3 grid(static_cast<unsigned>(ncg)), block(static_cast<unsigned>(64 * njt));
-            const int Zq = static_cast<int>(fold ? Zh : d.Za);  // contraction rows
+            const int Zq = static_cast<int>(fold ? Zh : d.Za);  // contraction rows (staged)
             const size_t lds = projLdsBytes<T>(NT, fold ? 2 * Zq : Zq, static_cast<int>(d.Ja));
+            // folded, when the last 16-row tile holds only the midpoint row
+            // z = L/2 (Nz = 16 m + 1): MFMA over the full tiles, that row as
+            // a rank-1 term
+            const int half = static_cast<int>((d.c.Nz - 1) / 2);
+            const bool mid = fold && (d.c.Nz - 1) % 2 == 0 && (half + 1) % 16 == 1;
+            const int Zk = mid ? Zq - 16 : Zq, zmid = mid ? half : -1;
             const int ncolI = static_cast<int>(ncol), Ja = static_cast<int>(d.Ja), Nz = static_cast<int>(d.c.Nz),
                       Nj = static_cast<int>(d.c.Nj), Ny = static_cast<int>(d.c.Ny), nkxP = static_cast<int>(d.nkxP),
                       M = static_cast<int>(d.M);
             // batch length: 4, 3 or 2 steps, whichever divides the step count in pairs
-            const int nsteps = Zq / 4;
+            const int nsteps = Zk / 4;
             const int ks = nsteps % 8 == 0 ? 4 : nsteps % 6 == 0 ? 3 : 2;
             using Fn = void (*)(const cx<T>*, const ColumnDev*, int, SynthMats<T>, int, int, int, int, int, int, int,
-                                cx<T>*);
+                                int, int, cx<T>*);
             Fn fn;
             if (fold)
                 fn = NT == 4 ? (ks == 4 ? k_proj<T, 4, 4, true> : ks == 3 ? k_proj<T, 4, 3, true> : k_proj<T, 4, 2, true>)
@@ -2042,8 +2108,8 @@ struct KernelImpl {
                              : (ks == 4 ? k_proj<T, 3, 4, false> : ks == 3 ? k_proj<T, 3, 3, false> : k_proj<T, 3, 2, false>);
             const SynthMats<T> ana = fold ? SynthMats<T>{{anaCE.as<T>(), anaSE.as<T>(), anaCO.as<T>(), anaSO.as<T>(), nullptr}}
                                           : SynthMats<T>{{anaC.as<T>(), anaS.as<T>(), nullptr, nullptr, nullptr}};
-            hipLaunchKernelGGL(fn, grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI, ana, Ja, Zq, Nz, Nj,
-                               Ny, nkxP, M, Tg.as<cx<T>>());
+            hipLaunchKernelGGL(fn, grid, block, lds, s, Q.as<cx<T>>(), cols.as<ColumnDev>(), ncolI, ana, Ja, Zq, Zk, zmid,
+                               Nz, Nj, Ny, nkxP, M, Tg.as<cx<T>>());
         });
     }
 
