@@ -333,8 +333,7 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile, const Stratif
     // Power-of-two n: the Stockham pass twiddles of the LDS FFT, pass (Ns, R)
     // entry (r, k) at r*Ns + k - 1 holding exp(-2 pi i r k / (Ns R)), so that
     // consecutive k (consecutive threads) read consecutive entries; the pass
-    // order is the FFT's (radix 2^(log2 n mod lr) first, then radix 2^lr;
-    // lr = 4, or 2 for the radix-4 sequence).
+    // order is the FFT's (radix 2^(log2 n mod lr) first, then radix 2^lr).
     // Other n: the plain table exp(-2 pi i q / n) of the direct DFT.
     auto twiddles = [](size_t n, std::vector<double>& out, int lr) {
         out.assign(2 * n, 0.0);
@@ -359,7 +358,6 @@ std::string Descriptor::build(const wv_config& cfg, size_t kxTile, const Stratif
         while (Ns < n) pass(size_t{1} << lr);
     };
     twiddles(c.Nx, twX, 4);
-    twiddles(c.Nx, twX4, 2);
     twiddles(c.Ny, twY, 4);
     return {};
 }

@@ -82,7 +82,6 @@ struct Descriptor {
     size_t Zs = 0, Js = 0, Ja = 0, Za = 0;
     std::vector<double> synC, synS, anaC, anaS, synGz, synFh, synEz;
     std::vector<double> twX, twY;  // FFT twiddles (Stockham pass order for powers of two), interleaved
-    std::vector<double> twX4;      // x twiddles of the radix-4 pass sequence
 
     // Returns an empty string on success, otherwise the reference's message.
     // strat != nullptr selects the arbitrary-stratification vertical structure.

@@ -745,10 +745,10 @@ template <typename T>
 __host__ __device__ constexpr int xfusedStride(int Nx) { return ldsLength(Nx); }
 template <typename T>
 __host__ __device__ constexpr int xfusedRowStride(int Nx, int pairs) { return pairs * xfusedStride<T>(Nx); }
-// LDS bytes of a k_xfused block of RB rows (two twiddle tables + transforms)
+// LDS bytes of a k_xfused block of RB rows (twiddle table + transforms)
 template <typename T>
 __host__ __device__ constexpr size_t xfusedLdsBytes(int Nx, int pairs, int RB) {
-    return (2 * static_cast<size_t>(Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(Nx, pairs)) * sizeof(cx<T>);
+    return (static_cast<size_t>(Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(Nx, pairs)) * sizeof(cx<T>);
 }
 // rows per k_xfused block: two while four blocks still fit a CU's 160 KB of LDS
 template <typename T>
@@ -792,14 +792,15 @@ struct XMap<3> {
 // Per (y, z) row: c2r of the spatial channels along x (two real transforms
 // per complex FFT), the NT advection products and r2c of the products.
 // kxv[k] = the x wavenumber of half-spectrum column k.  L2N > 0 fixes
-// Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored);
-// the forward transforms then run radix-4 passes (twX4), so that all threads
-// share their four transforms.
+// Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored).
+// The forward transforms run the same radix-16 passes as the inverse ones, on
+// the first 16 threads per transform (one wave for four transforms): fewer
+// instructions, passes and barriers than radix-4 passes spread over all
+// threads.
 template <typename T, int NT, int RB, int L2N>
 __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P,
                                                         int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
-                                                        const cx<T>* __restrict__ twX, const cx<T>* __restrict__ twX4,
-                                                        const T* __restrict__ kxv, const cx<T>* __restrict__ zeros,
+                                                        const cx<T>* __restrict__ twX, const T* __restrict__ kxv, const cx<T>* __restrict__ zeros,
                                                         T etaScale) {
     using XM = XMap<NT>;
     constexpr int PAIRS = XM::pairs;
@@ -809,13 +810,11 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     const int NxP = xfusedStride<T>(Nx);
     const int RS = xfusedRowStride<T>(Nx, PAIRS);
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
-    cx<T>* tw4 = tw + Nx;
-    cx<T>* buf = tw4 + Nx;  // [RB][RS], row rr holds PAIRS transforms of stride NxP
+    cx<T>* buf = tw + Nx;  // [RB][RS], row rr holds PAIRS transforms of stride NxP
     const int nrows = Ny * Nz;
     const int row0 = blockIdx.x * RB;  // row = z*Ny + y
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     loadTwiddles(tw, twX, Nx);
-    if (L2N > 0) loadTwiddles(tw4, twX4, Nx);
     // Transform t = rr*PAIRS + p packs spatial channels 2p (real part) and
     // 2p+1 (imaginary part) of row rr.  Wave w fills transforms w, w + 4, ...:
     // each retained entry k < nkx is read once and written at k and,
@@ -906,10 +905,7 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     }
     __syncthreads();
     // forward transforms: pairs 0 and 1 of every row
-    if constexpr (L2N > 0)
-        fftLds<T, -1, L2N, 2>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw4);
-    else
-        fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
+    fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
     // wave w unpacks (row, pair) rq = w, w + nwaves, ...; its lanes walk k
     constexpr int nwaves = kFftThreads / 64;
     for (int rq = waveIndex(); rq < RB * 2; rq += nwaves)
@@ -1561,7 +1557,7 @@ struct KernelImpl {
 
     DeviceBuffer cols, omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled, ApmD, ApmTE, A0TE,
         apmW, inertialScale, fWaveScale, gWaveScale, Fg, Gg;
-    DeviceBuffer synC, synS, anaC, anaS, synGz, synFh, synEz, twX, twX4, twY, mj, kxv, zeroRow;
+    DeviceBuffer synC, synS, anaC, anaS, synGz, synFh, synEz, twX, twY, mj, kxv, zeroRow;
     // even / odd j parts of the DCT-I, DST-I synthesis (rows z <= (Nz-1)/2) and
     // of the analysis (columns z <= (Nz-1)/2): constant N
     DeviceBuffer synCE, synSE, synCO, synSO, anaCE, anaCO, anaSE, anaSO;
@@ -1589,7 +1585,7 @@ struct KernelImpl {
         size_t s = 0;
         for (const DeviceBuffer* b : {&rowLimit, &lrow, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
                                       &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &synGz, &synFh, &synEz, &twX, &twX4, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &synCmE, &synSmE, &synCmO, &synSmO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &jTab, &stage, &weighted, &partial, &scalar})
+                                      &anaS, &synGz, &synFh, &synEz, &twX, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &synCmE, &synSmE, &synCmO, &synSmO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &jTab, &stage, &weighted, &partial, &scalar})
             s += b->bytes;
         return s;
     }
@@ -1701,7 +1697,6 @@ struct KernelImpl {
             uploadAs<T>(anaSO, anaPart(d.anaS, 1));
         }
         uploadAs<T>(twX, d.twX);
-        uploadAs<T>(twX4, d.twX4);
         uploadAs<T>(zeroRow, std::vector<double>(2 * d.nkxP, 0.0));  // a row of zero spectra
         uploadAs<T>(twY, d.twY);
         {
@@ -2042,8 +2037,8 @@ struct KernelImpl {
         const T es = static_cast<T>(std::ldexp(1.0, static_cast<int>(std::lround(std::log2(d.c.N0 > 0 ? d.c.N0 : 1.0)))));
         hipLaunchKernelGGL((k_xfused<T, NTc, RB, L2N>), dim3(static_cast<unsigned>((nrows + RB - 1) / RB)),
                            dim3(kFftThreads), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, log2Exact(d.c.Nx), Ny, Nz,
-                           static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), twX4.as<cx<T>>(),
-                           kxv.as<T>(), zeroRow.as<cx<T>>(), es);
+                           static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), kxv.as<T>(),
+                           zeroRow.as<cx<T>>(), es);
     }
 
     // Nx = 64 ... 512 run kernels compiled for that length; others the
