@@ -750,10 +750,12 @@ template <typename T>
 __host__ __device__ constexpr size_t xfusedLdsBytes(int Nx, int pairs, int RB) {
     return (static_cast<size_t>(Nx) + static_cast<size_t>(RB) * xfusedRowStride<T>(Nx, pairs)) * sizeof(cx<T>);
 }
-// rows per k_xfused block: two while four blocks still fit a CU's 160 KB of LDS
+// rows per k_xfused block: one.  At Nx = 256 in FP32 a one-row block takes
+// 19 KB of LDS, so 8 blocks (32 waves) share a CU; two-row blocks (37 KB, 4
+// blocks, all 256 threads busy in the inverse passes) measured 4 % slower.
 template <typename T>
-__host__ __device__ constexpr int xfusedRows(int Nx, int pairs) {
-    return xfusedLdsBytes<T>(Nx, pairs, 2) <= 40 * 1024 ? 2 : 1;
+__host__ __device__ constexpr int xfusedRows(int, int) {
+    return 1;
 }
 
 // Spatial channels of the flux products, packed two per complex x transform.
