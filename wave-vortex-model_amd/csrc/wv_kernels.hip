@@ -958,11 +958,16 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
 template <int NT>
 __host__ __device__ constexpr int projTarget(int t) { return NT == 4 ? t : (t == 2 ? 3 : t); }
 
-// LDS bytes of k_proj: the staged columns (Zq rows per target), later the
-// transposed results.
+// k_proj stages one family's targets at a time in FP64 (its LDS then bounds
+// the workgroups per CU), all targets at once in FP32.
+template <typename T>
+constexpr bool projSplitStaging() { return sizeof(T) == 8; }
+// LDS bytes of k_proj: the staged columns (Zq rows per target; one family's
+// targets at most two, or all), later the transposed results.
 template <typename T>
 constexpr size_t projLdsBytes(int NT, int Zq, int Ja) {
-    const size_t staged = static_cast<size_t>(NT) * Zq * 8 * sizeof(cx<T>);
+    const int nst = projSplitStaging<T>() ? (NT < 2 ? NT : 2) : NT;
+    const size_t staged = static_cast<size_t>(nst) * Zq * 8 * sizeof(cx<T>);
     const size_t results = static_cast<size_t>(Ja) * 4 * 17 * sizeof(T);
     return staged > results ? staged : results;
 }
@@ -987,7 +992,11 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
                                                int Nj, int Ny, int nkxP, int M, cx<T>* __restrict__ Tg) {
     using MF = Mfma<T>;
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* qs = reinterpret_cast<cx<T>*>(smem);  // [NT][Za][8], folded [NT][2][Za][8] (sums, differences)
+    // staged targets, slot s: [s][Za][8], folded [s][2][Za][8] (sums,
+    // differences); slot = target, or with split staging the target's index
+    // within its family (t & 1)
+    constexpr bool kSplit = projSplitStaging<T>();
+    cx<T>* qs = reinterpret_cast<cx<T>*>(smem);
     __shared__ ColumnDev scol[8];
     const int lane = threadIdx.x & 63;
     const int jt = waveIndex();
@@ -1004,10 +1013,14 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     const int nsteps = Zk / 4;  // steps contracted on MFMA: rows z < Zk
     const int L = Nz - 1;
     __syncthreads();  // scol ready
-    // stage the columns of every target: lane (zr, c) of wave w takes column
-    // c of levels z = 8 g + zr, g = w, w + nwaves, ...; all loads of a batch
-    // (every target; two levels each when folded) precede its stores
-    {
+    // stage the columns of family fam's targets (0, 1 | 2, 3; NT = 3: 0, 1 |
+    // 2, whose projection target is 3), or of all targets (fam < 0): lane
+    // (zr, c) of wave w takes column c of levels z = 8 g + zr, g = w,
+    // w + nwaves, ...; all loads of a batch (two levels each when folded)
+    // precede its stores.
+    auto stage = [&](int fam) {
+        constexpr int NS = kSplit ? 2 : NT;  // slots staged per call
+        const int t0 = fam < 0 ? 0 : 2 * fam;
         const int nwaves = blockDim.x >> 6;
         const int c = lane & 7, zr = lane >> 3;
         const ColumnDev& cc = scol[c];
@@ -1018,30 +1031,31 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
             const int z = 8 * g + zr;
             const int zc = z < Nz ? z : Nz - 1;  // clamped: loads are unconditional
             const int zm = L - zc >= 0 ? L - zc : 0;
-            cx<T> v[NT], w[NT];
+            cx<T> v[NS], w[NS];
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                v[t] = Q[t * tStride + zc * plane + colBase];
-                if constexpr (FOLD) w[t] = Q[t * tStride + zm * plane + colBase];
+            for (int tf = 0; tf < NS; ++tf) {
+                const int tt = t0 + tf < NT ? t0 + tf : NT - 1;
+                v[tf] = Q[tt * tStride + zc * plane + colBase];
+                if constexpr (FOLD) w[tf] = Q[tt * tStride + zm * plane + colBase];
             }
             __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
             if (z >= Za) continue;
             const cx<T> zero{T(0), T(0)};
 #pragma unroll
-            for (int t = 0; t < NT; ++t) {
+            for (int tf = 0; tf < NS; ++tf) {
+                if (t0 + tf >= NT) continue;
                 if constexpr (FOLD) {
                     const bool ok = colLive && 2 * z <= L, pair = z < L - z;
-                    const cx<T> sv = !ok ? zero : pair ? cadd(v[t], w[t]) : v[t];
-                    const cx<T> dv = !ok || !pair ? zero : csub(v[t], w[t]);
-                    qs[((2 * t) * Za + z) * 8 + c] = sv;
-                    qs[((2 * t + 1) * Za + z) * 8 + c] = dv;
+                    const cx<T> sv = !ok ? zero : pair ? cadd(v[tf], w[tf]) : v[tf];
+                    const cx<T> dv = !ok || !pair ? zero : csub(v[tf], w[tf]);
+                    qs[((2 * tf) * Za + z) * 8 + c] = sv;
+                    qs[((2 * tf + 1) * Za + z) * 8 + c] = dv;
                 } else {
-                    qs[(t * Za + z) * 8 + c] = colLive && z < Nz ? v[t] : zero;
+                    qs[(tf * Za + z) * 8 + c] = colLive && z < Nz ? v[tf] : zero;
                 }
             }
         }
-    }
-    __syncthreads();
+    };
     // analysis: each family's matrix is streamed once for all its targets,
     // in register batches of kS steps loaded one batch ahead
     typename MF::acc acc[4];
@@ -1054,6 +1068,11 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     constexpr int kS = KS;  // the step count Za / 4 is a multiple of 2 kS
 #pragma unroll
     for (int fam = 0; fam < 2; ++fam) {
+        if (kSplit || fam == 0) {
+            if (fam > 0) __syncthreads();  // every wave has read the previous family's block
+            stage(kSplit ? fam : -1);
+            __syncthreads();
+        }
         const T* At = ana.m[fam + 2 * P] + static_cast<size_t>(jh) * tsteps * 64;  // this row tile
         const T* Aj = At + lane;
         // staged block read by this family: the columns themselves, or
@@ -1073,7 +1092,8 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
                     if ((projTarget<NT>(t) >= 2) == (fam == 1))
-                        b[u][t] = FOLD ? qw[2 * (((2 * t + par) * Za + zrow) * 8)] : qw[2 * ((t * Za + zrow) * 8)];
+                        b[u][t] = FOLD ? qw[2 * (((2 * (kSplit ? t & 1 : t) + par) * Za + zrow) * 8)]
+                                       : qw[2 * (((kSplit ? t & 1 : t) * Za + zrow) * 8)];
             }
 #pragma unroll
             for (int u = 0; u < kS; ++u)
@@ -1108,7 +1128,8 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
 #pragma unroll
             for (int t = 0; t < NT; ++t)
                 if ((projTarget<NT>(t) >= 2) == (fam == 1)) {
-                    const T sm = FOLD ? qw[2 * (((2 * t + par) * Za + zmid) * 8)] : qw[2 * ((t * Za + zmid) * 8)];
+                    const T sm = FOLD ? qw[2 * (((2 * (kSplit ? t & 1 : t) + par) * Za + zmid) * 8)]
+                                      : qw[2 * (((kSplit ? t & 1 : t) * Za + zmid) * 8)];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) acc[t][i] += am[i] * sm;
                 }
