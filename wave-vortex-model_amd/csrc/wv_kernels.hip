@@ -108,7 +108,9 @@ struct Epilogue {
     double scale;  // 0.5h (stages 1,2), h (stage 3), h/6 (stage 4)
     // RK modes: also store the field spectra of the state the stage produces
     // (the next stage's input; after stage 4 the new y) at elapsed time
-    // nextElapsed, the input of the next flux evaluation
+    // nextElapsed, the input of the next flux evaluation.  Stages 1-3 then
+    // do not store the stage state itself (st*): the next evaluation starts
+    // from these spectra and nothing else reads it.
     int emitFields;
     double nextElapsed;
     cx<double>* out0;
@@ -1328,13 +1330,13 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
             case kRk1:
                 ws[a][idx] = kv;
                 next[a] = axpy(ys[a][idx], kv, ep.scale);
-                sts[a][idx] = next[a];
+                if (!ep.emitFields) sts[a][idx] = next[a];
                 break;
             case kRk2:
             case kRk3:
                 ws[a][idx] = axpy(ws[a][idx], kv, 2.0);
                 next[a] = axpy(ys[a][idx], kv, ep.scale);
-                sts[a][idx] = next[a];
+                if (!ep.emitFields) sts[a][idx] = next[a];
                 break;
             default: {
                 const cx<double> w = axpy(ws[a][idx], kv, 1.0);
@@ -2136,13 +2138,17 @@ struct KernelImpl {
     void launchStage(double elapsed, const Epilogue& ep, hipStream_t s) {
         const int M = static_cast<int>(d.M);
         const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
-        // per coefficient: the NT analysed values, the projection tables
-        // (48 B) and the epilogue's coefficient arrays: flux out 48 B; RK
-        // stage 1 reads y, writes w and the stage (144 B); stages 2, 3 read
-        // w, y and write w, stage (192 B); stage 4 reads w, y and writes y
-        // (144 B); with the next input's spectra the field tables (96 B) and
-        // four spectra
-        const double epBytes = ep.mode == kFluxOut ? 48.0 : (ep.mode == kRk2 || ep.mode == kRk3) ? 192.0 : 144.0;
+        // per coefficient: the NT analysed values and the epilogue's
+        // coefficient arrays: flux out 48 B; RK stage 1 reads y, writes w
+        // (and the stage unless the next input's spectra are formed: 96 or
+        // 144 B); stages 2, 3 read w, y and write w (and the stage: 144 or
+        // 192 B); stage 4 reads w, y and writes y (144 B); with the next
+        // input's spectra four spectra
+        const double stBytes = ep.emitFields ? 0.0 : 48.0;
+        const double epBytes = ep.mode == kFluxOut ? 48.0
+                               : ep.mode == kRk1   ? 96.0 + stBytes
+                               : ep.mode == kRk4   ? 144.0
+                                                   : 144.0 + stBytes;
         const double fieldBytes = (ep.mode != kFluxOut && ep.emitFields) ? 4.0 * sizeof(cx<T>) : 0.0;
         const double bytes = static_cast<double>(M) * (NT * sizeof(cx<T>) + epBytes + fieldBytes);
         // the kernel reads y and w in every mode
