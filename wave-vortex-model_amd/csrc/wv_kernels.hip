@@ -154,16 +154,20 @@ struct FieldTables {
     const double* NA0;
 };
 
-// Evolved field spectra (u, v, w, eta) of coefficient idx
+// Phase p = exp(i omega elapsed) (phase(), WVCoefficientFormulas.hpp:15).
+__device__ __forceinline__ cx<double> phaseAt(double omega, double elapsed) {
+    double sn, cs;
+    sincos(omega * elapsed, &sn, &cs);
+    return cx<double>{cs, sn};
+}
+
+// Evolved field spectra (u, v, w, eta) of coefficient idx with phase p
 // (WVCoefficientFormulas.hpp:15-33), stored to FV[f*M + idx].
 template <typename T>
 __device__ __forceinline__ void storeFieldSpectra(cx<double> UAp, cx<double> VAp, cx<double> WAp, double NAp,
                                                   cx<double> UA0, cx<double> VA0, double NA0, int idx, int M,
-                                                  cx<double> Ap, cx<double> Am, cx<double> A0, double omega,
-                                                  double elapsed, cx<T>* __restrict__ FV) {
-    double sn, cs;
-    sincos(omega * elapsed, &sn, &cs);
-    const cx<double> p{cs, sn};
+                                                  cx<double> Ap, cx<double> Am, cx<double> A0, cx<double> p,
+                                                  cx<T>* __restrict__ FV) {
     const cx<double> ap = cmul(Ap, p), am = cmul(Am, cconj(p)), a0 = A0;
     const cx<double> u = cadd(cadd(cmul(UAp, ap), cmul(cconj(UAp), am)), cmul(UA0, a0));
     const cx<double> v = cadd(cadd(cmul(VAp, ap), cmul(cconj(VAp), am)), cmul(VA0, a0));
@@ -182,7 +186,7 @@ __global__ __launch_bounds__(256) void k_fields(const cx<double>* __restrict__ A
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= M) return;
     storeFieldSpectra(ft.UAp[idx], ft.VAp[idx], ft.WAp[idx], ft.NAp[idx], ft.UA0[idx], ft.VA0[idx], ft.NA0[idx], idx,
-                      M, Ap[idx], Am[idx], A0[idx], ft.omega[idx], elapsed, FV);
+                      M, Ap[idx], Am[idx], A0[idx], phaseAt(ft.omega[idx], elapsed), FV);
 }
 
 // F- or G-family field spectrum Apm*waveScale + A0*geostrophicScale per
@@ -1275,9 +1279,7 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
     cx<T> tv[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) tv[t] = Tg[t * static_cast<size_t>(M) + idx];
-    double sn, cs;
-    sincos(om * elapsed, &sn, &cs);
-    const cx<double> p{cs, sn};
+    const cx<double> p = phaseAt(om, elapsed);
     cx<double> Fp{0.0, 0.0}, Fm{0.0, 0.0}, F0{0.0, 0.0};
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -1346,9 +1348,13 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
             }
         }
     }
-    if (ep.mode != kFluxOut && ep.emitFields)
-        storeFieldSpectra(cr.UAp, cr.VAp, cr.WAp, cr.NAp, cr.UA0, cr.VA0, cr.NA0, idx, M, next[0], next[1], next[2], om,
-                          ep.nextElapsed, FV);
+    if (ep.mode != kFluxOut && ep.emitFields) {
+        // stages 2 and 4 produce the state of their own time: the same phase
+        cx<double> pn = p;
+        if (ep.nextElapsed != elapsed) pn = phaseAt(om, ep.nextElapsed);
+        storeFieldSpectra(cr.UAp, cr.VAp, cr.WAp, cr.NAp, cr.UA0, cr.VA0, cr.NA0, idx, M, next[0], next[1], next[2], pn,
+                          FV);
+    }
 }
 
 // r2c along x of nf real FP64 fields [Nx,Ny,Nz,nf] into P[c][z][y][kx < nkx]
