@@ -45,7 +45,9 @@ def _kernel(cfg, precision="fp64"):
 
 
 @pytest.mark.parametrize("N,Nz,Lx,hydro,nsteps", [(16, 9, 50e3, False, 6), (16, 9, 50e3, True, 6),
-                                                  (32, 17, 50e3, False, 4)])
+                                                  (32, 17, 50e3, False, 4),
+                                                  # Nx = 256: the x kernel's 256-point forward transforms
+                                                  (256, 9, 200e3, False, 2), (256, 9, 200e3, True, 2)])
 def test_rk4_fp64_per_step_matches_oracle_small(torch_gpu, N, Nz, Lx, hydro, nsteps):
     cfg = _cfg(N, Nz, Lx, hydro)
     k = _kernel(cfg)

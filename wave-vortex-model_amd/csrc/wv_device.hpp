@@ -294,6 +294,59 @@ __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& b
     for (; l2s < log2N; l2s += 4) stockhamPass<T, 16, SIGN>(buf, log2N, l2s, bt, tw);
 }
 
+// Forward (SIGN = -1) FFT of NB 256-point transforms (unit stride, padded,
+// transform b at bt.base(b)) in the two radix-16 Stockham passes of fftLds
+// (same twiddle table tw), on the first 32 NB threads, two per butterfly.  With x_r (r < 16) the twiddled inputs of a butterfly
+// and W = exp(-2 pi i / 16), its outputs 2p + s (s = 0, 1) are the 8-point DFT
+// over r < 8 of
+//   u_r = (x_r + (-1)^s x_{r+8}) W^{r s},
+// so lane s of a pair forms u from all 16 inputs and runs an 8-point DFT:
+// every lane busy, and fewer instructions per lane than one radix-16
+// butterfly.  The caller synchronizes before the call; the function returns
+// after a final barrier.
+template <typename T, int NB, int NB1>
+__device__ __forceinline__ void fftFwd256(cx<T>* buf, const BatchRows<NB1>& bt, const cx<T>* __restrict__ tw) {
+    static_assert(32 * NB <= kFftThreads, "two threads per butterfly");
+    const int t = fftThread();
+    const bool active = t < 32 * NB;
+    const int s = t & 1, jj = (t >> 1) & 15;
+    const T sg = s ? T(-1) : T(1);
+    cx<T>* z = buf + bt.base(active ? t >> 5 : 0);
+    // W^{r s}, r = 1..7
+    cx<T> c[8];
+#pragma unroll
+    for (int r = 1; r < 8; ++r) c[r] = s ? cx<T>{T(w16cos(r)), T(w16sin(r))} : cx<T>{T(1), T(0)};
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        cx<T> u[8];
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                cx<T> x0 = z[BatchRows<1>::at(jj + 16 * r)];
+                cx<T> x1 = z[BatchRows<1>::at(jj + 16 * (r + 8))];
+                if (pass == 1) {
+                    // pass twiddles exp(-2 pi i r k / 256), k = jj (the table
+                    // holds the unit twiddle at k = 0 too)
+                    if (r > 0) x0 = cmul(x0, tw[(r << 4) + jj - 1]);
+                    x1 = cmul(x1, tw[((r + 8) << 4) + jj - 1]);
+                }
+                const cx<T> v{x0.re + sg * x1.re, x0.im + sg * x1.im};
+                u[r] = r == 0 ? v : cmul(v, c[r]);
+            }
+            dftReg<T, 8, -1>(u);
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                const int q = 2 * p + s;
+                z[BatchRows<1>::at(pass == 0 ? 16 * jj + q : jj + 16 * q)] = u[p];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Copies the N-entry twiddle table into LDS (block-wide; caller synchronizes).
 template <typename T>
 __device__ __forceinline__ void loadTwiddles(cx<T>* dst, const cx<T>* __restrict__ src, int N) {

@@ -912,8 +912,12 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
         rb[NxP + xl] = {F[2], F[3] * etaScale};
     }
     __syncthreads();
-    // forward transforms: pairs 0 and 1 of every row
-    fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
+    // forward transforms: pairs 0 and 1 of every row (Nx = 256: two threads
+    // per radix-16 butterfly)
+    if constexpr (L2N == 8)
+        fftFwd256<T, 2 * RB>(buf, BatchRows<2>{2 * RB, NxP, RS}, tw);
+    else
+        fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
     // wave w unpacks (row, pair) rq = w, w + nwaves, ...; its lanes walk k
     constexpr int nwaves = kFftThreads / 64;
     for (int rq = waveIndex(); rq < RB * 2; rq += nwaves)
