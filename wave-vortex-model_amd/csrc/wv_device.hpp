@@ -136,7 +136,9 @@ __device__ __forceinline__ double w16sin(int q) { return -w16cos((q + 12) & 15);
 
 // In-register DFT of R = 2, 4, 8 or 16 points, natural order in and out
 // (radix-2 decimation in frequency, then a compile-time bit-reversal).
-template <typename T, int R, int SIGN>
+// Bit r of ZM marks input r as known zero: the first stage then copies or
+// negates its partner instead of adding.
+template <typename T, int R, int SIGN, unsigned ZM = 0>
 __device__ __forceinline__ void dftReg(cx<T>* v) {
 #pragma unroll
     for (int half = R / 2; half >= 1; half /= 2) {
@@ -145,8 +147,10 @@ __device__ __forceinline__ void dftReg(cx<T>* v) {
 #pragma unroll
             for (int q = 0; q < half; ++q) {
                 const cx<T> a = v[s + q], b = v[s + q + half];
-                v[s + q] = cadd(a, b);
-                const cx<T> d = csub(a, b);
+                const bool za = half == R / 2 && ((ZM >> (s + q)) & 1u);
+                const bool zb = half == R / 2 && ((ZM >> (s + q + half)) & 1u);
+                v[s + q] = za ? b : zb ? a : cadd(a, b);
+                const cx<T> d = za ? cx<T>{-b.re, -b.im} : zb ? a : csub(a, b);
                 const int qi = q * (16 / (2 * half));  // exp(sign 2 pi i q/(2 half)) = W16^(qi)
                 if (qi == 0) {
                     v[s + q + half] = d;
@@ -176,8 +180,9 @@ template <int R>
 constexpr int log2Radix() { return R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4; }
 
 // One Stockham pass of radix R over transforms of length 2^log2N whose
-// sub-transforms so far have length 2^log2Ns.
-template <typename T, int R, int SIGN, typename Bt>
+// sub-transforms so far have length 2^log2Ns.  Bit r of ZM marks input r of
+// every butterfly as known zero (not read).
+template <typename T, int R, int SIGN, typename Bt, unsigned ZM = 0>
 __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, const Bt& bt,
                                              const cx<T>* __restrict__ tw) {
     constexpr int IPT = kFftRegs / R;
@@ -197,6 +202,10 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
             const int boff = bt.base(b);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
+                if ((ZM >> r) & 1u) {
+                    v[it][r] = cx<T>{T(0), T(0)};
+                    continue;
+                }
                 cx<T> x = buf[bt.at(jj + r * NR) + boff];
                 if (r > 0 && log2Ns > 0) {
                     // the table holds the unit twiddle at k = 0 too
@@ -206,7 +215,7 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
                 }
                 v[it][r] = x;
             }
-            dftReg<T, R, SIGN>(v[it]);
+            dftReg<T, R, SIGN, ZM>(v[it]);
         }
     }
     __syncthreads();

@@ -805,7 +805,11 @@ struct XMap<3> {
 // the first 16 threads per transform (one wave for four transforms): fewer
 // instructions, passes and barriers than radix-4 passes spread over all
 // threads.
-template <typename T, int NT, int RB, int L2N>
+// PRUNE (Nx = 256, nkx <= 96): entries 96 ... 159 of every inverse
+// transform lie in the zero band, so inputs r = 6 ... 9 of every butterfly
+// of the first radix-16 pass (entries jj + 16 r) are known zeros: neither
+// written nor read.
+template <typename T, int NT, int RB, int L2N, bool PRUNE>
 __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P,
                                                         int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
                                                         const cx<T>* __restrict__ twX, const T* __restrict__ kxv, const cx<T>* __restrict__ zeros,
@@ -883,12 +887,25 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
             }
         }
     }
+    static_assert(!PRUNE || L2N == 8, "pruned first pass for Nx = 256");
     for (int t = wave; t < RB * PAIRS; t += 4) {
         cx<T>* z = buf + (t / PAIRS) * RS + (t % PAIRS) * NxP;
-        for (int k = nkx + lane; k <= Nx - nkx; k += 64) z[ldsIndex(k, 1)] = cx<T>{T(0), T(0)};
+        if constexpr (PRUNE) {
+            // the band entries that the first pass reads: nkx ... 95, 160 ... Nx - nkx
+            const int lo = 96 - nkx;
+            for (int e = lane; e < lo + (Nx - nkx - 159); e += 64) z[ldsIndex(e < lo ? nkx + e : 160 + e - lo, 1)] = cx<T>{T(0), T(0)};
+        } else {
+            for (int k = nkx + lane; k <= Nx - nkx; k += 64) z[ldsIndex(k, 1)] = cx<T>{T(0), T(0)};
+        }
     }
     __syncthreads();
-    fftLds<T, +1, L2N>(buf, Nx, log2Nx, BatchRows<PAIRS>{RB * PAIRS, NxP, RS}, tw);
+    if constexpr (PRUNE) {
+        const BatchRows<PAIRS> bt{RB * PAIRS, NxP, RS};
+        stockhamPass<T, 16, +1, BatchRows<PAIRS>, 0x3C0u>(buf, 8, 0, bt, tw);
+        stockhamPass<T, 16, +1>(buf, 8, 4, bt, tw);
+    } else {
+        fftLds<T, +1, L2N>(buf, Nx, log2Nx, BatchRows<PAIRS>{RB * PAIRS, NxP, RS}, tw);
+    }
     for (int e = fftThread(); e < RB * Nx; e += kFftThreads) {
         int rr, x;
         divmodN(e, Nx, log2Nx, rr, x);
@@ -2070,7 +2087,10 @@ struct KernelImpl {
         if (lds > maxLdsPerBlock) throw Failure{WV_UNSUPPORTED_OPERATION, "x transforms exceed the LDS of a workgroup."};
         // power of two nearest N0 (exact scaling of the eta product)
         const T es = static_cast<T>(std::ldexp(1.0, static_cast<int>(std::lround(std::log2(d.c.N0 > 0 ? d.c.N0 : 1.0)))));
-        hipLaunchKernelGGL((k_xfused<T, NTc, RB, L2N>), dim3(static_cast<unsigned>((nrows + RB - 1) / RB)),
+        auto fn = k_xfused<T, NTc, RB, L2N, false>;
+        if constexpr (L2N == 8)
+            if (d.nkx <= 96) fn = k_xfused<T, NTc, RB, L2N, true>;
+        hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>((nrows + RB - 1) / RB)),
                            dim3(kFftThreads), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, log2Exact(d.c.Nx), Ny, Nz,
                            static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), kxv.as<T>(),
                            zeroRow.as<cx<T>>(), es);
