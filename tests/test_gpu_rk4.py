@@ -67,6 +67,27 @@ def test_rk4_fp64_per_step_matches_oracle_small(torch_gpu, N, Nz, Lx, hydro, nst
         assert err <= 1e-12
 
 
+def test_rk4_fp64_256x256x9_no_antialias_per_step(torch_gpu):
+    """Nx = 256 without the 2/3 rule (nkx = 129): the x kernel's first inverse
+    pass then reads the whole spectrum (no zero band)."""
+    import wvamd
+    cfg = wvamd.make_config(256, 256, 9, Lx=200e3, shouldAntialias=False)
+    k = _kernel(cfg)
+    o = Oracle(cfg)
+    Oracle.set_threads(1)
+    A = _flow(k, seed=29, jmax=k.Nj)
+    h = _dt(k, A)
+    dev = [k.to_device(a) for a in A]
+    host = [a.copy() for a in A]
+    t = 0.0
+    for step in range(2):
+        k.rk4_advance(*dev, t=t, h=h, nsteps=1)
+        *host, t = o.rk4(*host, t=t, dt=h, nsteps=1)
+        err = rel_err_joint([d.cpu().numpy() for d in dev], host)
+        print(f"[gpu] 256x256x9 no antialias step {step} rel err {err:.2e}", flush=True)
+        assert err <= 1e-12
+
+
 def test_rk4_fp64_64x64x33_50_steps_per_step(torch_gpu):
     """The 64x64x33 configuration (Lx=Ly=50 km, Lz=1300 m, N0=5.2e-3, lat 33,
     seeded flow j<=16, rms u 5 cm/s, dt = 0.5 CFL), 50 steps, every step."""

@@ -801,10 +801,10 @@ struct XMap<3> {
 // per complex FFT), the NT advection products and r2c of the products.
 // kxv[k] = the x wavenumber of half-spectrum column k.  L2N > 0 fixes
 // Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored).
-// The forward transforms run the same radix-16 passes as the inverse ones, on
-// the first 16 threads per transform (one wave for four transforms): fewer
-// instructions, passes and barriers than radix-4 passes spread over all
-// threads.
+// The forward transforms run the same radix-16 passes as the inverse ones:
+// at Nx = 256 with two threads per butterfly (fftFwd256, one full wave for
+// the two transforms of a row), at other lengths on the first 16 threads per
+// transform.
 // PRUNE (Nx = 256, nkx <= 96): entries 96 ... 159 of every inverse
 // transform lie in the zero band, so inputs r = 6 ... 9 of every butterfly
 // of the first radix-16 pass (entries jj + 16 r) are known zeros: neither
