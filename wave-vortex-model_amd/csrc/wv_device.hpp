@@ -317,14 +317,20 @@ template <typename T, int NB, int NB1>
 __device__ __forceinline__ void fftFwd256(cx<T>* buf, const BatchRows<NB1>& bt, const cx<T>* __restrict__ tw) {
     static_assert(32 * NB <= kFftThreads, "two threads per butterfly");
     const int t = fftThread();
-    const bool active = t < 32 * NB;
-    const int s = t & 1, jj = (t >> 1) & 15;
-    const T sg = s ? T(-1) : T(1);
-    cx<T>* z = buf + bt.base(active ? t >> 5 : 0);
-    // W^{r s}, r = 1..7
-    cx<T> c[8];
+    const bool active = t < 32 * NB;  // whole waves when NB is even
+    // lane set-up inside the active branch: the idle waves skip it
+    int s = 0, jj = 0;
+    T sg = T(1);
+    cx<T>* z = buf;
+    cx<T> c[8];  // W^{r s}, r = 1..7
+    if (active) {
+        s = t & 1;
+        jj = (t >> 1) & 15;
+        sg = s ? T(-1) : T(1);
+        z = buf + bt.base(t >> 5);
 #pragma unroll
-    for (int r = 1; r < 8; ++r) c[r] = s ? cx<T>{T(w16cos(r)), T(w16sin(r))} : cx<T>{T(1), T(0)};
+        for (int r = 1; r < 8; ++r) c[r] = s ? cx<T>{T(w16cos(r)), T(w16sin(r))} : cx<T>{T(1), T(0)};
+    }
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
         cx<T> u[8];
