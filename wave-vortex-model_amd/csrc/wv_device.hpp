@@ -305,9 +305,10 @@ __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& b
 
 // Forward (SIGN = -1) FFT of NB 256-point transforms (unit stride, padded,
 // transform b at bt.base(b)) in the two radix-16 Stockham passes of fftLds
-// (same twiddle table tw), on the first 32 NB threads, two per butterfly.  With x_r (r < 16) the twiddled inputs of a butterfly
-// and W = exp(-2 pi i / 16), its outputs 2p + s (s = 0, 1) are the 8-point DFT
-// over r < 8 of
+// (same twiddle table tw), on the first 32 NB threads, two per butterfly.
+// With x_r (r < 16) the twiddled inputs of a butterfly and W =
+// exp(-2 pi i / 16), its outputs 2p + s (s = 0, 1) are the 8-point DFT over
+// r < 8 of
 //   u_r = (x_r + (-1)^s x_{r+8}) W^{r s},
 // so lane s of a pair forms u from all 16 inputs and runs an 8-point DFT:
 // every lane busy, and fewer instructions per lane than one radix-16
