@@ -32,9 +32,38 @@ METRIC = "RK4 steps/s at 256×256×129: mixed vs FP64 GPU at ≤1e-6 rel energy 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_rk4_256x256x129_mixed.md")
-# stage name -> kernel of the mixed-precision RHS
-STAGE_KERNEL = {"fields": "k_fields<float", "synth": "k_synth<float", "xfused": "k_xfused<float",
-                "proj": "k_proj<float", "stage": "k_stage<float"}
+# per-dispatch SQ counters (MFMA instructions among them) of the same kernels
+COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "sq_counters_rk4_256x256x129_mixed.md")
+MFMA_PEAK_TFLOPS_F32 = 157.3
+MFMA_FLOPS_F32_16X16X4 = 2 * 16 * 16 * 4  # the only MFMA shape of the mixed RHS
+# stage name -> kernel of the mixed-precision RHS at 256x256x129
+STAGE_KERNEL = {"fields": "k_fields<float", "synth": "k_synth<float", "yfft_inv": "k_yfft<float, 8, 4, true",
+                "xfused": "k_xfused<float", "yfft_fwd": "k_yfft<float, 8, 4, false", "proj": "k_proj<float",
+                "stage": "k_stage<float"}
+
+
+def per_kernel_rooflines(rec):
+    """Every kernel of the mixed RHS against the chip's peaks: algorithmic
+    bytes per launch over its HIP-event time against HBM, its measured HBM
+    bytes per dispatch, and for the MFMA kernels the FP32 MFMA rate (MFMA
+    instructions per dispatch from the counter summary)."""
+    from wvamd.rocprof_summary import hbm_traffic, kernel_counters
+    out = []
+    for r in rec:
+        avg_ms = r["total_ms"] / max(r["launches"], 1)
+        e = {"kernel": r["name"], "avg_ms": round(avg_ms, 4), "algorithmic_MB": round(r["bytes_per_launch"] / 1e6, 1),
+             "hbm_GBs": round(r["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9, 1)}
+        e["hbm_frac"] = round(e["hbm_GBs"] / HBM_PEAK_GBS, 4)
+        prefix = STAGE_KERNEL.get(r["name"], "?")
+        t = hbm_traffic(TRAFFIC_SUMMARY, prefix)
+        e["traffic_MB"] = round(t["traffic_bytes"] / 1e6, 1) if t else None
+        c = kernel_counters(COUNTER_SUMMARY, prefix)
+        if c and c.get("SQ_INSTS_MFMA"):
+            tflops = c["SQ_INSTS_MFMA"] * MFMA_FLOPS_F32_16X16X4 / (avg_ms * 1e-3) / 1e12
+            e["mfma_TFLOPs"] = round(tflops, 1)
+            e["mfma_frac"] = round(tflops / MFMA_PEAK_TFLOPS_F32, 4)
+        out.append(e)
+    return out
 
 
 def step_model_bytes(Nx, Ny, Nz, M, b_spec, b_real):
@@ -211,6 +240,7 @@ def main():
                   "fp64_bytes_per_step": b64, "fp64_GBs": round(b64 * sps64 / 1e9, 1),
                   "fp64_frac": round(b64 * sps64 / 1e9 / HBM_PEAK_GBS, 4)}
     stages = {r["name"]: round(r["total_ms"] / max(r["launches"], 1), 4) for r in rec}
+    kernels = per_kernel_rooflines(rec)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -234,7 +264,7 @@ def main():
         "roofline": roofline, "reference_dataflow_roofline": step_model,
         "host_memory_flux_per_s": {"mixed": round(host32, 2), "fp64": round(host64, 2),
                                    "note": "PCIe-inclusive: state copied in and flux copied out per call"},
-        "stage_ms_per_launch_mixed": stages, "cpu_baseline": cpu,
+        "stage_ms_per_launch_mixed": stages, "kernels_mixed": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

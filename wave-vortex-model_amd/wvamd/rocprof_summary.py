@@ -3,6 +3,7 @@ library, and read the HBM-traffic figures back.
 
   python -m wvamd.rocprof_summary trace STATS_CSV OUT_MD --title T --command C
   python -m wvamd.rocprof_summary pmc FETCH_CSV WRITE_CSV OUT_MD --title T --command C
+  python -m wvamd.rocprof_summary counters COUNTER_CSV OUT_MD --title T --command C
 
 `trace` tabulates a --kernel-trace --stats run (calls, average and total device
 time per kernel).  `pmc` tabulates FETCH_SIZE and WRITE_SIZE (KiB per
@@ -10,7 +11,9 @@ dispatch, one --pmc pass each) per kernel and scales them by what the same
 counters read for the calibration kernel: an elementwise multiply of
 64 Mi doubles (torch.mul(a, 2.0, out=b)) that reads and writes exactly
 536870912 bytes.  The scaled per-dispatch bytes are also written as a JSON
-block, which `hbm_traffic()` reads.
+block, which `hbm_traffic()` reads.  `counters` tabulates the mean per
+dispatch of every counter of one --pmc pass per kernel (JSON block read by
+`kernel_counters()`).
 """
 import argparse
 import csv
@@ -102,9 +105,28 @@ def write_pmc(fetch_csv, write_csv, out_md, title, command):
     open(out_md, "w").write("\n".join(lines) + "\n")
 
 
-def hbm_traffic(md_path, kernel_prefix):
-    """Scaled HBM bytes per dispatch of the kernels whose short name starts with
-    `kernel_prefix` in a summary written by write_pmc; None if absent or ambiguous."""
+def write_counters(counter_csv, out_md, title, command):
+    acc = defaultdict(lambda: defaultdict(list))
+    for r in csv.DictReader(open(counter_csv)):
+        if _is_calibration(r["Kernel_Name"]):
+            continue
+        acc[(short_name(r["Kernel_Name"]), int(r["Grid_Size"]))][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = sorted({c for d in acc.values() for c in d})
+    lines = [f"# {title}", "", "Command (from the repository root on the GPU host):", "", "```", command, "```", "",
+             "Mean per dispatch of each counter, summed over the device.", "",
+             "| kernel | grid | " + " | ".join(names) + " |", "|---|---:|" + "---:|" * len(names)]
+    table = OrderedDict()
+    for key, d in acc.items():
+        means = {c: sum(v) / len(v) for c, v in d.items()}
+        table[f"{key[0]}|{key[1]}"] = means
+        lines.append(f"| `{key[0]}` | {key[1]} | " + " | ".join(f"{means.get(c, 0.0):.0f}" for c in names) + " |")
+    lines += ["", "```json", json.dumps({"kernels": table}, indent=1), "```"]
+    open(out_md, "w").write("\n".join(lines) + "\n")
+
+
+def _lookup(md_path, kernel_prefix):
+    """The JSON entry of the one kernel whose short name starts with
+    `kernel_prefix` in a summary written here; None if absent or ambiguous."""
     try:
         text = open(md_path).read()
     except OSError:
@@ -119,6 +141,18 @@ def hbm_traffic(md_path, kernel_prefix):
     return hits[0]
 
 
+def hbm_traffic(md_path, kernel_prefix):
+    """Scaled HBM bytes per dispatch (read, write, traffic) of a kernel in a
+    summary written by write_pmc."""
+    return _lookup(md_path, kernel_prefix)
+
+
+def kernel_counters(md_path, kernel_prefix):
+    """Mean counter values per dispatch of a kernel in a summary written by
+    write_counters."""
+    return _lookup(md_path, kernel_prefix)
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -129,14 +163,19 @@ def main():
     p.add_argument("fetch_csv")
     p.add_argument("write_csv")
     p.add_argument("out_md")
-    for s in (t, p):
+    c = sub.add_parser("counters")
+    c.add_argument("counter_csv")
+    c.add_argument("out_md")
+    for s in (t, p, c):
         s.add_argument("--title", required=True)
         s.add_argument("--command", required=True)
     a = ap.parse_args()
     if a.mode == "trace":
         write_trace(a.stats_csv, a.out_md, a.title, a.command)
-    else:
+    elif a.mode == "pmc":
         write_pmc(a.fetch_csv, a.write_csv, a.out_md, a.title, a.command)
+    else:
+        write_counters(a.counter_csv, a.out_md, a.title, a.command)
 
 
 if __name__ == "__main__":
