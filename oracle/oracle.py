@@ -101,6 +101,9 @@ class _Lib:
                 getattr(lib, p + fn).argtypes = [C.c_void_p, _P, C.c_double, C.c_double, _P, _P, _P]
             getattr(lib, p + "fg_all_derivatives").restype = C.c_int
             getattr(lib, p + "fg_all_derivatives").argtypes = [C.c_void_p, C.c_int, _P, _P, _P]
+            if hasattr(lib, p + "advect_f_grid_scalar"):
+                getattr(lib, p + "advect_f_grid_scalar").restype = C.c_int
+                getattr(lib, p + "advect_f_grid_scalar").argtypes = [C.c_void_p, _P, _P, C.c_int, _P]
             if p == "wvo_":
                 lib.wvo_rk4_advance.restype = C.c_int
                 lib.wvo_rk4_advance.argtypes = [C.c_void_p, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int]
@@ -191,6 +194,18 @@ class _Lib:
         if rc:
             raise RuntimeError(f"forward failed: {rc}")
         return Ap, Am, A0
+
+    def advect_f_grid_scalar(self, scalar, uvw, antialias):
+        """advectFGridScalar (...Kernel.cpp:1032-1138); arrays [Nz, Ny, Nx]
+        (the reference's [Nx, Ny, Nz], x fastest) and [3, Nz, Ny, Nx]."""
+        Nx, Ny, Nz = self.spatial
+        scalar = np.ascontiguousarray(scalar, dtype=np.float64).reshape(Nz, Ny, Nx)
+        uvw = np.ascontiguousarray(uvw, dtype=np.float64).reshape(3, Nz, Ny, Nx)
+        out = np.zeros((Nz, Ny, Nx))
+        rc = self._fn("advect_f_grid_scalar")(self.h, _ptr(scalar), _ptr(uvw), int(bool(antialias)), _ptr(out))
+        if rc:
+            raise RuntimeError(f"advect_f_grid_scalar failed: {rc}")
+        return out
 
     def fg_all(self, f_family, Apm, A0):
         Apm, A0 = self._coef(Apm), self._coef(A0)

@@ -160,4 +160,15 @@ int wvr_fg_all_derivatives(void* h, int fFamily, const double* Apm, const double
     return static_cast<int>(st.code);
 }
 
+// advectFGridScalar (...Kernel.cpp:1032-1138): rhs = -(u, v, w) . grad(scalar)
+// of an F-grid scalar, optionally antialiased.
+int wvr_advect_f_grid_scalar(void* h, const double* scalar, const double* uvw, int antialias, double* rhs) {
+    auto& k = *static_cast<Handle*>(h)->kernel;
+    const auto sp = k.descriptor().spatialShape();
+    WVRealVolumeConstView in{scalar, sp};
+    WVRealFieldBundleConstView adv{uvw, {sp.first, sp.second, sp.third, 3}};
+    WVRealVolumeView out{rhs, sp};
+    return static_cast<int>(k.advectFGridScalar(in, adv, antialias != 0, out).code);
+}
+
 }  // extern "C"

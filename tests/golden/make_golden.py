@@ -148,12 +148,33 @@ def c0_flux():
                                         Fp=Fp, Fm=Fm, F0=F0, seed=np.array(4001), jmax=np.array(16)))
 
 
+# advectFGridScalar fixtures: even and odd grids, with and without the 2/3 rule
+SCALAR = [("s16x16x9_nh", 16, 16, 9, False, True), ("s15x13x9_h", 15, 13, 9, True, True),
+          ("s16x12x9_nh_noaa", 16, 12, 9, False, False)]
+
+
+def scalar_cases():
+    for name, Nx, Ny, Nz, hydro, aa in SCALAR:
+        cfg = make_config(Nx, Ny, Nz, Lx=50e3, Ly=40e3, isHydrostatic=hydro, shouldAntialias=aa)
+        r = Reference(cfg)
+        rng = np.random.default_rng(7000 + SCALAR.index((name, Nx, Ny, Nz, hydro, aa)))
+        scalar = rng.standard_normal((Nz, Ny, Nx))
+        uvw = rng.standard_normal((3, Nz, Ny, Nx)) * 0.1
+        rhs = r.advect_f_grid_scalar(scalar, uvw, False)
+        rhs_aa = r.advect_f_grid_scalar(scalar, uvw, True)
+        write("scalar_" + name, dict(cfg_arrays(cfg), scalar=scalar, uvw=uvw, rhs=rhs, rhs_antialiased=rhs_aa))
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--c0", action="store_true", help="also write the 64x64x33 reference flux (slow)")
+    ap.add_argument("--scalar", action="store_true", help="write only the advectFGridScalar fixtures")
     args = ap.parse_args()
     if not Reference.available():
         sys.exit("oracle/_ref/libwvref.so is missing: run `make -C oracle ref` where the reference tree exists")
+    if args.scalar:
+        scalar_cases()
+        sys.exit(0)
     tiny_cases()
     descriptor_cases()
     if args.c0:

@@ -980,6 +980,74 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
     }
 }
 
+// advectFGridScalar (...Kernel.cpp:1032-1138), FP64: the derivative assembly
+// of the scalar's half spectrum C[0][z][l][k] (k < nxh = Nx/2 + 1, row
+// stride nxhP), for a workgroup's 8 columns k0 ... k0 + 7 of row l.  With
+// v = scale * C[0] (scale = 1/(Nx Ny)): C[0] = i k v (k = 0 at the x Nyquist
+// column), C[1] = i l v, C[2] = sum_w D[z][w] v_w, D the column operator of
+// the reference's DCT-I, -m_j, DST-I sequence (:1078-1094; rows z = 0 and
+// Nz - 1 zero); imaginary parts cleared on the self-conjugate rows
+// (:1095-1100).  LDS: the 8 columns over z.
+constexpr double kScalarPi = 3.14159265358979323846;
+__global__ __launch_bounds__(256) void k_scalarDeriv(cx<double>* __restrict__ C, int Nx, int Ny, int Nz, int nxh,
+                                                     int nxhP, double Lx, double Ly, const double* __restrict__ D,
+                                                     double scale) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<double>* col = reinterpret_cast<cx<double>*>(smem);  // [Nz][8]
+    const int k0 = static_cast<int>(blockIdx.x) * 8, l = static_cast<int>(blockIdx.y);
+    const size_t plane = static_cast<size_t>(Ny) * nxhP;
+    const size_t chStride = static_cast<size_t>(Nz) * plane;
+    const size_t base = static_cast<size_t>(l) * nxhP + k0;
+    for (int e = threadIdx.x; e < Nz * 8; e += blockDim.x) {
+        const int z = e >> 3, c = e & 7;
+        const cx<double> v = k0 + c < nxhP ? C[z * plane + base + c] : cx<double>{0.0, 0.0};
+        col[e] = cx<double>{v.re * scale, v.im * scale};
+    }
+    __syncthreads();
+    const int lMode = l < (Ny + 1) / 2 ? l : l - Ny;
+    const double lw = 2.0 * kScalarPi * static_cast<double>(lMode) / Ly;
+    const bool lSelf = l == 0 || (Ny % 2 == 0 && l == Ny / 2);
+    for (int e = threadIdx.x; e < Nz * 8; e += blockDim.x) {
+        const int z = e >> 3, c = e & 7, k = k0 + c;
+        if (k >= nxh) continue;
+        const bool nyq = Nx % 2 == 0 && k == Nx / 2;
+        const double kw = nyq ? 0.0 : 2.0 * kScalarPi * static_cast<double>(k) / Lx;
+        const cx<double> v = col[e];
+        cx<double> dx{-v.im * kw, v.re * kw}, dy{-v.im * lw, v.re * lw}, dz{0.0, 0.0};
+        const double* Dz = D + static_cast<size_t>(z) * Nz;
+        for (int w = 0; w < Nz; ++w) {
+            const double a = Dz[w];
+            const cx<double> u = col[w * 8 + c];
+            dz.re += a * u.re;
+            dz.im += a * u.im;
+        }
+        if (lSelf && (k == 0 || nyq)) dx.im = dy.im = dz.im = 0.0;
+        const size_t o = z * plane + base + c;
+        C[o] = dx;
+        C[chStride + o] = dy;
+        C[2 * chStride + o] = dz;
+    }
+}
+
+// rhs = -(u dx + v dy + w dz) over the R points (advectFGridScalar :1124-1128):
+// adv and d3 are [3][R].
+__global__ __launch_bounds__(256) void k_scalarProduct(const double* __restrict__ adv, const double* __restrict__ d3,
+                                                       double* __restrict__ rhs, size_t R) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    rhs[i] = -(adv[i] * d3[i] + adv[R + i] * d3[R + i] + adv[2 * R + i] * d3[2 * R + i]);
+}
+
+// The scalar antialias's row mask and scaling (:1018-1022): C[z][l][k] times
+// keep[l][k] (1/(Nx Ny) on kept rows, 0 elsewhere).
+__global__ __launch_bounds__(256) void k_scalarMask(cx<double>* __restrict__ C, const double* __restrict__ keep,
+                                                    size_t plane, size_t n) {
+    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double w = keep[i % plane];
+    C[i] = cx<double>{C[i].re * w, C[i].im * w};
+}
+
 // Projection targets of the NT transformed products: u, v, w, eta
 // (nonhydrostatic) or u, v, eta (hydrostatic).
 template <int NT>
@@ -2219,6 +2287,114 @@ struct KernelImpl {
         launchStage<T>(elapsed, ep, s);
     }
 
+    // ---- advectFGridScalar (FP64 in either precision, as the reference's)
+    // Scratch, allocated on the first call and kept: three half spectra
+    // [3][Nz][Ny][nxhP] (nxh = Nx/2 + 1 columns padded to the y tile), three
+    // fields [3][R], the d/dz column operator, the antialias row weights and
+    // FP64 twiddles.
+    DeviceBuffer scC, scR, scD, scKeep, scTwX, scTwY;
+    int scNxh = 0, scNxhP = 0;
+
+    // d/dz of an F-grid column (...Kernel.cpp:1078-1094) as one matrix: the
+    // DCT-I (REDFT00) over z divided by L = Nz - 1 (last coefficient halved),
+    // coefficients j = 1 ... L-1 times -pi j / Lz (j = 0 and L dropped), then
+    // halved and synthesized by the DST-I (RODFT00) of z = 1 ... L-1:
+    //   D[z][w] = sum_j -m_j sin(pi j z / L) c_w cos(pi j w / L) / L,
+    // c_w = 1 at w = 0 and L, 2 inside; rows z = 0 and L are zero.
+    static std::vector<double> scalarDzOperator(size_t Nz, double Lz) {
+        const size_t L = Nz - 1;
+        std::vector<double> D(Nz * Nz, 0.0);
+        if (L < 2) return D;
+        auto trig = [L](size_t p, bool sine) {
+            const double a = kScalarPi * static_cast<double>(p % (2 * L)) / static_cast<double>(L);
+            return sine ? std::sin(a) : std::cos(a);
+        };
+        for (size_t z = 1; z < L; ++z)
+            for (size_t w = 0; w <= L; ++w) {
+                const double cw = (w == 0 || w == L) ? 1.0 : 2.0;
+                double acc = 0.0;
+                for (size_t j = 1; j < L; ++j) {
+                    const double mj = kScalarPi * static_cast<double>(j) / Lz;
+                    acc += -mj * trig(j * z, true) * cw * trig(j * w, false);
+                }
+                D[z * Nz + w] = acc / static_cast<double>(L);
+            }
+        return D;
+    }
+
+    void ensureScalarScratch() {
+        if (scD.p) return;
+        const size_t Nx = d.c.Nx, Ny = d.c.Ny, Nz = d.c.Nz;
+        scNxh = static_cast<int>(Nx / 2 + 1);
+        scNxhP = (scNxh + KT - 1) / KT * KT;
+        const size_t yLds = (static_cast<size_t>(KT) + 1) * Ny * sizeof(cx<double>);
+        const size_t xLds = 2 * static_cast<size_t>(ldsLength(static_cast<int>(Nx))) * sizeof(cx<double>);
+        const size_t zLds = Nz * 8 * sizeof(cx<double>);
+        if (yLds > maxLdsPerBlock || xLds > maxLdsPerBlock || zLds > maxLdsPerBlock)
+            throw Failure{WV_UNSUPPORTED_OPERATION, "Scalar advection needs more LDS per workgroup than the device provides."};
+        const size_t half = Nz * Ny * static_cast<size_t>(scNxhP);
+        scC.alloc(3 * half * sizeof(cx<double>));
+        scR.alloc(3 * d.R * sizeof(double));
+        upload(scD, scalarDzOperator(Nz, d.c.Lz));
+        // the rows that store or complete a wave-vortex mode (:467-471)
+        std::vector<double> keep(Ny * static_cast<size_t>(scNxhP), 0.0);
+        const double scale = 1.0 / static_cast<double>(Nx * Ny);
+        for (const Column& c : d.columns) keep[static_cast<size_t>(c.iy) * scNxhP + static_cast<size_t>(c.kx)] = scale;
+        upload(scKeep, keep);
+        upload(scTwX, d.twX);
+        upload(scTwY, d.twY);
+    }
+
+    void advectScalar(const double* scalar, const double* adv, bool antialias, double* rhs, hipStream_t s) {
+        ensureScalarScratch();
+        const int Nx = static_cast<int>(d.c.Nx), Ny = static_cast<int>(d.c.Ny), Nz = static_cast<int>(d.c.Nz);
+        const int l2x = log2Exact(d.c.Nx), l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
+        cx<double>* C = scC.as<cx<double>>();
+        const cx<double>* twx = scTwX.as<cx<double>>();
+        const dim3 rows(static_cast<unsigned>(Ny), static_cast<unsigned>(Nz));
+        auto xfwd = [&](const double* in) {
+            hipLaunchKernelGGL(k_xfwd<double>, rows, dim3(256), ldsLength(Nx) * sizeof(cx<double>), s, in, C, 1, Nx, l2x,
+                               Ny, Nz, scNxh, scNxhP, twx, 0);
+        };
+        using YFn = void (*)(const cx<double>*, cx<double>*, int, int, int, int, int, int, const cx<double>*, int,
+                             const int*, int, int, YDerivPlan, const double*);
+        YFn yfn = k_yfft<double, 0, 0, false>;
+        if (l2y == 6 && l2t == 4) yfn = k_yfft<double, 6, 4, false>;
+        if (l2y == 7 && l2t == 4) yfn = k_yfft<double, 7, 4, false>;
+        if (l2y == 8 && l2t == 4) yfn = k_yfft<double, 8, 4, false>;
+        if (l2y == 9 && l2t == 3) yfn = k_yfft<double, 9, 3, false>;
+        if (l2y == 10 && l2t == 2) yfn = k_yfft<double, 10, 2, false>;
+        auto yfft = [&](int nch, int sign) {  // in place: a workgroup loads its whole tile before it stores
+            hipLaunchKernelGGL(yfn, dim3(static_cast<unsigned>(scNxhP / KT), static_cast<unsigned>(Nz), nch),
+                               dim3(kFftThreads), (static_cast<size_t>(KT) + 1) * Ny * sizeof(cx<double>), s, C, C, Ny,
+                               l2y, scNxh, scNxhP, l2t, Nz, scTwY.as<cx<double>>(), sign, nullptr, 0, 0, YDerivPlan{},
+                               nullptr);
+        };
+        auto xinv = [&](double* out, int nch) {
+            hipLaunchKernelGGL(k_xinv<double>, rows, dim3(256), static_cast<size_t>((nch + 1) / 2) * ldsLength(Nx) * sizeof(cx<double>),
+                               s, C, out, nch, Nx, l2x, Ny, Nz, scNxh, scNxhP, twx);
+        };
+        xfwd(scalar);
+        yfft(1, -1);
+        hipLaunchKernelGGL(k_scalarDeriv, dim3(static_cast<unsigned>((scNxhP + 7) / 8), static_cast<unsigned>(Ny)), dim3(256),
+                           static_cast<size_t>(Nz) * 8 * sizeof(cx<double>), s, C, Nx, Ny, Nz, scNxh, scNxhP, d.c.Lx,
+                           d.c.Ly, scD.as<double>(), 1.0 / static_cast<double>(d.c.Nx * d.c.Ny));
+        yfft(3, +1);
+        xinv(scR.as<double>(), 3);
+        const unsigned blocks = static_cast<unsigned>((d.R + 255) / 256);
+        hipLaunchKernelGGL(k_scalarProduct, dim3(blocks), dim3(256), 0, s, adv, scR.as<double>(), rhs, d.R);
+        if (antialias) {
+            xfwd(rhs);
+            yfft(1, -1);
+            const size_t plane = static_cast<size_t>(Ny) * scNxhP, n = static_cast<size_t>(Nz) * plane;
+            hipLaunchKernelGGL(k_scalarMask, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, C,
+                               scKeep.as<double>(), plane, n);
+            yfft(1, +1);
+            xinv(rhs, 1);
+        }
+        hipCheck(hipGetLastError(), "advectFGridScalar");
+    }
+
     void rhsAny(const double* Ap, const double* Am, const double* A0, double t, double t0, const Epilogue& ep,
                 hipStream_t s, bool fieldsReady = false) {
         if (precision == WV_PRECISION_FP64) rhs<double>(Ap, Am, A0, t, t0, ep, s, fieldsReady);
@@ -2887,6 +3063,26 @@ wv_status wv_nonlinear_flux_with_advection_fields_device(wv_kernel* kernel, doub
     wv_status st = wv_transform_wave_vortex_to_uvw_device(kernel, t, t0, Ap, Am, A0, uvw, stream);
     if (st != WV_SUCCESS) return st;
     return wv_nonlinear_flux_device(kernel, t, t0, Ap, Am, A0, Fp, Fm, F0, stream);
+}
+
+wv_status wv_advect_f_grid_scalar_device(wv_kernel* kernel, const double* scalar, const double* advectionFields,
+                                         int shouldAntialias, double* rightHandSide, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (!scalar || !rightHandSide)
+        return fail(WV_INVALID_POINTER, "F-grid scalar input and output pointers must be nonnull.");
+    if (!advectionFields) return fail(WV_INVALID_POINTER, "Advection fields has a null data pointer.");
+    const size_t bytes = k.d.R * sizeof(double);
+    if (overlaps(scalar, bytes, rightHandSide, bytes) || overlaps(advectionFields, 3 * bytes, scalar, bytes) ||
+        overlaps(advectionFields, 3 * bytes, rightHandSide, bytes))
+        return fail(WV_OVERLAPPING_ARRAYS, "Tracer scalar, velocity, and RHS arrays must not overlap.");
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        k.advectScalar(scalar, advectionFields, shouldAntialias != 0, rightHandSide, S(stream));
+        return WV_SUCCESS;
+    });
 }
 
 static wv_status inverseCommon(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,

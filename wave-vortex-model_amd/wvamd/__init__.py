@@ -90,7 +90,7 @@ _lib = None
 EXPORTED_SYMBOLS = [
     "wv_create", "wv_destroy", "wv_last_error", "wv_contract_version", "wv_nkl", "wv_coriolis_frequency",
     "wv_descriptor_modes", "wv_descriptor_table", "wv_persistent_bytes", "wv_nonlinear_flux_device",
-    "wv_nonlinear_flux", "wv_nonlinear_flux_with_advection_fields_device",
+    "wv_nonlinear_flux", "wv_nonlinear_flux_with_advection_fields_device", "wv_advect_f_grid_scalar_device",
     "wv_transform_wave_vortex_to_uvweta_device", "wv_transform_wave_vortex_to_uvw_device",
     "wv_transform_uvweta_to_wave_vortex_device", "wv_transform_uveta_to_wave_vortex_device",
     "wv_transform_to_spatial_domain_with_f_all_derivatives_device",
@@ -139,6 +139,8 @@ def library() -> C.CDLL:
     lib.wv_nonlinear_flux.argtypes = [vp, C.c_double, C.c_double] + [vp] * 6
     lib.wv_nonlinear_flux_with_advection_fields_device.restype = C.c_int
     lib.wv_nonlinear_flux_with_advection_fields_device.argtypes = [vp, C.c_double, C.c_double] + [vp] * 7 + [vp]
+    lib.wv_advect_f_grid_scalar_device.restype = C.c_int
+    lib.wv_advect_f_grid_scalar_device.argtypes = [vp, vp, vp, C.c_int, vp, vp]
     for fn in ("wv_transform_wave_vortex_to_uvweta_device", "wv_transform_wave_vortex_to_uvw_device"):
         getattr(lib, fn).restype = C.c_int
         getattr(lib, fn).argtypes = [vp, C.c_double, C.c_double, vp, vp, vp, vp, vp]
@@ -323,6 +325,19 @@ class WVTransformConstantStratificationKernel:
         _check(library().wv_transform_wave_vortex_to_uvw_device(
             self._h, float(t), float(t0), self._coef(Ap), self._coef(Am), self._coef(A0),
             self._dev(out, shape, "torch.float64"), self._stream(stream)))
+        return out
+
+    def advect_f_grid_scalar(self, scalar, advection_fields, should_antialias=True, out=None, stream=None):
+        """advectFGridScalar (...Kernel.cpp:1032-1138): -(u, v, w) . grad(scalar)
+        of an F-grid scalar [Nz, Ny, Nx] (the reference's [Nx, Ny, Nz]) with the
+        velocity bundle [3, Nz, Ny, Nx], antialiased when should_antialias."""
+        torch = _torch()
+        if out is None:
+            out = torch.zeros(self.spatial_shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        _check(library().wv_advect_f_grid_scalar_device(
+            self._h, self._dev(scalar, self.spatial_shape, "torch.float64"),
+            self._dev(advection_fields, (3,) + self.spatial_shape, "torch.float64"), int(bool(should_antialias)),
+            self._dev(out, self.spatial_shape, "torch.float64"), self._stream(stream)))
         return out
 
     def transform_to_wave_vortex(self, fields, t=0.0, t0=0.0, out=None, stream=None):
