@@ -166,6 +166,41 @@ def cpu_baseline(cfg, state, dt, threads, steps=4):
                       f"state, {sec:.1f} s"}
 
 
+def scalar_advection_leg(k64, calls=20, cpu=False):
+    """advectFGridScalar (FP64, antialiased) at the workload's grid: GPU time
+    per call with the inputs resident, against its algorithmic bytes (the
+    scalar, the three velocities and the result, 8 B each per point); with
+    `cpu`, one call of the numpy restatement on the host (the CPU reference
+    is the same algorithm in numpy/FFTW-style transforms)."""
+    import numpy as np
+    import torch
+    Nz, Ny, Nx = k64.spatial_shape
+    rng = np.random.default_rng(5)
+    scalar_h = rng.standard_normal((Nz, Ny, Nx))
+    uvw_h = rng.standard_normal((3, Nz, Ny, Nx)) * 0.05
+    scalar = torch.as_tensor(scalar_h).to(f"cuda:{k64.device}")
+    uvw = torch.as_tensor(uvw_h).to(f"cuda:{k64.device}")
+    out = torch.empty_like(scalar)
+    k64.advect_f_grid_scalar(scalar, uvw, True, out=out)  # the first call allocates the path's scratch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        k64.advect_f_grid_scalar(scalar, uvw, True, out=out)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / calls * 1e3
+    alg = 5 * 8 * Nx * Ny * Nz
+    res = {"precision": "fp64", "antialias": True, "calls": calls, "ms_per_call": round(ms, 4),
+           "algorithmic_MB": round(alg / 1e6, 1), "hbm_GBs": round(alg / (ms * 1e-3) / 1e9, 1)}
+    if cpu:
+        from oracle import Oracle
+        from scalar_advection import advect_f_grid_scalar
+        rows = Oracle(k64.config).modes()["row"]
+        t0 = time.perf_counter()
+        advect_f_grid_scalar(k64.config, scalar_h, uvw_h, True, rows)
+        res["cpu_numpy_s_per_call"] = round(time.perf_counter() - t0, 3)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +277,7 @@ def main():
     stages = {r["name"]: round(r["total_ms"] / max(r["launches"], 1), 4) for r in rec}
     kernels = per_kernel_rooflines(rec)
 
+    scalar_leg = scalar_advection_leg(k64, cpu=not args.no_cpu_baseline)
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -264,7 +300,8 @@ def main():
         "roofline": roofline, "reference_dataflow_roofline": step_model,
         "host_memory_flux_per_s": {"mixed": round(host32, 2), "fp64": round(host64, 2),
                                    "note": "PCIe-inclusive: state copied in and flux copied out per call"},
-        "stage_ms_per_launch_mixed": stages, "kernels_mixed": kernels, "cpu_baseline": cpu,
+        "stage_ms_per_launch_mixed": stages, "kernels_mixed": kernels, "scalar_advection": scalar_leg,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

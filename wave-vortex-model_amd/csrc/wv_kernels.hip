@@ -952,31 +952,39 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     }
 }
 
-// c2r along x of nch channels to FP64 real fields [Nx,Ny,Nz,nch] (inverse transforms).
+// c2r along x of nch channels to FP64 real fields [Nx,Ny,Nz,nch] (inverse
+// transforms).  A workgroup takes rpb consecutive (y, z) rows: its real
+// signals s = (row, channel) = (s / nch, s % nch) are paired two per complex
+// transform (2q real part, 2q + 1 imaginary part), so that small channel
+// counts still give the radix-16 passes several transforms.
 template <typename T>
-__global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, double* __restrict__ out, int nch, int Nx,
-                                              int log2Nx, int Ny, int Nz, int nkx, int nkxP,
+__global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, double* __restrict__ out, int nch, int rpb,
+                                              int Nx, int log2Nx, int Ny, int Nz, int nkx, int nkxP,
                                               const cx<T>* __restrict__ twX) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
-    const int pairs = (nch + 1) / 2;
+    const int nrows = Ny * Nz, row0 = static_cast<int>(blockIdx.x) * rpb;
+    const int nsig = rpb * nch, ntr = (nsig + 1) / 2;
     const int NxP = ldsLength(Nx);
-    const int y = blockIdx.x, z = blockIdx.y;
-    const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
-    for (int e = threadIdx.x; e < pairs * Nx; e += blockDim.x) {
-        const int p = e / Nx, k = e % Nx;
-        const cx<T> A = hermitianX(G, rowBase, chStride, 2 * p, k, Nx, nkx);
-        const cx<T> B = (2 * p + 1 < nch) ? hermitianX(G, rowBase, chStride, 2 * p + 1, k, Nx, nkx) : cx<T>{T(0), T(0)};
-        buf[p * NxP + ldsIndex(k, 1)] = {A.re - B.im, A.im + B.re};
+    for (int e = threadIdx.x; e < ntr * Nx; e += blockDim.x) {
+        const int q = e / Nx, k = e % Nx;
+        cx<T> A{T(0), T(0)}, B{T(0), T(0)};
+        const int sa = 2 * q, sb = 2 * q + 1;
+        if (row0 + sa / nch < nrows)
+            A = hermitianX(G, static_cast<size_t>(row0 + sa / nch) * nkxP, chStride, sa % nch, k, Nx, nkx);
+        if (sb < nsig && row0 + sb / nch < nrows)
+            B = hermitianX(G, static_cast<size_t>(row0 + sb / nch) * nkxP, chStride, sb % nch, k, Nx, nkx);
+        buf[q * NxP + ldsIndex(k, 1)] = {A.re - B.im, A.im + B.re};
     }
     __syncthreads();
-    fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<1>{pairs, 0, NxP}, twX);
+    fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
-    for (int e = threadIdx.x; e < nch * Nx; e += blockDim.x) {
-        const int c = e / Nx, x = e % Nx;
-        const cx<T> v = buf[(c / 2) * NxP + ldsIndex(x, 1)];
-        out[c * R + (static_cast<size_t>(z) * Ny + y) * Nx + x] = static_cast<double>((c & 1) ? v.im : v.re);
+    for (int e = threadIdx.x; e < nsig * Nx; e += blockDim.x) {
+        const int sg = e / Nx, x = e % Nx, row = row0 + sg / nch;
+        if (row >= nrows) continue;
+        const cx<T> v = buf[(sg / 2) * NxP + ldsIndex(x, 1)];
+        out[(sg % nch) * R + static_cast<size_t>(row) * Nx + x] = static_cast<double>((sg & 1) ? v.im : v.re);
     }
 }
 
@@ -1447,37 +1455,44 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
 }
 
 // r2c along x of nf real FP64 fields [Nx,Ny,Nz,nf] into P[c][z][y][kx < nkx]
-// (the horizontal forward plans of the forward transforms).
+// (the horizontal forward plans of the forward transforms).  A workgroup
+// takes rpb consecutive (y, z) rows, their signals paired as in k_xinv.
 template <typename T>
-__global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<T>* __restrict__ P, int nf, int Nx,
-                                              int log2Nx, int Ny, int Nz, int nkx, int nkxP,
+__global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<T>* __restrict__ P, int nf, int rpb,
+                                              int Nx, int log2Nx, int Ny, int Nz, int nkx, int nkxP,
                                               const cx<T>* __restrict__ twX, int adjointWeights) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
-    const int pairs = (nf + 1) / 2;
+    const int nrows = Ny * Nz, row0 = static_cast<int>(blockIdx.x) * rpb;
+    const int nsig = rpb * nf, ntr = (nsig + 1) / 2;
     const int NxP = ldsLength(Nx);
-    const int y = blockIdx.x, z = blockIdx.y;
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
-    const size_t row = (static_cast<size_t>(z) * Ny + y) * Nx;
-    for (int e = threadIdx.x; e < pairs * Nx; e += blockDim.x) {
-        const int p = e / Nx, x = e % Nx;
-        const T a = T(in[(2 * p) * R + row + x]);
-        const T b = (2 * p + 1 < nf) ? T(in[(2 * p + 1) * R + row + x]) : T(0);
-        buf[p * NxP + ldsIndex(x, 1)] = {a, b};
+    auto value = [&](int sg, int x) -> T {
+        const int row = row0 + sg / nf;
+        if (sg >= nsig || row >= nrows) return T(0);
+        return T(in[(sg % nf) * R + static_cast<size_t>(row) * Nx + x]);
+    };
+    for (int e = threadIdx.x; e < ntr * Nx; e += blockDim.x) {
+        const int q = e / Nx, x = e % Nx;
+        buf[q * NxP + ldsIndex(x, 1)] = {value(2 * q, x), value(2 * q + 1, x)};
     }
     __syncthreads();
-    fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<1>{pairs, 0, NxP}, twX);
-    const size_t rowBase = (static_cast<size_t>(z) * Ny + y) * nkxP;
+    fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
-    for (int e = threadIdx.x; e < pairs * nkx; e += blockDim.x) {
+    for (int e = threadIdx.x; e < ntr * nkx; e += blockDim.x) {
         const int q = e / nkx, k = e % nkx;
         const cx<T> Za = buf[q * NxP + ldsIndex(k, 1)];
         const cx<T> Zb = buf[q * NxP + ldsIndex(k == 0 ? 0 : Nx - k, 1)];
         // adjointWeights: the transpose of the Hermitian c2r (twice the r2c
         // for kx > 0; kx = 0 is real either way)
         const T h = adjointWeights && k > 0 ? T(1) : T(0.5);
-        P[(2 * q) * chStride + rowBase + k] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
-        if (2 * q + 1 < nf) P[(2 * q + 1) * chStride + rowBase + k] = {(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
+        const int sa = 2 * q, sb = 2 * q + 1;
+        if (row0 + sa / nf < nrows)
+            P[(sa % nf) * chStride + static_cast<size_t>(row0 + sa / nf) * nkxP + k] =
+                {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
+        if (sb < nsig && row0 + sb / nf < nrows)
+            P[(sb % nf) * chStride + static_cast<size_t>(row0 + sb / nf) * nkxP + k] =
+                {(Za.im + Zb.im) * h, (Zb.re - Za.re) * h};
     }
 }
 
@@ -2287,6 +2302,29 @@ struct KernelImpl {
         launchStage<T>(elapsed, ep, s);
     }
 
+    // The generic x transforms take rows in groups of about 16 real signals
+    // (8 complex transforms per workgroup).
+    static int xRowsPerBlock(int signalsPerRow) { return std::max(1, 16 / std::max(1, signalsPerRow)); }
+    template <typename T>
+    void launchXfwdRows(const double* in, cx<T>* P, int nf, int nkx, int nkxP, const cx<T>* tw, int adjointWeights,
+                        hipStream_t s) {
+        const int rpb = xRowsPerBlock(nf), ntr = (rpb * nf + 1) / 2;
+        const int nrows = static_cast<int>(d.c.Ny * d.c.Nz);
+        const size_t lds = static_cast<size_t>(ntr) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
+        hipLaunchKernelGGL(k_xfwd<T>, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256), lds, s, in, P,
+                           nf, rpb, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx), static_cast<int>(d.c.Ny),
+                           static_cast<int>(d.c.Nz), nkx, nkxP, tw, adjointWeights);
+    }
+    template <typename T>
+    void launchXinvRows(const cx<T>* G, double* out, int nch, int nkx, int nkxP, const cx<T>* tw, hipStream_t s) {
+        const int rpb = xRowsPerBlock(nch), ntr = (rpb * nch + 1) / 2;
+        const int nrows = static_cast<int>(d.c.Ny * d.c.Nz);
+        const size_t lds = static_cast<size_t>(ntr) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
+        hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256), lds, s, G, out,
+                           nch, rpb, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx), static_cast<int>(d.c.Ny),
+                           static_cast<int>(d.c.Nz), nkx, nkxP, tw);
+    }
+
     // ---- advectFGridScalar (FP64 in either precision, as the reference's)
     // Scratch, allocated on the first call and kept: three half spectra
     // [3][Nz][Ny][nxhP] (nxh = Nx/2 + 1 columns padded to the y tile), three
@@ -2348,13 +2386,11 @@ struct KernelImpl {
     void advectScalar(const double* scalar, const double* adv, bool antialias, double* rhs, hipStream_t s) {
         ensureScalarScratch();
         const int Nx = static_cast<int>(d.c.Nx), Ny = static_cast<int>(d.c.Ny), Nz = static_cast<int>(d.c.Nz);
-        const int l2x = log2Exact(d.c.Nx), l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
+        const int l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
         cx<double>* C = scC.as<cx<double>>();
         const cx<double>* twx = scTwX.as<cx<double>>();
-        const dim3 rows(static_cast<unsigned>(Ny), static_cast<unsigned>(Nz));
         auto xfwd = [&](const double* in) {
-            hipLaunchKernelGGL(k_xfwd<double>, rows, dim3(256), ldsLength(Nx) * sizeof(cx<double>), s, in, C, 1, Nx, l2x,
-                               Ny, Nz, scNxh, scNxhP, twx, 0);
+            launchXfwdRows<double>(in, C, 1, scNxh, scNxhP, twx, 0, s);
         };
         using YFn = void (*)(const cx<double>*, cx<double>*, int, int, int, int, int, int, const cx<double>*, int,
                              const int*, int, int, YDerivPlan, const double*);
@@ -2371,8 +2407,7 @@ struct KernelImpl {
                                nullptr);
         };
         auto xinv = [&](double* out, int nch) {
-            hipLaunchKernelGGL(k_xinv<double>, rows, dim3(256), static_cast<size_t>((nch + 1) / 2) * ldsLength(Nx) * sizeof(cx<double>),
-                               s, C, out, nch, Nx, l2x, Ny, Nz, scNxh, scNxhP, twx);
+            launchXinvRows<double>(C, out, nch, scNxh, scNxhP, twx, s);
         };
         xfwd(scalar);
         yfft(1, -1);
@@ -2413,13 +2448,8 @@ struct KernelImpl {
         else sp.g[1] = {1, 1, {{2, 0, 2, -1, -1}, {}, {}, {}}};
         launchSynth<T>(sp, s);
         launchYfft<T>(H, G, nf, +1, s, "yfft_inv", true, false);
-        const int pairs = (nf + 1) / 2;
-        const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xinv", static_cast<double>(nf) * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
-            hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
-                               lds, s, G.as<cx<T>>(), out, nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
-                               static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
-                               static_cast<int>(d.nkxP), twX.as<cx<T>>());
+            launchXinvRows<T>(G.as<cx<T>>(), out, nf, static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), s);
         });
     }
 
@@ -2428,13 +2458,8 @@ struct KernelImpl {
     template <typename T>
     void forward(const double* fields, double t, double t0, int nf, double* Ap, double* Am, double* A0,
                  hipStream_t s) {
-        const int pairs = (nf + 1) / 2;
-        const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xfwd", static_cast<double>(nf) * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
-            hipLaunchKernelGGL(k_xfwd<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
-                               lds, s, fields, P.as<cx<T>>(), nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
-                               static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
-                               static_cast<int>(d.nkxP), twX.as<cx<T>>(), 0);
+            launchXfwdRows<T>(fields, P.as<cx<T>>(), nf, static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), 0, s);
         });
         launchYfft<T>(P, Q, nf, -1, s, "yfft_fwd", false, true);
         const int ncg = static_cast<int>((ncol + 7) / 8);
@@ -2484,12 +2509,8 @@ struct KernelImpl {
             sp.g[1] = {b, 1, {{0, 1, 3, -1, -1}, {}, {}, {}}};
         launchSynth<T>(sp, s);
         launchYfft<T>(H, G, 4, +1, s, "yfft_inv", true, false);
-        const size_t lds = static_cast<size_t>(2) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xinv", 4.0 * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
-            hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
-                               lds, s, G.as<cx<T>>(), out, 4, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
-                               static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
-                               static_cast<int>(d.nkxP), twX.as<cx<T>>());
+            launchXinvRows<T>(G.as<cx<T>>(), out, 4, static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), s);
         });
     }
 
@@ -2664,13 +2685,8 @@ struct KernelImpl {
     template <typename T>
     void inverseVjp(const double* fieldsBar, double t, double t0, double* Gp, double* Gm, double* G0, hipStream_t s) {
         const int nf = 4;
-        const int pairs = 2;
-        const size_t lds = static_cast<size_t>(pairs) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
         timed("xfwd_adj", 0.0, s, [&] {
-            hipLaunchKernelGGL(k_xfwd<T>, dim3(static_cast<unsigned>(d.c.Ny), static_cast<unsigned>(d.c.Nz)), dim3(256),
-                               lds, s, fieldsBar, Gs.as<cx<T>>(), nf, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx),
-                               static_cast<int>(d.c.Ny), static_cast<int>(d.c.Nz), static_cast<int>(d.nkx),
-                               static_cast<int>(d.nkxP), twX.as<cx<T>>(), 1);
+            launchXfwdRows<T>(fieldsBar, Gs.as<cx<T>>(), nf, static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), 1, s);
         });
         launchYfft<T>(Gs, G, nf, -1, s, "yfft_adj", false, true);
         SynthPlan sp{};
