@@ -213,6 +213,8 @@ def main():
     ap.add_argument("--profile", choices=["mixed", "fp64", "both"], default=None,
                     help="run only `steps` RK4 steps of the workload plus a 512 MiB copy kernel "
                          "(the byte-counter calibration), for rocprofv3")
+    ap.add_argument("--profile-scalar", action="store_true",
+                    help="run only five F-grid scalar advection calls at the workload's grid, for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this path runs on a single MI355X "
@@ -224,6 +226,10 @@ def main():
     import wvamd
 
     cfg = workload_config()
+    if args.profile_scalar:
+        res = scalar_advection_leg(wvamd.WVTransformConstantStratificationKernel(cfg, "fp64"), calls=5)
+        print(f"scalar advection: {res['ms_per_call']} ms per call", flush=True)
+        return
     if args.profile:
         kk = wvamd.WVTransformConstantStratificationKernel(cfg, "fp64")
         state, dt = initial_state(kk)
