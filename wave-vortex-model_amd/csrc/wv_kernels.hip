@@ -992,7 +992,7 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
 // of the scalar's half spectrum C[0][z][l][k] (k < nxh = Nx/2 + 1, row
 // stride nxhP), for a workgroup's 8 columns k0 ... k0 + 7 of row l.  With
 // v = scale * C[0] (scale = 1/(Nx Ny)): C[0] = i k v (k = 0 at the x Nyquist
-// column), C[1] = i l v, C[2] = sum_w D[z][w] v_w, D the column operator of
+// column), C[1] = i l v, C[2] = sum_w DT[w][z] v_w, DT the transposed column operator of
 // the reference's DCT-I, -m_j, DST-I sequence (:1078-1094; rows z = 0 and
 // Nz - 1 zero); imaginary parts cleared on the self-conjugate rows
 // (:1095-1100).  LDS: the 8 columns over z.
@@ -1021,19 +1021,34 @@ __global__ __launch_bounds__(256) void k_scalarDeriv(cx<double>* __restrict__ C,
         const bool nyq = Nx % 2 == 0 && k == Nx / 2;
         const double kw = nyq ? 0.0 : 2.0 * kScalarPi * static_cast<double>(k) / Lx;
         const cx<double> v = col[e];
-        cx<double> dx{-v.im * kw, v.re * kw}, dy{-v.im * lw, v.re * lw}, dz{0.0, 0.0};
-        const double* Dz = D + static_cast<size_t>(z) * Nz;
-        for (int w = 0; w < Nz; ++w) {
-            const double a = Dz[w];
-            const cx<double> u = col[w * 8 + c];
-            dz.re += a * u.re;
-            dz.im += a * u.im;
-        }
-        if (lSelf && (k == 0 || nyq)) dx.im = dy.im = dz.im = 0.0;
+        cx<double> dx{-v.im * kw, v.re * kw}, dy{-v.im * lw, v.re * lw};
+        if (lSelf && (k == 0 || nyq)) dx.im = dy.im = 0.0;
         const size_t o = z * plane + base + c;
         C[o] = dx;
         C[chStride + o] = dy;
-        C[2 * chStride + o] = dz;
+    }
+    // d/dz: thread (z, half) sums four columns c = 4 half ... over w, one
+    // operator entry DT[w][z] (consecutive z, consecutive threads) per w
+    for (int e = threadIdx.x; e < Nz * 2; e += blockDim.x) {
+        const int z = e >> 1, c0 = 4 * (e & 1);
+        cx<double> acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+        for (int w = 0; w < Nz; ++w) {
+            const double a = D[static_cast<size_t>(w) * Nz + z];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const cx<double> u = col[w * 8 + c0 + i];
+                acc[i].re += a * u.re;
+                acc[i].im += a * u.im;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = k0 + c0 + i;
+            if (k >= nxh) continue;
+            const bool nyq = Nx % 2 == 0 && k == Nx / 2;
+            if (lSelf && (k == 0 || nyq)) acc[i].im = 0.0;
+            C[2 * chStride + z * plane + base + c0 + i] = acc[i];
+        }
     }
 }
 
@@ -2373,7 +2388,13 @@ struct KernelImpl {
         const size_t half = Nz * Ny * static_cast<size_t>(scNxhP);
         scC.alloc(3 * half * sizeof(cx<double>));
         scR.alloc(3 * d.R * sizeof(double));
-        upload(scD, scalarDzOperator(Nz, d.c.Lz));
+        {
+            const std::vector<double> D = scalarDzOperator(Nz, d.c.Lz);
+            std::vector<double> DT(Nz * Nz);
+            for (size_t z = 0; z < Nz; ++z)
+                for (size_t w = 0; w < Nz; ++w) DT[w * Nz + z] = D[z * Nz + w];
+            upload(scD, DT);  // [w][z]
+        }
         // the rows that store or complete a wave-vortex mode (:467-471)
         std::vector<double> keep(Ny * static_cast<size_t>(scNxhP), 0.0);
         const double scale = 1.0 / static_cast<double>(Nx * Ny);
