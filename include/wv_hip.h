@@ -152,8 +152,8 @@ wv_status wv_nonlinear_flux_with_advection_fields_device(wv_kernel* kernel, doub
  * grad(scalar) for an F-grid scalar [Nx,Ny,Nz] advected by the velocity bundle
  * advectionFields [Nx,Ny,Nz,3] (u, v, w), then antialiased to the retained
  * modes' rows when shouldAntialias is nonzero (:1008-1030).  FP64 in either
- * precision.  The first call allocates the path's scratch (three half spectra
- * and three fields); later calls allocate nothing.  Same null-pointer,
+ * precision.  The first call allocates the path's scratch (three half
+ * spectra); later calls allocate nothing.  Same null-pointer,
  * overlap and reentrancy errors as the reference. */
 wv_status wv_advect_f_grid_scalar_device(wv_kernel* kernel, const double* scalar, const double* advectionFields,
                                          int shouldAntialias, double* rightHandSide, void* stream);

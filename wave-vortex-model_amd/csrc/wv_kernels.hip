@@ -956,11 +956,13 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
 // transforms).  A workgroup takes rpb consecutive (y, z) rows: its real
 // signals s = (row, channel) = (s / nch, s % nch) are paired two per complex
 // transform (2q real part, 2q + 1 imaginary part), so that small channel
-// counts still give the radix-16 passes several transforms.
-template <typename T>
+// counts still give the radix-16 passes several transforms.  PRODUCT (nch =
+// 3, the scalar advection's d/dx, d/dy, d/dz): out = -(u, v, w) . (the three
+// fields) of each row, with adv = [u, v, w] ([3][R]), instead of the fields.
+template <typename T, bool PRODUCT = false>
 __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, double* __restrict__ out, int nch, int rpb,
                                               int Nx, int log2Nx, int Ny, int Nz, int nkx, int nkxP,
-                                              const cx<T>* __restrict__ twX) {
+                                              const cx<T>* __restrict__ twX, const double* __restrict__ adv = nullptr) {
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nrows = Ny * Nz, row0 = static_cast<int>(blockIdx.x) * rpb;
@@ -980,6 +982,19 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
     __syncthreads();
     fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
+    if constexpr (PRODUCT) {
+        auto field = [&](int sg, int x) {
+            const cx<T> v = buf[(sg / 2) * NxP + ldsIndex(x, 1)];
+            return static_cast<double>((sg & 1) ? v.im : v.re);
+        };
+        for (int e = threadIdx.x; e < rpb * Nx; e += blockDim.x) {
+            const int r = e / Nx, x = e % Nx, row = row0 + r;
+            if (row >= nrows) continue;
+            const size_t i = static_cast<size_t>(row) * Nx + x;
+            out[i] = -(adv[i] * field(3 * r, x) + adv[R + i] * field(3 * r + 1, x) + adv[2 * R + i] * field(3 * r + 2, x));
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < nsig * Nx; e += blockDim.x) {
         const int sg = e / Nx, x = e % Nx, row = row0 + sg / nch;
         if (row >= nrows) continue;
@@ -1050,15 +1065,6 @@ __global__ __launch_bounds__(256) void k_scalarDeriv(cx<double>* __restrict__ C,
             C[2 * chStride + z * plane + base + c0 + i] = acc[i];
         }
     }
-}
-
-// rhs = -(u dx + v dy + w dz) over the R points (advectFGridScalar :1124-1128):
-// adv and d3 are [3][R].
-__global__ __launch_bounds__(256) void k_scalarProduct(const double* __restrict__ adv, const double* __restrict__ d3,
-                                                       double* __restrict__ rhs, size_t R) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= R) return;
-    rhs[i] = -(adv[i] * d3[i] + adv[R + i] * d3[R + i] + adv[2 * R + i] * d3[2 * R + i]);
 }
 
 // The scalar antialias's row mask and scaling (:1018-1022): C[z][l][k] times
@@ -2342,10 +2348,9 @@ struct KernelImpl {
 
     // ---- advectFGridScalar (FP64 in either precision, as the reference's)
     // Scratch, allocated on the first call and kept: three half spectra
-    // [3][Nz][Ny][nxhP] (nxh = Nx/2 + 1 columns padded to the y tile), three
-    // fields [3][R], the d/dz column operator, the antialias row weights and
-    // FP64 twiddles.
-    DeviceBuffer scC, scR, scD, scKeep, scTwX, scTwY;
+    // [3][Nz][Ny][nxhP] (nxh = Nx/2 + 1 columns padded to the y tile), the
+    // d/dz column operator, the antialias row weights and FP64 twiddles.
+    DeviceBuffer scC, scD, scKeep, scTwX, scTwY;
     int scNxh = 0, scNxhP = 0;
 
     // d/dz of an F-grid column (...Kernel.cpp:1078-1094) as one matrix: the
@@ -2387,7 +2392,6 @@ struct KernelImpl {
             throw Failure{WV_UNSUPPORTED_OPERATION, "Scalar advection needs more LDS per workgroup than the device provides."};
         const size_t half = Nz * Ny * static_cast<size_t>(scNxhP);
         scC.alloc(3 * half * sizeof(cx<double>));
-        scR.alloc(3 * d.R * sizeof(double));
         {
             const std::vector<double> D = scalarDzOperator(Nz, d.c.Lz);
             std::vector<double> DT(Nz * Nz);
@@ -2436,9 +2440,14 @@ struct KernelImpl {
                            static_cast<size_t>(Nz) * 8 * sizeof(cx<double>), s, C, Nx, Ny, Nz, scNxh, scNxhP, d.c.Lx,
                            d.c.Ly, scD.as<double>(), 1.0 / static_cast<double>(d.c.Nx * d.c.Ny));
         yfft(3, +1);
-        xinv(scR.as<double>(), 3);
-        const unsigned blocks = static_cast<unsigned>((d.R + 255) / 256);
-        hipLaunchKernelGGL(k_scalarProduct, dim3(blocks), dim3(256), 0, s, adv, scR.as<double>(), rhs, d.R);
+        {
+            // c2r of d/dx, d/dy, d/dz with the product -(u, v, w) . grad in the epilogue
+            const int rpb = xRowsPerBlock(3), ntr = (rpb * 3 + 1) / 2;
+            const int nrows = Ny * Nz;
+            hipLaunchKernelGGL((k_xinv<double, true>), dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256),
+                               static_cast<size_t>(ntr) * ldsLength(Nx) * sizeof(cx<double>), s, C, rhs, 3, rpb, Nx,
+                               log2Exact(d.c.Nx), Ny, Nz, scNxh, scNxhP, twx, adv);
+        }
         if (antialias) {
             xfwd(rhs);
             yfft(1, -1);
