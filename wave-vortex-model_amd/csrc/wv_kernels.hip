@@ -1067,14 +1067,39 @@ __global__ __launch_bounds__(256) void k_scalarDeriv(cx<double>* __restrict__ C,
     }
 }
 
-// The scalar antialias's row mask and scaling (:1018-1022): C[z][l][k] times
-// keep[l][k] (1/(Nx Ny) on kept rows, 0 elsewhere).
-__global__ __launch_bounds__(256) void k_scalarMask(cx<double>* __restrict__ C, const double* __restrict__ keep,
-                                                    size_t plane, size_t n) {
-    const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double w = keep[i % plane];
-    C[i] = cx<double>{C[i].re * w, C[i].im * w};
+// The scalar antialias's y round trip in one pass (:1014-1029): a workgroup
+// loads its [Ny x KT] tile of the x half spectrum C[z][y][k], transforms it
+// along y, weights each (ky, kx) entry by keep (1/(Nx Ny) on the kept rows,
+// 0 elsewhere), transforms back and stores the columns k < nkx.
+template <int L2N, int L2KT>
+__global__ __launch_bounds__(kFftThreads) void k_ymask(cx<double>* __restrict__ C, int NyArg, int log2NyArg, int nkx,
+                                                       int nkxP, int log2KTArg, const cx<double>* __restrict__ twY,
+                                                       const double* __restrict__ keep) {
+    const int Ny = L2N > 0 ? 1 << L2N : NyArg;
+    const int log2Ny = L2N > 0 ? L2N : log2NyArg;
+    const int log2KT = L2N > 0 ? L2KT : log2KTArg;
+    const int KT = 1 << log2KT;
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<double>* tw = reinterpret_cast<cx<double>*>(smem);
+    cx<double>* tile = tw + Ny;
+    const int kx0 = static_cast<int>(blockIdx.x) * KT;
+    const size_t plane = static_cast<size_t>(blockIdx.y) * Ny * nkxP;
+    const int nel = Ny * KT;
+    loadTwiddles(tw, twY, Ny);
+    for (int e = fftThread(); e < nel; e += kFftThreads)
+        tile[e] = C[plane + static_cast<size_t>(e >> log2KT) * nkxP + kx0 + (e & (KT - 1))];
+    __syncthreads();
+    fftLds<double, -1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
+    for (int e = fftThread(); e < nel; e += kFftThreads) {
+        const double w = keep[static_cast<size_t>(e >> log2KT) * nkxP + kx0 + (e & (KT - 1))];
+        tile[e] = cx<double>{tile[e].re * w, tile[e].im * w};
+    }
+    __syncthreads();
+    fftLds<double, +1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
+    for (int e = fftThread(); e < nel; e += kFftThreads) {
+        const int kk = e & (KT - 1);
+        if (kx0 + kk < nkx) C[plane + static_cast<size_t>(e >> log2KT) * nkxP + kx0 + kk] = tile[e];
+    }
 }
 
 // Projection targets of the NT transformed products: u, v, w, eta
@@ -2450,11 +2475,16 @@ struct KernelImpl {
         }
         if (antialias) {
             xfwd(rhs);
-            yfft(1, -1);
-            const size_t plane = static_cast<size_t>(Ny) * scNxhP, n = static_cast<size_t>(Nz) * plane;
-            hipLaunchKernelGGL(k_scalarMask, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, C,
-                               scKeep.as<double>(), plane, n);
-            yfft(1, +1);
+            using MFn = void (*)(cx<double>*, int, int, int, int, int, const cx<double>*, const double*);
+            MFn mfn = k_ymask<0, 0>;
+            if (l2y == 6 && l2t == 4) mfn = k_ymask<6, 4>;
+            if (l2y == 7 && l2t == 4) mfn = k_ymask<7, 4>;
+            if (l2y == 8 && l2t == 4) mfn = k_ymask<8, 4>;
+            if (l2y == 9 && l2t == 3) mfn = k_ymask<9, 3>;
+            if (l2y == 10 && l2t == 2) mfn = k_ymask<10, 2>;
+            hipLaunchKernelGGL(mfn, dim3(static_cast<unsigned>(scNxhP / KT), static_cast<unsigned>(Nz)),
+                               dim3(kFftThreads), (static_cast<size_t>(KT) + 1) * Ny * sizeof(cx<double>), s, C, Ny,
+                               l2y, scNxh, scNxhP, l2t, scTwY.as<cx<double>>(), scKeep.as<double>());
             xinv(rhs, 1);
         }
         hipCheck(hipGetLastError(), "advectFGridScalar");
