@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
 // Nz - 1 zero); imaginary parts cleared on the self-conjugate rows
 // (:1095-1100).  LDS: the 8 columns over z.
 constexpr double kScalarPi = 3.14159265358979323846;
-__global__ __launch_bounds__(256) void k_scalarDeriv(cx<double>* __restrict__ C, int Nx, int Ny, int Nz, int nxh,
+__global__ __launch_bounds__(1024) void k_scalarDeriv(cx<double>* __restrict__ C, int Nx, int Ny, int Nz, int nxh,
                                                      int nxhP, double Lx, double Ly, const double* __restrict__ D,
                                                      double scale) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -2461,7 +2461,10 @@ struct KernelImpl {
         };
         xfwd(scalar);
         yfft(1, -1);
-        hipLaunchKernelGGL(k_scalarDeriv, dim3(static_cast<unsigned>((scNxhP + 7) / 8), static_cast<unsigned>(Ny)), dim3(256),
+        // one thread per (z, half of the tile) for d/dz: whole waves, at most 1024
+        const unsigned derivThreads = static_cast<unsigned>(std::min(1024, (2 * Nz + 63) / 64 * 64));
+        hipLaunchKernelGGL(k_scalarDeriv, dim3(static_cast<unsigned>((scNxhP + 7) / 8), static_cast<unsigned>(Ny)),
+                           dim3(derivThreads),
                            static_cast<size_t>(Nz) * 8 * sizeof(cx<double>), s, C, Nx, Ny, Nz, scNxh, scNxhP, d.c.Lx,
                            d.c.Ly, scD.as<double>(), 1.0 / static_cast<double>(d.c.Nx * d.c.Ny));
         yfft(3, +1);
