@@ -1042,27 +1042,35 @@ __global__ __launch_bounds__(1024) void k_scalarDeriv(cx<double>* __restrict__ C
         C[o] = dx;
         C[chStride + o] = dy;
     }
-    // d/dz: thread (z, half) sums four columns c = 4 half ... over w, one
-    // operator entry DT[w][z] (consecutive z, consecutive threads) per w
-    for (int e = threadIdx.x; e < Nz * 2; e += blockDim.x) {
-        const int z = e >> 1, c0 = 4 * (e & 1);
-        cx<double> acc[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-        for (int w = 0; w < Nz; ++w) {
-            const double a = D[static_cast<size_t>(w) * Nz + z];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const cx<double> u = col[w * 8 + c0 + i];
-                acc[i].re += a * u.re;
-                acc[i].im += a * u.im;
-            }
+    // d/dz on FP64 MFMA 16x16x4: the tile's 8 complex columns are the 16 real
+    // GEMM columns n (column n / 2, real or imaginary part n % 2); wave w
+    // takes 16-row z tiles w, w + nwaves, ... and contracts over w in steps of
+    // 4 (entries beyond Nz are zeros).  Lane l: A = DT[4 s + l/16][16 zt + l%16],
+    // B = the staged value (w = 4 s + l/16, n = l%16); C element i is
+    // z = 16 zt + l/16 + 4 i, n = l%16.
+    using MF = Mfma<double>;
+    const int lane = threadIdx.x & 63, nwaves = static_cast<int>(blockDim.x >> 6);
+    const int n = lane & 15, kq = lane >> 4, c = n >> 1, part = n & 1;
+    const double* colw = reinterpret_cast<const double*>(col) + 2 * c + part;  // word of (w, c, part) at w * 16
+    const int k = k0 + c;
+    const bool nyqc = Nx % 2 == 0 && k == Nx / 2;
+    const bool zeroIm = part == 1 && lSelf && (k == 0 || nyqc);
+    const int nsteps = (Nz + 3) / 4, ztiles = (Nz + 15) / 16;
+    for (int zt = static_cast<int>(threadIdx.x >> 6); zt < ztiles; zt += nwaves) {
+        const int za = 16 * zt + n;  // this lane's A row
+        typename MF::acc acc = MF::zero();
+        for (int st = 0; st < nsteps; ++st) {
+            const int w = 4 * st + kq;
+            const double a = (w < Nz && za < Nz) ? D[static_cast<size_t>(w) * Nz + za] : 0.0;
+            const double b = w < Nz ? colw[w * 16] : 0.0;
+            acc = MF::run(a, b, acc);
         }
+        if (k >= nxh) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int k = k0 + c0 + i;
-            if (k >= nxh) continue;
-            const bool nyq = Nx % 2 == 0 && k == Nx / 2;
-            if (lSelf && (k == 0 || nyq)) acc[i].im = 0.0;
-            C[2 * chStride + z * plane + base + c0 + i] = acc[i];
+            const int z = 16 * zt + MF::row(kq, i);
+            if (z >= Nz) continue;
+            reinterpret_cast<double*>(C + 2 * chStride + z * plane + base + c)[part] = zeroIm ? 0.0 : acc[i];
         }
     }
 }
@@ -2461,8 +2469,8 @@ struct KernelImpl {
         };
         xfwd(scalar);
         yfft(1, -1);
-        // one thread per (z, half of the tile) for d/dz: whole waves, at most 1024
-        const unsigned derivThreads = static_cast<unsigned>(std::min(1024, (2 * Nz + 63) / 64 * 64));
+        // one wave per 16-row z tile of d/dz, at most 16 waves
+        const unsigned derivThreads = static_cast<unsigned>(64 * std::min(16, (Nz + 15) / 16));
         hipLaunchKernelGGL(k_scalarDeriv, dim3(static_cast<unsigned>((scNxhP + 7) / 8), static_cast<unsigned>(Ny)),
                            dim3(derivThreads),
                            static_cast<size_t>(Nz) * 8 * sizeof(cx<double>), s, C, Nx, Ny, Nz, scNxh, scNxhP, d.c.Lx,
