@@ -959,10 +959,12 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
 // counts still give the radix-16 passes several transforms.  PRODUCT (nch =
 // 3, the scalar advection's d/dx, d/dy, d/dz): out = -(u, v, w) . (the three
 // fields) of each row, with adv = [u, v, w] ([3][R]), instead of the fields.
-template <typename T, bool PRODUCT = false>
+template <typename T, bool PRODUCT = false, int L2N = 0>
 __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, double* __restrict__ out, int nch, int rpb,
-                                              int Nx, int log2Nx, int Ny, int Nz, int nkx, int nkxP,
+                                              int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
                                               const cx<T>* __restrict__ twX, const double* __restrict__ adv = nullptr) {
+    const int Nx = L2N > 0 ? 1 << L2N : NxArg;  // L2N > 0: the length fixed at compile time
+    const int log2Nx = L2N > 0 ? L2N : log2NxArg;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nrows = Ny * Nz, row0 = static_cast<int>(blockIdx.x) * rpb;
@@ -980,7 +982,7 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
         buf[q * NxP + ldsIndex(k, 1)] = {A.re - B.im, A.im + B.re};
     }
     __syncthreads();
-    fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
+    fftLds<T, +1, L2N>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
     if constexpr (PRODUCT) {
         auto field = [&](int sg, int x) {
@@ -1511,10 +1513,12 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
 // r2c along x of nf real FP64 fields [Nx,Ny,Nz,nf] into P[c][z][y][kx < nkx]
 // (the horizontal forward plans of the forward transforms).  A workgroup
 // takes rpb consecutive (y, z) rows, their signals paired as in k_xinv.
-template <typename T>
+template <typename T, int L2N = 0>
 __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<T>* __restrict__ P, int nf, int rpb,
-                                              int Nx, int log2Nx, int Ny, int Nz, int nkx, int nkxP,
+                                              int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
                                               const cx<T>* __restrict__ twX, int adjointWeights) {
+    const int Nx = L2N > 0 ? 1 << L2N : NxArg;  // L2N > 0: the length fixed at compile time
+    const int log2Nx = L2N > 0 ? L2N : log2NxArg;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
     const int nrows = Ny * Nz, row0 = static_cast<int>(blockIdx.x) * rpb;
@@ -1531,7 +1535,7 @@ __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<
         buf[q * NxP + ldsIndex(x, 1)] = {value(2 * q, x), value(2 * q + 1, x)};
     }
     __syncthreads();
-    fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
+    fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     for (int e = threadIdx.x; e < ntr * nkx; e += blockDim.x) {
         const int q = e / nkx, k = e % nkx;
@@ -2365,7 +2369,15 @@ struct KernelImpl {
         const int rpb = xRowsPerBlock(nf), ntr = (rpb * nf + 1) / 2;
         const int nrows = static_cast<int>(d.c.Ny * d.c.Nz);
         const size_t lds = static_cast<size_t>(ntr) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
-        hipLaunchKernelGGL(k_xfwd<T>, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256), lds, s, in, P,
+        auto fn = k_xfwd<T, 0>;
+        switch (log2Exact(d.c.Nx)) {  // lengths 64 ... 512 compiled as such
+            case 6: fn = k_xfwd<T, 6>; break;
+            case 7: fn = k_xfwd<T, 7>; break;
+            case 8: fn = k_xfwd<T, 8>; break;
+            case 9: fn = k_xfwd<T, 9>; break;
+            default: break;
+        }
+        hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256), lds, s, in, P,
                            nf, rpb, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx), static_cast<int>(d.c.Ny),
                            static_cast<int>(d.c.Nz), nkx, nkxP, tw, adjointWeights);
     }
@@ -2374,9 +2386,17 @@ struct KernelImpl {
         const int rpb = xRowsPerBlock(nch), ntr = (rpb * nch + 1) / 2;
         const int nrows = static_cast<int>(d.c.Ny * d.c.Nz);
         const size_t lds = static_cast<size_t>(ntr) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
-        hipLaunchKernelGGL(k_xinv<T>, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256), lds, s, G, out,
+        auto fn = k_xinv<T, false, 0>;
+        switch (log2Exact(d.c.Nx)) {  // lengths 64 ... 512 compiled as such
+            case 6: fn = k_xinv<T, false, 6>; break;
+            case 7: fn = k_xinv<T, false, 7>; break;
+            case 8: fn = k_xinv<T, false, 8>; break;
+            case 9: fn = k_xinv<T, false, 9>; break;
+            default: break;
+        }
+        hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256), lds, s, G, out,
                            nch, rpb, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx), static_cast<int>(d.c.Ny),
-                           static_cast<int>(d.c.Nz), nkx, nkxP, tw);
+                           static_cast<int>(d.c.Nz), nkx, nkxP, tw, static_cast<const double*>(nullptr));
     }
 
     // ---- advectFGridScalar (FP64 in either precision, as the reference's)
@@ -2480,7 +2500,15 @@ struct KernelImpl {
             // c2r of d/dx, d/dy, d/dz with the product -(u, v, w) . grad in the epilogue
             const int rpb = xRowsPerBlock(3), ntr = (rpb * 3 + 1) / 2;
             const int nrows = Ny * Nz;
-            hipLaunchKernelGGL((k_xinv<double, true>), dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256),
+            auto pfn = k_xinv<double, true, 0>;
+            switch (log2Exact(d.c.Nx)) {
+                case 6: pfn = k_xinv<double, true, 6>; break;
+                case 7: pfn = k_xinv<double, true, 7>; break;
+                case 8: pfn = k_xinv<double, true, 8>; break;
+                case 9: pfn = k_xinv<double, true, 9>; break;
+                default: break;
+            }
+            hipLaunchKernelGGL(pfn, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256),
                                static_cast<size_t>(ntr) * ldsLength(Nx) * sizeof(cx<double>), s, C, rhs, 3, rpb, Nx,
                                log2Exact(d.c.Nx), Ny, Nz, scNxh, scNxhP, twx, adv);
         }
