@@ -41,9 +41,11 @@ def test_scalar_advection_matches_reference_golden(torch_gpu, path, precision):
         assert err <= TOL, (key, err)
 
 
-@pytest.mark.parametrize("N,Nz,hydro,aa", [(64, 33, False, True), (256, 33, False, True), (96, 17, True, False)])
+@pytest.mark.parametrize("N,Nz,hydro,aa", [(64, 33, False, True), (256, 33, False, True), (96, 17, True, False),
+                                             (128, 17, False, True), (512, 9, False, True)])
 def test_scalar_advection_matches_oracle(torch_gpu, N, Nz, hydro, aa):
-    """Larger grids (power-of-two and not) against the numpy restatement;
+    """Larger grids against the numpy restatement: x lengths 64, 128, 256 and
+    512 (the transforms compiled for their length) and 96 (runtime length);
     calling twice checks that the kept scratch holds no state."""
     import wvamd
     from scalar_advection import advect_f_grid_scalar
