@@ -23,8 +23,13 @@
  * points without the suffix take host pointers, copy in and out, and
  * synchronize (the reference's host-memory semantics).
  *
- * Like the reference kernel, a wv_kernel is not reentrant: concurrent calls on
- * one handle return WV_REENTRANT_EXECUTION.  Distinct handles are independent.
+ * Like the reference kernel, a wv_kernel is not reentrant: a call made while
+ * another call on the same handle is still inside the library returns
+ * WV_REENTRANT_EXECUTION.  Calls on one handle share its device workspaces;
+ * each call records an event on its stream once its work is enqueued, and a
+ * later call on a different stream makes that stream wait for it, so the
+ * calls on one handle run on the device in the order they were made,
+ * whatever streams they use.  Distinct handles are independent.
  */
 #ifndef WV_HIP_H
 #define WV_HIP_H
@@ -69,6 +74,36 @@ typedef struct wv_config {
 } wv_config;
 
 typedef struct wv_kernel wv_kernel;
+
+/* WVKernelMetrics (...Kernel.hpp:12-45): same members, order and meaning.
+ *   descriptorBytes   device bytes of the descriptor tables and column maps
+ *   planBytes         device bytes of the transform operators and twiddles
+ *   engineBytes, kernelManagementBytes  0 / host bytes of the handle
+ *   scratchCapacityBytes = halfSpectrumScratchCapacityBytes (the half-spectrum
+ *                     and coefficient workspaces) + realScratchCapacityBytes
+ *                     (0: no real field is ever stored by the flux)
+ *   planCount         the 17 logical transform plans of the contract's
+ *                     schedule (PlanIndex, ...Kernel.cpp:47-55), all prepared
+ *                     at create, + 1 once scalar advection is prepared
+ *                     (ensureScalarInversePlan, :992-1002)
+ *   execution counts  the reference's plan executions that each call stands
+ *                     for (per flux: 1 + 2 per target horizontal, 3 + 4 per
+ *                     target vertical), counted per call
+ *   *Seconds          device time of the last flux (phase = evolved spectra,
+ *                     reconstruction = synthesis + inverse y transform,
+ *                     product = fused x transforms and products, projection =
+ *                     forward y transform + analysis + projection) when stage
+ *                     instrumentation is on, else 0 */
+typedef struct wv_kernel_metrics {
+    size_t descriptorBytes, planBytes, engineBytes, kernelManagementBytes, scratchCapacityBytes,
+        scratchHighWaterBytes, halfSpectrumScratchCapacityBytes, realScratchCapacityBytes, planCount, executionCount,
+        horizontalExecutionCount, verticalExecutionCount, nonlinearFluxCallCount, nonlinearFluxPhaseEvaluationCount,
+        advectionVelocityReconstructionCount, scalarAdvectionCount, scalarAntialiasCount, bytesCopied;
+    double phaseSeconds, reconstructionSeconds, derivativeReconstructionSeconds, productSeconds, projectionSeconds,
+        coefficientAssemblySeconds, derivativeCoefficientAssemblySeconds, coefficientProjectionSeconds,
+        scalarForwardSeconds, scalarDerivativeAssemblySeconds, scalarVerticalDerivativeSeconds, scalarInverseSeconds,
+        scalarProductSeconds, scalarAntialiasSeconds;
+} wv_kernel_metrics;
 
 /* Per-stage device time of the last calls (WVKernelMetrics, ...Kernel.hpp:12-45),
  * filled from HIP events when stage instrumentation is on. */
@@ -131,6 +166,24 @@ long wv_descriptor_table(const wv_kernel* kernel, const char* name, double* out)
 /* bytes held on the device by this kernel (descriptor tables + workspaces);
  * WVTransformConstantStratificationKernel::persistentBytes (...Kernel.cpp:514-519). */
 size_t wv_persistent_bytes(const wv_kernel* kernel);
+/* scratchBytes (...Kernel.hpp:70): bytes of the kernel's workspaces (half
+ * spectra, coefficient work arrays). */
+size_t wv_scratch_bytes(const wv_kernel* kernel);
+/* metrics() (...Kernel.cpp:522-537): copies the counters into *out. */
+wv_status wv_metrics(const wv_kernel* kernel, wv_kernel_metrics* out);
+/* Identifier accessors (...Kernel.hpp:59-67, ...Kernel.cpp:539-568): static
+ * strings naming this implementation; NULL for a null kernel. */
+const char* wv_engine_identifier(const wv_kernel* kernel);
+const char* wv_engine_library_identity(const wv_kernel* kernel);
+const char* wv_nonlinear_flux_schedule_identifier(const wv_kernel* kernel);
+const char* wv_phase_implementation_identifier(const wv_kernel* kernel);
+const char* wv_coefficient_arithmetic_mode_identifier(const wv_kernel* kernel);
+const char* wv_inverse_normalization_placement_identifier(const wv_kernel* kernel);
+const char* wv_optimization_implementation_identifier(const wv_kernel* kernel);
+/* coefficientWorkerCount (:563): 0, no host worker threads. */
+size_t wv_coefficient_worker_count(const wv_kernel* kernel);
+/* phaseReservationBytes (:565-568): NxHalf * Ny * Nz complex values. */
+size_t wv_phase_reservation_bytes(const wv_kernel* kernel);
 
 /* --- the flux ---------------------------------------------------------------
  * WVTransformConstantStratificationKernel::nonlinearFlux (...Kernel.cpp:1140-1146,
@@ -143,18 +196,34 @@ wv_status wv_nonlinear_flux(wv_kernel* kernel, double t, double t0, const double
                             const double* A0, double* Fp, double* Fm, double* F0);
 
 /* nonlinearFluxWithAdvectionFields (...Kernel.cpp:1148-1158): also returns the
- * advecting (u,v,w) as a [Nx,Ny,Nz,3] bundle. */
+ * advecting (u,v,w) as a [Nx,Ny,Nz,3] bundle.  Every argument is validated
+ * before anything is written. */
 wv_status wv_nonlinear_flux_with_advection_fields_device(wv_kernel* kernel, double t, double t0, const double* Ap,
                                                          const double* Am, const double* A0, double* Fp,
                                                          double* Fm, double* F0, double* uvw, void* stream);
+
+/* nonlinearFluxUsingAdvectionFields (...Kernel.cpp:1160-1171): the flux with
+ * the caller's advecting (u,v,w) [Nx,Ny,Nz,3] in the products instead of the
+ * state's own (the fields a previous nonlinearFluxWithAdvectionFields
+ * returned, shared with tracer advection).  uvw may not overlap the state or
+ * the flux. */
+wv_status wv_nonlinear_flux_using_advection_fields_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                          const double* Am, const double* A0, const double* uvw,
+                                                          double* Fp, double* Fm, double* F0, void* stream);
+
+/* prepareScalarAdvection (...Kernel.cpp:1004-1006): allocates the scalar
+ * advection path's workspace now, so that advectFGridScalar calls allocate
+ * nothing.  Idempotent. */
+wv_status wv_prepare_scalar_advection(wv_kernel* kernel);
 
 /* advectFGridScalar (...Kernel.cpp:1032-1138): rightHandSide = -(u, v, w) .
  * grad(scalar) for an F-grid scalar [Nx,Ny,Nz] advected by the velocity bundle
  * advectionFields [Nx,Ny,Nz,3] (u, v, w), then antialiased to the retained
  * modes' rows when shouldAntialias is nonzero (:1008-1030).  FP64 in either
- * precision.  The first call allocates the path's scratch (three half
- * spectra); later calls allocate nothing.  Same null-pointer,
- * overlap and reentrancy errors as the reference. */
+ * precision.  Without a prior wv_prepare_scalar_advection the first call
+ * allocates the path's scratch (three half spectra), as the reference's
+ * first call creates its plan (:1009); later calls allocate nothing.  Same
+ * null-pointer, overlap and reentrancy errors as the reference. */
 wv_status wv_advect_f_grid_scalar_device(wv_kernel* kernel, const double* scalar, const double* advectionFields,
                                          int shouldAntialias, double* rightHandSide, void* stream);
 
@@ -174,6 +243,13 @@ wv_status wv_transform_uvweta_to_wave_vortex_device(wv_kernel* kernel, const dou
 wv_status wv_transform_uveta_to_wave_vortex_device(wv_kernel* kernel, const double* fields, double t, double t0,
                                                    double* Ap, double* Am, double* A0, void* stream);
 
+/* transformStateFieldDerivatives (...Kernel.cpp:719-742): d/dx, d/dy, d/dz of
+ * the dynamical field `field` (0 u, 1 v, 2 w, 3 eta; WVDynamicalField,
+ * ...Kernel.hpp:10) of the state at time t, derivatives [Nx,Ny,Nz,3]. */
+wv_status wv_transform_state_field_derivatives_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                      const double* Am, const double* A0, int field,
+                                                      double* derivatives, void* stream);
+
 /* transformToSpatialDomainWithFAllDerivatives / ...GAllDerivatives (...Kernel.cpp:819-863):
  * the F-family (cosine) or G-family (sine) field Apm*waveScale + A0*geostrophicScale
  * and its x, y, z derivatives, fields [Nx,Ny,Nz,4] = (value, d/dx, d/dy, d/dz). */
@@ -191,6 +267,18 @@ wv_status wv_transform_to_spatial_domain_with_g_all_derivatives_device(wv_kernel
  * stage combinations are fused into the flux projection. */
 wv_status wv_rk4_advance_device(wv_kernel* kernel, double* Ap, double* Am, double* A0, double* t, double t0,
                                 double h, int nsteps, void* stream);
+
+/* WVFixedStepRK4::advanceToTime (WVRungeKutta.cpp:464-480): steps of size
+ * min(h, finalTime - t) while t < finalTime (the last one clipped), then
+ * t = finalTime.  finalTime below *t or not finite: WV_INVALID_CONFIGURATION.
+ * Each step is committed by its last launch (WVRungeKutta.cpp:441): if a step
+ * fails, the state and *t are those after the last whole step. */
+wv_status wv_rk4_advance_to_time_device(wv_kernel* kernel, double* Ap, double* Am, double* A0, double* t, double t0,
+                                        double finalTime, double h, void* stream);
+
+/* WVFixedStepRK4 metrics (WVRungeKutta.hpp): steps taken on this handle and
+ * the size of the last one. */
+wv_status wv_rk4_counters(const wv_kernel* kernel, uint64_t* stepCount, double* lastStepSize);
 
 /* Total energy sum(ApmTE*(|Ap|^2+|Am|^2) + A0TE*|A0|^2), FP64 reduction
  * (@WVTransformConstantStratification/WVTransformConstantStratification.m:374-376). */
@@ -235,6 +323,12 @@ wv_status wv_set_stage_instrumentation(wv_kernel* kernel, int enabled);
 /* Copies up to `capacity` stage records; returns the number of stages. */
 int wv_stage_timings(wv_kernel* kernel, wv_stage_timing* out, int capacity);
 void wv_reset_stage_timings(wv_kernel* kernel);
+
+/* Test hook, the counterpart of the reference tests' failing FFT engine
+ * (PortableRuntime/tests/TestWVForcingAndRK4.cpp:313-320): with n > 0, the
+ * n-th next kernel launch of the flux pipeline is not made and its call
+ * fails with WV_FFT_EXECUTION_FAILURE; n = 0 disarms it. */
+wv_status wv_debug_inject_launch_failure(wv_kernel* kernel, long n);
 
 #ifdef __cplusplus
 }

@@ -157,7 +157,9 @@ class HydrostaticOracle:
         p = np.exp(1j * self.omega * (t - t0))
         return Ap * np.conj(p), Am * p, A0
 
-    def nonlinear_flux(self, Ap, Am, A0, t=0.0, t0=0.0):
+    def spatial_products(self, Ap, Am, A0, t=0.0, t0=0.0):
+        """The advection products (Fu, Fv, Feta) [Nz x Ny x Nx] before the
+        projection (WVNonlinearAdvection.m:103-107)."""
         u, v, w, n = self._spectra(Ap, Am, A0, t - t0)
         Su, Sv = self._to_grid(u, self.F), self._to_grid(v, self.F)
         Sw, Sn = self._to_grid(w, self.G), self._to_grid(n, self.G)
@@ -168,7 +170,10 @@ class HydrostaticOracle:
         Fu = -(U * self._dx(Su) + V * self._dy(Su) + W * dzF(U))
         Fv = -(U * self._dx(Sv) + V * self._dy(Sv) + W * dzF(V))
         Fn = -(U * self._dx(Sn) + V * self._dy(Sn) + W * (dzG(Nn) + Nn * dl))
-        return self.forward(Fu, Fv, Fn, t, t0)
+        return Fu, Fv, Fn
+
+    def nonlinear_flux(self, Ap, Am, A0, t=0.0, t0=0.0):
+        return self.forward(*self.spatial_products(Ap, Am, A0, t, t0), t, t0)
 
     def rk4(self, Ap, Am, A0, t, dt, nsteps, t0=0.0):
         y = [np.array(a, np.complex128) for a in (Ap, Am, A0)]

@@ -101,6 +101,12 @@ class _Lib:
                 getattr(lib, p + fn).argtypes = [C.c_void_p, _P, C.c_double, C.c_double, _P, _P, _P]
             getattr(lib, p + "fg_all_derivatives").restype = C.c_int
             getattr(lib, p + "fg_all_derivatives").argtypes = [C.c_void_p, C.c_int, _P, _P, _P]
+            getattr(lib, p + "nonlinear_flux_using_advection_fields").restype = C.c_int
+            getattr(lib, p + "nonlinear_flux_using_advection_fields").argtypes = (
+                [C.c_void_p, C.c_double, C.c_double] + [_P] * 7)
+            getattr(lib, p + "state_field_derivatives").restype = C.c_int
+            getattr(lib, p + "state_field_derivatives").argtypes = (
+                [C.c_void_p, C.c_double, C.c_double, _P, _P, _P, C.c_int, _P])
             if hasattr(lib, p + "advect_f_grid_scalar"):
                 getattr(lib, p + "advect_f_grid_scalar").restype = C.c_int
                 getattr(lib, p + "advect_f_grid_scalar").argtypes = [C.c_void_p, _P, _P, C.c_int, _P]
@@ -169,6 +175,24 @@ class _Lib:
         if rc:
             raise RuntimeError(f"nonlinear_flux failed: {rc}")
         return Fp, Fm, F0
+
+    def nonlinear_flux_using_advection_fields(self, Ap, Am, A0, uvw, t=0.0, t0=0.0):
+        Ap, Am, A0 = self._coef(Ap), self._coef(Am), self._coef(A0)
+        Nx, Ny, Nz = self.spatial
+        uvw = np.ascontiguousarray(uvw, dtype=np.float64).reshape(3, Nz, Ny, Nx)
+        Fp, Fm, F0 = self._coef(), self._coef(), self._coef()
+        rc = self._fn("nonlinear_flux_using_advection_fields")(self.h, t, t0, *map(_ptr, (Ap, Am, A0, uvw, Fp, Fm, F0)))
+        if rc:
+            raise RuntimeError(f"nonlinear_flux_using_advection_fields failed: {rc}")
+        return Fp, Fm, F0
+
+    def state_field_derivatives(self, Ap, Am, A0, field, t=0.0, t0=0.0):
+        Ap, Am, A0 = self._coef(Ap), self._coef(Am), self._coef(A0)
+        out = self._fields(3)
+        rc = self._fn("state_field_derivatives")(self.h, t, t0, *map(_ptr, (Ap, Am, A0)), int(field), _ptr(out))
+        if rc:
+            raise RuntimeError(f"state_field_derivatives failed: {rc}")
+        return out
 
     def inverse_uvweta(self, Ap, Am, A0, t=0.0, t0=0.0):
         Ap, Am, A0 = self._coef(Ap), self._coef(Am), self._coef(A0)

@@ -160,6 +160,30 @@ int wvr_fg_all_derivatives(void* h, int fFamily, const double* Apm, const double
     return static_cast<int>(st.code);
 }
 
+// nonlinearFluxUsingAdvectionFields (...Kernel.cpp:1160-1171).
+int wvr_nonlinear_flux_using_advection_fields(void* h, double t, double t0, const double* Ap, const double* Am,
+                                              const double* A0, const double* uvw, double* Fp, double* Fm,
+                                              double* F0) {
+    auto& k = *static_cast<Handle*>(h)->kernel;
+    const auto s = k.descriptor().spectralShape();
+    const auto sp = k.descriptor().spatialShape();
+    WVState state{t, t0, {cview(Ap, s), cview(Am, s), cview(A0, s)}};
+    WVFlux flux{mview(Fp, s), mview(Fm, s), mview(F0, s)};
+    WVRealFieldBundleConstView adv{uvw, {sp.first, sp.second, sp.third, 3}};
+    return static_cast<int>(k.nonlinearFluxUsingAdvectionFields(state, flux, adv).code);
+}
+
+// transformStateFieldDerivatives (...Kernel.cpp:719-742).
+int wvr_state_field_derivatives(void* h, double t, double t0, const double* Ap, const double* Am, const double* A0,
+                                int field, double* out) {
+    auto& k = *static_cast<Handle*>(h)->kernel;
+    const auto s = k.descriptor().spectralShape();
+    const auto sp = k.descriptor().spatialShape();
+    WVState state{t, t0, {cview(Ap, s), cview(Am, s), cview(A0, s)}};
+    WVRealFieldBundleView d{out, {sp.first, sp.second, sp.third, 3}};
+    return static_cast<int>(k.transformStateFieldDerivatives(state, static_cast<WVDynamicalField>(field), d).code);
+}
+
 // advectFGridScalar (...Kernel.cpp:1032-1138): rhs = -(u, v, w) . grad(scalar)
 // of an F-grid scalar, optionally antialiased.
 int wvr_advect_f_grid_scalar(void* h, const double* scalar, const double* uvw, int antialias, double* rhs) {

@@ -525,8 +525,11 @@ struct Oracle {
         }
     }
 
-    // nonlinearFluxImpl :1173-1244
-    void flux(double t, double t0, const cplx* Ap, const cplx* Am, const cplx* A0, cplx* Fp, cplx* Fm, cplx* F0) const {
+    // nonlinearFluxImpl :1173-1244; uvwGiven (nonlinearFluxUsingAdvectionFields,
+    // :1160-1171): the advecting fields of the products instead of the
+    // synthesized ones.
+    void flux(double t, double t0, const cplx* Ap, const cplx* Am, const cplx* A0, cplx* Fp, cplx* Fm, cplx* F0,
+              const double* uvwGiven = nullptr) const {
         const double elapsed = t - t0;
         std::vector<cplx> ph(M);
         std::vector<Evolved> ev(M);
@@ -536,7 +539,8 @@ struct Oracle {
             ev[i] = evolved(Ap, Am, A0, i, ph[i]);
         }
         std::vector<double> uvw(3 * R), d(3 * R);
-        synthesizeUVW(ev, uvw.data());
+        if (uvwGiven) std::copy(uvwGiven, uvwGiven + 3 * R, uvw.begin());
+        else synthesizeUVW(ev, uvw.data());
         const int hydroTargets[] = {0, 1, 3}, nonhydroTargets[] = {0, 1, 2, 3};
         const int nt = c.isHydrostatic ? 3 : 4;
         const int* targets = c.isHydrostatic ? hydroTargets : nonhydroTargets;
@@ -707,6 +711,25 @@ long wvo_table(void* h, const char* name, double* out) {
 
 int wvo_nonlinear_flux(void* h, double t, double t0, const double* Ap, const double* Am, const double* A0, double* Fp, double* Fm, double* F0) {
     O(h)->flux(t, t0, C(Ap), C(Am), C(A0), C(Fp), C(Fm), C(F0));
+    return 0;
+}
+
+int wvo_nonlinear_flux_using_advection_fields(void* h, double t, double t0, const double* Ap, const double* Am,
+                                              const double* A0, const double* uvw, double* Fp, double* Fm,
+                                              double* F0) {
+    O(h)->flux(t, t0, C(Ap), C(Am), C(A0), C(Fp), C(Fm), C(F0), uvw);
+    return 0;
+}
+
+// transformStateFieldDerivatives :719-742 (phase, then the derivative
+// synthesis of :907-945 for one target).
+int wvo_state_field_derivatives(void* h, double t, double t0, const double* Ap, const double* Am, const double* A0,
+                                int field, double* out) {
+    Oracle* o = O(h);
+    if (field < 0 || field > 3) return 1;
+    std::vector<Oracle::Evolved> ev(o->M);
+    for (size_t i = 0; i < o->M; ++i) ev[i] = o->evolved(C(Ap), C(Am), C(A0), i, phase(o->omega[i] * (t - t0)));
+    o->synthesizeDerivatives(field, ev, out);
     return 0;
 }
 

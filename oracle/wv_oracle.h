@@ -21,6 +21,9 @@
  *   wvo_forward_uvweta / wvo_forward_uveta pinned: oracle/_ref golden coefficients,
  *                                         forward(inverse(A)) = A round trip
  *   wvo_fg_all_derivatives ............. pinned: oracle/_ref golden fields
+ *   wvo_nonlinear_flux_using_advection_fields, wvo_state_field_derivatives
+ *                                        pinned: oracle/_ref golden vectors
+ *                                         (tests/golden/contract_*.npz)
  *   wvo_rk4_advance .................... pinned: linear test y'=lambda*y closed
  *                                         form (via wvo_rk4_linear_test) and the
  *                                         reference step arithmetic
@@ -58,6 +61,13 @@ double wvo_coriolis(void* h);
 
 int wvo_nonlinear_flux(void* h, double t, double t0, const double* Ap, const double* Am,
                        const double* A0, double* Fp, double* Fm, double* F0);
+/* nonlinearFluxUsingAdvectionFields: the products use uvw [Nx,Ny,Nz,3]. */
+int wvo_nonlinear_flux_using_advection_fields(void* h, double t, double t0, const double* Ap, const double* Am,
+                                              const double* A0, const double* uvw, double* Fp, double* Fm,
+                                              double* F0);
+/* transformStateFieldDerivatives: d/dx, d/dy, d/dz of field (0 u,1 v,2 w,3 eta), [Nx,Ny,Nz,3]. */
+int wvo_state_field_derivatives(void* h, double t, double t0, const double* Ap, const double* Am, const double* A0,
+                                int field, double* out);
 int wvo_inverse_uvweta(void* h, double t, double t0, const double* Ap, const double* Am,
                        const double* A0, double* fields);
 int wvo_inverse_uvw(void* h, double t, double t0, const double* Ap, const double* Am,

@@ -165,15 +165,40 @@ def scalar_cases():
         write("scalar_" + name, dict(cfg_arrays(cfg), scalar=scalar, uvw=uvw, rhs=rhs, rhs_antialiased=rhs_aa))
 
 
+# transformStateFieldDerivatives and nonlinearFluxUsingAdvectionFields
+# fixtures (the advecting fields are seeded draws, not the state's own)
+CONTRACT = [("k8x6x7_nh_noaa", 8, 6, 7, 5, False, False), ("k6x5x7_h", 6, 5, 7, 4, True, True),
+            ("k16x16x9_nh", 16, 16, 9, 5, False, True)]
+
+
+def contract_cases():
+    for i, (name, Nx, Ny, Nz, Nj, hydro, aa) in enumerate(CONTRACT):
+        cfg = make_config(Nx, Ny, Nz, Nj, Lx=15e3, Ly=12e3, isHydrostatic=hydro, shouldAntialias=aa)
+        r = Reference(cfg)
+        rng = np.random.default_rng(9000 + i)
+        Ap, Am, A0 = rand_coef(rng, r)
+        t, t0 = 21.0, 3.0
+        deriv = np.stack([r.state_field_derivatives(Ap, Am, A0, f, t=t, t0=t0) for f in range(4)])
+        uvw = rng.standard_normal((3, Nz, Ny, Nx)) * 0.1
+        Fp, Fm, F0 = r.nonlinear_flux_using_advection_fields(Ap, Am, A0, uvw, t=t, t0=t0)
+        write("contract_" + name, dict(cfg_arrays(cfg), t=np.array(t), t0=np.array(t0), Ap=Ap, Am=Am, A0=A0,
+                                       derivatives=deriv, uvw=uvw, Fp=Fp, Fm=Fm, F0=F0))
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--c0", action="store_true", help="also write the 64x64x33 reference flux (slow)")
     ap.add_argument("--scalar", action="store_true", help="write only the advectFGridScalar fixtures")
+    ap.add_argument("--contract", action="store_true",
+                    help="write only the transformStateFieldDerivatives / UsingAdvectionFields fixtures")
     args = ap.parse_args()
     if not Reference.available():
         sys.exit("oracle/_ref/libwvref.so is missing: run `make -C oracle ref` where the reference tree exists")
     if args.scalar:
         scalar_cases()
+        sys.exit(0)
+    if args.contract:
+        contract_cases()
         sys.exit(0)
     tiny_cases()
     descriptor_cases()

@@ -84,3 +84,29 @@ def test_uniform_counter_rng_independent_of_length():
     assert np.array_equal(flows.uniform(5, 1, 10), flows.uniform(5, 1, 1000)[:10])
     u = flows.uniform(5, 1, 100000)
     assert 0 <= u.min() and u.max() < 1 and abs(u.mean() - 0.5) < 0.01
+
+
+def test_new_entry_points_reject_null_kernel_without_gpu_work():
+    lib = wvamd.library()
+    assert lib.wv_scratch_bytes(None) == 0 and lib.wv_phase_reservation_bytes(None) == 0
+    m = wvamd.KernelMetrics()
+    assert lib.wv_metrics(None, ctypes.byref(m)) == 3
+    for name in wvamd.IDENTIFIERS:
+        assert getattr(lib, "wv_" + name)(None) is None
+    assert lib.wv_prepare_scalar_advection(None) == 3
+    assert lib.wv_nonlinear_flux_using_advection_fields_device(None, 0.0, 0.0, *([None] * 8)) == 3
+    assert lib.wv_transform_state_field_derivatives_device(None, 0.0, 0.0, None, None, None, 0, None, None) == 3
+    t = ctypes.c_double(0.0)
+    assert lib.wv_rk4_advance_to_time_device(None, None, None, None, ctypes.byref(t), 0.0, 1.0, 0.1, None) == 3
+    assert lib.wv_rk4_counters(None, None, None) == 3
+    assert lib.wv_debug_inject_launch_failure(None, 1) == 3
+    e = ctypes.c_double()
+    assert lib.wv_total_energy_device(None, None, None, None, ctypes.byref(e), None) == 3
+
+
+def test_metrics_struct_mirrors_reference_members():
+    # WVKernelMetrics (...Kernel.hpp:12-45): 18 counters then 14 timings
+    names = [n for n, _ in wvamd.KernelMetrics._fields_]
+    assert names[0] == "descriptorBytes" and names[8] == "planCount" and names[17] == "bytesCopied"
+    assert names[18] == "phaseSeconds" and names[-1] == "scalarAntialiasSeconds" and len(names) == 32
+    assert ctypes.sizeof(wvamd.KernelMetrics) == 32 * 8

@@ -104,24 +104,43 @@ def test_stratified_rk4_steps_match_oracle(torch_gpu):
         assert t_gpu == t
 
 
-def test_c3_shape_512x512x65_runs_and_mixed_tracks_fp64(torch_gpu):
-    """The C3 workload shape (512x512, 65 quadrature points, 64 modes,
-    exponential N^2): one FP64 and one mixed flux from a seeded flow agree to
-    the mixed tolerance, and the FP64 flux is finite."""
+def test_c3_512x512x65_flux_and_100_steps(torch_gpu):
+    """The arbitrary-stratification workload (512 x 512, 65 quadrature points,
+    64 modes, N^2 = N0^2 exp(2z/b), |k|^-2 waves + |k|^-3 geostrophic flow at
+    rms 0.1 m/s, dt = 0.5 CFL): one FP64 flux against the numpy oracle at
+    1e-12, then 100 FP64 and 100 mixed RK4 steps with the mixed total energy
+    within 1e-6 of FP64 at every 10th step."""
+    import time
     import wvamd
-    from wvamd import flows
-    from wvamd import stratification as S
-    cfg = wvamd.make_config(512, 512, 65, 64, Lx=400e3, Lz=4000.0, isHydrostatic=True)
-    st = S.Stratification(*S.exponential()[:1], Lz=4000.0, g=cfg["g"], dN2=S.exponential()[1])
-    ops = S.vertical_operators(st, 65, 64)
-    k64 = _kernel(cfg, ops)
+    from hydrostatic import HydrostaticOracle
+    from oracle import Oracle
+    from wvamd.workloads import arbitrary_stratification_512
+    cfg, ops, k64, A, dt = arbitrary_stratification_512(wvamd)
     k32 = _kernel(cfg, ops, "mixed")
-    md = k64.modes()
-    A = flows.random_flow_shape(md["Kh"], 64, k64.coefficient_table("ApmTE"), k64.coefficient_table("A0TE"),
-                                seed=11, jmax=64)
-    F64 = _host(k64.nonlinear_flux(*[k64.to_device(a) for a in A], t=0.0))
-    F32 = _host(k32.nonlinear_flux(*[k32.to_device(a) for a in A], t=0.0))
-    assert all(np.all(np.isfinite(x)) for x in F64)
-    err = rel_err_joint(F32, F64)
-    print(f"[gpu] 512x512x65 mixed vs fp64 flux rel err {err:.2e}", flush=True)
-    assert err <= MIXED_TOL
+    print(f"[gpu] C3 512x512x65 Nkl={k64.Nkl} dt={dt:.1f}s", flush=True)
+    dev = [k64.to_device(a) for a in A]
+    F = _host(k64.nonlinear_flux(*dev, t=0.0))
+    t0 = time.perf_counter()
+    ho = HydrostaticOracle(cfg, ops, Oracle(cfg).modes())
+    R = ho.nonlinear_flux(*A, t=0.0)
+    err = rel_err_joint(F, R)
+    print(f"[gpu] C3 fp64 flux vs oracle rel err {err:.2e} (oracle {time.perf_counter() - t0:.1f}s)", flush=True)
+    assert err <= FP64_TOL
+    del ho, R
+    d64 = [k64.to_device(a) for a in A]
+    d32 = [k32.to_device(a) for a in A]
+    e0 = k64.total_energy(*d64)
+    t = 0.0
+    worst = 0.0
+    for block in range(10):
+        t64 = k64.rk4_advance(*d64, t=t, h=dt, nsteps=10)
+        t32 = k32.rk4_advance(*d32, t=t, h=dt, nsteps=10)
+        assert t64 == t32
+        t = t64
+        e64, e32 = k64.total_energy(*d64), k64.total_energy(*d32)
+        rel = abs(e32 - e64) / abs(e64)
+        worst = max(worst, rel)
+        assert np.isfinite(e64) and rel <= 1e-6, (block, rel)
+    print(f"[gpu] C3 100 steps: E0={e0:.6e} E={e64:.6e} mixed-vs-fp64 worst energy rel err {worst:.2e}", flush=True)
+    print(f"[gpu] C3 mixed-vs-fp64 coefficient rel err "
+          f"{rel_err_joint([d.cpu().numpy() for d in d32], [d.cpu().numpy() for d in d64]):.2e}", flush=True)

@@ -110,6 +110,45 @@ def test_rk4_fp64_64x64x33_50_steps_per_step(torch_gpu):
     print(f"[gpu] 64x64x33 50 steps worst per-step rel err {worst:.2e} (dt={h:.3f}s)", flush=True)
 
 
+def test_rk4_fp64_256x256x129_flux_and_increments_match_oracle(torch_gpu):
+    """The 256x256x129 workload (seed 4001, j <= 64, rms 5 cm/s, dt = 0.5
+    CFL): one FP64 flux against the oracle at 1e-12, then the per-step
+    increments y_{n+1} - y_n of steps 1-5 and 196-200, each from the GPU's own
+    state at step n, against the oracle's increment from that state at 1e-12
+    of max|increment| (an increment is ~100x more sensitive than the state)."""
+    cfg = _cfg(256, 129, 200e3)
+    k = _kernel(cfg)
+    A = _flow(k, seed=4001, jmax=64)
+    h = _dt(k, A)
+    o = Oracle(cfg)
+    Oracle.set_threads(16)
+    F = [f.cpu().numpy() for f in k.nonlinear_flux(*[k.to_device(a) for a in A], t=0.0)]
+    print("[gpu] oracle 256x256x129 flux (CPU) ...", flush=True)
+    err = rel_err_joint(F, o.nonlinear_flux(*A, t=0.0))
+    print(f"[gpu] 256x256x129 flux rel err {err:.2e}", flush=True)
+    assert err <= 1e-12
+    dev = [k.to_device(a) for a in A]
+    t = 0.0
+
+    def check(step):
+        nonlocal t
+        y0 = [d.cpu().numpy() for d in dev]
+        t_new = k.rk4_advance(*dev, t=t, h=h, nsteps=1)
+        y1 = [d.cpu().numpy() for d in dev]
+        ref = o.rk4(*y0, t=t, dt=h, nsteps=1)
+        assert ref[3] == t_new
+        e = rel_err_joint([a - b for a, b in zip(y1, y0)], [a - b for a, b in zip(ref[:3], y0)])
+        print(f"[gpu] 256x256x129 step {step} increment rel err {e:.2e}", flush=True)
+        assert e <= 1e-12, (step, e)
+        t = t_new
+
+    for step in range(1, 6):
+        check(step)
+    t = k.rk4_advance(*dev, t=t, h=h, nsteps=190)
+    for step in range(196, 201):
+        check(step)
+
+
 def test_rk4_fp64_256x256x129_one_step_matches_oracle(torch_gpu):
     cfg = _cfg(256, 129, 200e3)
     k = _kernel(cfg)

@@ -10,7 +10,10 @@ Pins:
                   hydrostatic flux (golden vectors) and the C++ oracle's
   exponential N^2 N0^2 exp(2z/b): eigenvalues from the closed-form Bessel
                   determinant J0(sD) Y0(s0) - J0(s0) Y0(sD) = 0, s = (N0 b/c) e^{z/b},
-                  and mode shapes G ~ J0(s) Y0(s0) - Y0(s) J0(s0)
+                  and mode shapes G ~ J0(s) Y0(s0) - Y0(s) J0(s0); the
+                  hydrostatic oracle's advection products (incl. the eta dLnN2
+                  term, dLnN2 = 2/b) equal the closed-form products of a
+                  Bessel-mode flow
   operators       PF PFinv = I, QG QGinv = I, the quadrature weights integrate
                   F_j to Lz delta_j0 (verticalProjectionOperatorsWithRigidLid.m:35-37);
                   the dense diffZF/diffZG operators applied to synthesized
@@ -143,3 +146,127 @@ def test_hydrostatic_oracle_forward_inverts_inverse():
     f = ho.fields(*A, t=250.0)
     B = ho.forward(f[0], f[1], f[3], t=250.0)
     assert rel_err_joint(B, A) <= 1e-12
+
+
+def _bessel_mode(N0, b, Lz, h_guess):
+    """Closed-form rigid-lid mode of N^2 = N0^2 exp(2z/b): the eigen speed c
+    (root of J0(sD) Y0(s0) - J0(s0) Y0(sD)), and callables G(z), G'(z), G''(z)
+    of G = J0(s) Y0(s0) - Y0(s) J0(s0), s = (N0 b / c) e^{z/b} (ds/dz = s/b,
+    J0' = -J1, J1' = J0 - J1/s)."""
+    from scipy.special import j1, y1
+
+    def det(c):
+        s0 = N0 * b / c
+        sD = s0 * np.exp(-Lz / b)
+        return j0(sD) * y0(s0) - j0(s0) * y0(sD)
+
+    c0 = np.sqrt(G_ * h_guess)
+    c = brentq(det, 0.999 * c0, 1.001 * c0, xtol=1e-18)
+    s0 = N0 * b / c
+
+    def G(z):
+        s = s0 * np.exp(z / b)
+        return j0(s) * y0(s0) - y0(s) * j0(s0)
+
+    def dG(z):
+        s = s0 * np.exp(z / b)
+        return -(s / b) * (j1(s) * y0(s0) - y1(s) * j0(s0))
+
+    def d2G(z):
+        s = s0 * np.exp(z / b)
+        return -(s / b) ** 2 * G(z)
+
+    return c * c / G_, G, dG, d2G
+
+
+def test_exponential_n2_advection_products_match_bessel_closed_form():
+    """Pins the arbitrary-N^2 flux of oracle/hydrostatic.py before projection:
+    a flow of j = 1 and j = 2 modes of N^2 = N0^2 exp(2z/b) at three
+    wavevectors, whose u, v, w, eta and their x, y, z derivatives are written
+    in closed form (Bessel functions), gives
+        Fu = -(u u_x + v u_y + w u_z),  Fv = -(u v_x + v v_y + w v_z),
+        Feta = -(u eta_x + v eta_y + w (eta_z + eta dLnN2)),  dLnN2 = 2/b
+    (Forcing/WVNonlinearAdvection.m:103-107), with the vertical structure
+    and its derivatives from the Bessel solution, not from the operators.
+    Observed agreement is 1e-11 (checked at 1e-9); a sign flip of the
+    eta dLnN2 term (a mistake planted in the oracle's input) misses by 1.8."""
+    from hydrostatic import HydrostaticOracle
+    from oracle import Oracle, make_config
+    N0, b, Lz = 5.2e-3, 1300.0, 4000.0
+    Nx, Ny, Nz, Nj = 16, 16, 65, 12
+    cfg = make_config(Nx, Ny, Nz, Nj, Lx=100e3, Ly=80e3, Lz=Lz, isHydrostatic=True)
+    N2f, dN2f = S.exponential(N0, b)
+    st = S.Stratification(N2f, Lz=Lz, g=G_, dN2=dN2f)
+    ops = S.vertical_operators(st, Nz, Nj)
+    z = ops["z"]
+    # d ln N^2 / dz of the exponential is exactly 2/b
+    assert np.max(np.abs(ops["dLnN2"] - 2.0 / b)) <= 1e-13 * (2.0 / b)
+    md = Oracle(cfg).modes()
+    ho = HydrostaticOracle(cfg, ops, md)
+
+    # closed-form F_j = h_j G_j', G_j, with the operators' normalization and sign
+    Fcl, Gcl, dFcl, dGcl = {}, {}, {}, {}
+    for j in (1, 2):
+        h, G, dG, d2G = _bessel_mode(N0, b, Lz, st.h[j - 1])
+        assert abs(h - ops["h"][j]) <= 1e-10 * h
+        Gop = ho.G[:, j]
+        alpha = (Gop @ G(z)) / (G(z) @ G(z))
+        Gcl[j], dGcl[j] = alpha * G(z), alpha * dG(z)
+        Fcl[j], dFcl[j] = h * dGcl[j], h * alpha * d2G(z)
+        assert np.max(np.abs(Gop - Gcl[j])) <= 1e-6 * np.max(np.abs(Gcl[j])), j
+        assert np.max(np.abs(ho.F[:, j] - Fcl[j])) <= 1e-6 * np.max(np.abs(Fcl[j])), j
+        # d/dz F = -(N^2/g) G: the Bessel equation, so the closed forms agree
+        assert np.max(np.abs(dFcl[j] + N2f(z) / G_ * Gcl[j])) <= 1e-12 * np.max(np.abs(dFcl[j]))
+
+    # a state of two waves, a geostrophic mode and a mean density anomaly
+    # (K = 0, horizontally uniform eta: it enters Feta only through
+    # w (eta_z + eta dLnN2)), each scaled to a physical amplitude
+    km, lm = md["kMode"], md["lMode"]
+    pick = [int(np.flatnonzero((km == a) & (lm == c))[0]) for a, c in ((1, 0), (1, 2), (0, 1), (0, 0))]
+    parts = []
+    for arr, im, j, phase, target in ((0, pick[0], 1, 0.3, ("u", 0.05)), (1, pick[1], 2, 1.1, ("u", 0.04)),
+                                      (2, pick[2], 1, -0.7, ("u", 0.02)), (2, pick[3], 2, 0.0, ("eta", 30.0))):
+        A = [np.zeros((ho.Nkl, Nj), complex) for _ in range(3)]
+        A[arr][im, j] = np.exp(1j * phase)
+        f = ho.fields(*A, t=0.0)
+        A[arr] *= target[1] / np.max(np.abs(f[0 if target[0] == "u" else 3]))
+        parts.append(A)
+    Ap, Am, A0 = (sum(p[a] for p in parts) for a in range(3))
+    t = 500.0
+    u, v, w, n = ho._spectra(Ap, Am, A0, t)
+
+    x = np.arange(Nx) * cfg["Lx"] / Nx
+    y = np.arange(Ny) * cfg["Ly"] / Ny
+    X, Y = np.meshgrid(x, y)  # [Ny, Nx]
+
+    def synth(spec, prof, dk=0.0, dl=0.0):
+        """sum over the picked modes of spec e^{i(kx+ly)} + c.c. (the real part
+        alone at K = 0) times the vertical profile."""
+        out = np.zeros((Nz, Ny, Nx))
+        for im in pick:
+            for j in (1, 2):
+                if spec[im, j] == 0:
+                    continue
+                k, l = md["k"][im], md["l"][im]
+                fac = (1j * k) ** dk * (1j * l) ** dl
+                wgt = 1.0 if (km[im] == 0 and lm[im] == 0) else 2.0
+                plane = wgt * np.real(fac * spec[im, j] * np.exp(1j * (k * X + l * Y)))
+                out += prof[j][:, None, None] * plane[None]
+        return out
+
+    U, Ux, Uy, Uz = synth(u, Fcl), synth(u, Fcl, 1), synth(u, Fcl, 0, 1), synth(u, dFcl)
+    V, Vx, Vy, Vz = synth(v, Fcl), synth(v, Fcl, 1), synth(v, Fcl, 0, 1), synth(v, dFcl)
+    W = synth(w, Gcl)
+    E, Ex, Ey, Ez = synth(n, Gcl), synth(n, Gcl, 1), synth(n, Gcl, 0, 1), synth(n, dGcl)
+    Fu = -(U * Ux + V * Uy + W * Uz)
+    Fv = -(U * Vx + V * Vy + W * Vz)
+    Fe = -(U * Ex + V * Ey + W * (Ez + E * (2.0 / b)))
+    # the eta dLnN2 term is an O(1) part of Feta for this flow
+    assert np.max(np.abs(W * E * (2.0 / b))) > 0.2 * np.max(np.abs(Fe))
+
+    mine = ho.spatial_products(Ap, Am, A0, t)
+    for a, ref, name in zip(mine, (Fu, Fv, Fe), ("Fu", "Fv", "Feta")):
+        err = np.max(np.abs(a - ref)) / np.max(np.abs(ref))
+        assert err <= 1e-9, (name, err)
+    wrong = HydrostaticOracle(cfg, dict(ops, dLnN2=-ops["dLnN2"]), md).spatial_products(Ap, Am, A0, t)
+    assert np.max(np.abs(wrong[2] - Fe)) / np.max(np.abs(Fe)) > 0.1

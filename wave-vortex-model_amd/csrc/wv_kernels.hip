@@ -142,52 +142,41 @@ __device__ __forceinline__ int xcdBlock(int b, int nblocks) {
 }
 
 
-// Coefficient tables of the evolved field spectra.
-struct FieldTables {
-    const double* omega;
-    const cx<double>* UAp;
-    const cx<double>* VAp;
-    const cx<double>* WAp;
-    const double* NAp;
-    const cx<double>* UA0;
-    const cx<double>* VA0;
-    const double* NA0;
-};
 
 // Phase p = exp(i omega elapsed) (phase(), WVCoefficientFormulas.hpp:15).
 __device__ __forceinline__ cx<double> phaseAt(double omega, double elapsed) {
     double sn, cs;
-    sincos(omega * elapsed, &sn, &cs);
+    sincos(__dmul_rn(omega, elapsed), &sn, &cs);  // the angle rounded on its own in every kernel
     return cx<double>{cs, sn};
 }
 
 // Evolved field spectra (u, v, w, eta) of coefficient idx with phase p
 // (WVCoefficientFormulas.hpp:15-33), stored to FV[f*M + idx].
+// Every operation is rounded on its own (no contraction into fused
+// multiply-adds), so the spectra are the same bits in every kernel that
+// forms them (k_fields from memory, k_stage from the stage it produced).
 template <typename T>
 __device__ __forceinline__ void storeFieldSpectra(cx<double> UAp, cx<double> VAp, cx<double> WAp, double NAp,
                                                   cx<double> UA0, cx<double> VA0, double NA0, int idx, int M,
                                                   cx<double> Ap, cx<double> Am, cx<double> A0, cx<double> p,
                                                   cx<T>* __restrict__ FV) {
-    const cx<double> ap = cmul(Ap, p), am = cmul(Am, cconj(p)), a0 = A0;
-    const cx<double> u = cadd(cadd(cmul(UAp, ap), cmul(cconj(UAp), am)), cmul(UA0, a0));
-    const cx<double> v = cadd(cadd(cmul(VAp, ap), cmul(cconj(VAp), am)), cmul(VA0, a0));
-    const cx<double> w = cadd(cmul(WAp, ap), cmul(WAp, am));
-    const cx<double> e = cadd(cadd(cscale(ap, NAp), cscale(am, -NAp)), cscale(a0, NA0));
+    auto mul = [](cx<double> a, cx<double> b) {
+        return cx<double>{__dsub_rn(__dmul_rn(a.re, b.re), __dmul_rn(a.im, b.im)),
+                          __dadd_rn(__dmul_rn(a.re, b.im), __dmul_rn(a.im, b.re))};
+    };
+    auto add = [](cx<double> a, cx<double> b) { return cx<double>{__dadd_rn(a.re, b.re), __dadd_rn(a.im, b.im)}; };
+    auto scale = [](cx<double> a, double s) { return cx<double>{__dmul_rn(a.re, s), __dmul_rn(a.im, s)}; };
+    const cx<double> ap = mul(Ap, p), am = mul(Am, cconj(p)), a0 = A0;
+    const cx<double> u = add(add(mul(UAp, ap), mul(cconj(UAp), am)), mul(UA0, a0));
+    const cx<double> v = add(add(mul(VAp, ap), mul(cconj(VAp), am)), mul(VA0, a0));
+    const cx<double> w = add(mul(WAp, ap), mul(WAp, am));
+    const cx<double> e = add(add(scale(ap, NAp), scale(am, -NAp)), scale(a0, NA0));
     FV[idx] = {T(u.re), T(u.im)};
     FV[M + idx] = {T(v.re), T(v.im)};
     FV[2 * M + idx] = {T(w.re), T(w.im)};
     FV[3 * M + idx] = {T(e.re), T(e.im)};
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_fields(const cx<double>* __restrict__ Ap, const cx<double>* __restrict__ Am,
-                                                const cx<double>* __restrict__ A0, FieldTables ft, double elapsed,
-                                                int M, cx<T>* __restrict__ FV) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= M) return;
-    storeFieldSpectra(ft.UAp[idx], ft.VAp[idx], ft.WAp[idx], ft.NAp[idx], ft.UA0[idx], ft.VA0[idx], ft.NA0[idx], idx,
-                      M, Ap[idx], Am[idx], A0[idx], phaseAt(ft.omega[idx], elapsed), FV);
-}
 
 // F- or G-family field spectrum Apm*waveScale + A0*geostrophicScale per
 // coefficient (fieldFamilyDerivativeSpectrum, WVCoefficientFormulas.hpp:60-73).
@@ -478,11 +467,19 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
                 for (int q = 0; q < NQ; ++q) {
                     mid[q] += __shfl_xor(mid[q], 16);
                     mid[q] += __shfl_xor(mid[q], 32);
+                    // i k and i l times the value: the partner lane n ^ 1
+                    // holds the other part (read with every lane active)
+                    const T v = mid[q] * oscale;
+                    const T other = pairSwap(v);
                     if (q0 + q >= nch) continue;
-                    const int oV = plan.g[gi].ch[q0 + q].outV;
-                    if (oV >= 0 && kq == 0 && store)
-                        reinterpret_cast<T*>(H)[oV * chStride + static_cast<size_t>(zmid) * zStride + colOff] =
-                            mid[q] * oscale;
+                    const SynthChannel& sc = plan.g[gi].ch[q0 + q];
+                    const size_t o = static_cast<size_t>(zmid) * zStride + colOff;
+                    const T rot = part ? other : -other;
+                    if (kq == 0 && store) {
+                        if (sc.outV >= 0) reinterpret_cast<T*>(H)[sc.outV * chStride + o] = v;
+                        if (sc.outX >= 0) reinterpret_cast<T*>(H)[sc.outX * chStride + o] = ks * rot;
+                        if (sc.outY >= 0) reinterpret_cast<T*>(H)[sc.outY * chStride + o] = ls * rot;
+                    }
                 }
             }
 #pragma unroll
@@ -809,11 +806,14 @@ struct XMap<3> {
 // transform lie in the zero band, so inputs r = 6 ... 9 of every butterfly
 // of the first radix-16 pass (entries jj + 16 r) are known zeros: neither
 // written nor read.
-template <typename T, int NT, int RB, int L2N, bool PRUNE>
+// EXT (nonlinearFluxUsingAdvectionFields, ...Kernel.cpp:1160-1171): the
+// advecting U, V, W of the products are the caller's FP64 fields adv
+// ([3][Nz][Ny][Nx]) instead of the synthesized ones.
+template <typename T, int NT, int RB, int L2N, bool PRUNE, bool EXT = false>
 __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict__ G, cx<T>* __restrict__ P,
                                                         int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
                                                         const cx<T>* __restrict__ twX, const T* __restrict__ kxv, const cx<T>* __restrict__ zeros,
-                                                        T etaScale) {
+                                                        T etaScale, const double* __restrict__ adv) {
     using XM = XMap<NT>;
     constexpr int PAIRS = XM::pairs;
     const int Nx = L2N > 0 ? 1 << L2N : NxArg;
@@ -918,7 +918,15 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
             ch[2 * p] = v.re;
             ch[2 * p + 1] = v.im;
         }
-        const T U = ch[XM::uvw[0]], V = ch[XM::uvw[1]], W = ch[XM::uvw[2]];
+        T U = ch[XM::uvw[0]], V = ch[XM::uvw[1]], W = ch[XM::uvw[2]];
+        if constexpr (EXT) {
+            const int row = row0 + rr;
+            const size_t R = static_cast<size_t>(nrows) * Nx;
+            const size_t i = static_cast<size_t>(row < nrows ? row : nrows - 1) * Nx + x;
+            U = static_cast<T>(adv[i]);
+            V = static_cast<T>(adv[R + i]);
+            W = static_cast<T>(adv[2 * R + i]);
+        }
         T F[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
         for (int t = 0; t < NT; ++t)
@@ -1409,6 +1417,23 @@ __device__ __forceinline__ CoefRow coefficients(const CoefConsts& c, int j, cons
     return r;
 }
 
+// Evolved field spectra of (Ap, Am, A0) at the given elapsed time, with the
+// coefficient tables formed by coefficients() -- the same values k_stage
+// forms when it emits the next evaluation's spectra, so that an RK4 run gives
+// the same bits however its steps are split over calls.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fields(const cx<double>* __restrict__ Ap, const cx<double>* __restrict__ Am,
+                                                const cx<double>* __restrict__ A0,
+                                                const ModeDev* __restrict__ modes, int Nj, CoefConsts cc,
+                                                double elapsed, int M, cx<T>* __restrict__ FV) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M) return;
+    const int wv = idx / Nj, j = idx - wv * Nj;
+    const CoefRow cr = coefficients(cc, j, modes[wv]);
+    storeFieldSpectra(cr.UAp, cr.VAp, cr.WAp, cr.NAp, cr.UA0, cr.VA0, cr.NA0, idx, M, Ap[idx], Am[idx], A0[idx],
+                      phaseAt(cr.om, elapsed), FV);
+}
+
 // Wave-vortex projection of the analysed products (projectFluxTargetMode,
 // ...Kernel.cpp) and the flux store or RK4 stage of WVFixedStepRK4::step,
 // one coefficient (j, mode) per thread; in RK modes with emitFields set it
@@ -1765,6 +1790,35 @@ struct KernelImpl {
     double retainedRowFraction = 1.0;
     size_t maxLdsPerBlock = 65536;
 
+    // counters of the reference's WVKernelMetrics (...Kernel.hpp:12-45)
+    wv_kernel_metrics metrics{};
+    uint64_t rkStepCount = 0;
+    double lastStepSize = 0.0;
+    bool scalarPrepared = false;
+
+    // Calls on one handle share its workspaces.  Each call records lastCall
+    // on its stream once its work is enqueued; a call on another stream
+    // first makes that stream wait for it, so calls stay ordered on the
+    // device whatever streams they come from.
+    hipEvent_t lastCall = nullptr;
+    hipStream_t lastStream = nullptr;
+    bool lastCallValid = false;
+    void orderAfterLastCall(hipStream_t s) {
+        if (lastCallValid && s != lastStream) hipCheck(hipStreamWaitEvent(s, lastCall, 0), "hipStreamWaitEvent");
+    }
+    void recordCall(hipStream_t s) {
+        if (!lastCall) hipCheck(hipEventCreateWithFlags(&lastCall, hipEventDisableTiming), "hipEventCreate");
+        hipCheck(hipEventRecord(lastCall, s), "hipEventRecord");
+        lastStream = s;
+        lastCallValid = true;
+    }
+
+    // Test hook (the counterpart of the reference tests' failing FFT engine,
+    // PortableRuntime/tests/TestWVForcingAndRK4.cpp:313-320): when set to
+    // n > 0, the n-th next timed launch is not made and throws
+    // WV_FFT_EXECUTION_FAILURE on the host.
+    long failAfterLaunches = 0;
+
     bool instrument = false;
     std::vector<StageRecord> records;
     struct Pending {
@@ -1774,14 +1828,31 @@ struct KernelImpl {
     };
     std::vector<Pending> pending;
 
-    size_t persistentBytes() const {
+    static size_t sumBytes(std::initializer_list<const DeviceBuffer*> bs) {
         size_t s = 0;
-        for (const DeviceBuffer* b : {&rowLimit, &lrow, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy, &ApmN,
-                                      &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &synC, &synS, &anaC,
-                                      &anaS, &synGz, &synFh, &synEz, &twX, &twY, &mj, &zeroRow, &kxv, &synCE, &synSE, &synCO, &synSO, &synCmE, &synSmE, &synCmO, &synSmO, &anaCE, &anaCO, &anaSE, &anaSO, &FV, &H, &G, &P, &Q, &Tg, &modeTab, &jTab, &stage, &weighted, &partial, &scalar})
-            s += b->bytes;
+        for (const DeviceBuffer* b : bs) s += b->bytes;
         return s;
     }
+    // descriptor tables and column / mode maps on the device
+    size_t descriptorBytes() const {
+        return sumBytes({&rowLimit, &lrow, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy,
+                         &ApmN, &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &fWaveScale, &gWaveScale,
+                         &Fg, &Gg, &modeTab, &jTab, &mainColumn, &compColumn, &scKeep});
+    }
+    // transform operators and twiddle tables (the "plans")
+    size_t planBytes() const {
+        size_t s = sumBytes({&synC, &synS, &anaC, &anaS, &synGz, &synFh, &synEz, &twX, &twY, &mj, &zeroRow, &kxv,
+                             &synCE, &synSE, &synCO, &synSO, &synCmE, &synSmE, &synCmO, &synSmO, &anaCE, &anaCO,
+                             &anaSE, &anaSO, &anaTC, &anaTS, &scD, &scTwX, &scTwY});
+        for (const DeviceBuffer& b : synT) s += b.bytes;
+        return s;
+    }
+    // workspaces: half spectra, coefficient work arrays, reductions
+    size_t scratchBytes() const {
+        return sumBytes({&FV, &H, &G, &P, &Q, &Tg, &stage, &weighted, &partial, &scalar, &scC, &Qs, &Gs, &colbuf,
+                         &rkWork});
+    }
+    size_t persistentBytes() const { return descriptorBytes() + planBytes() + scratchBytes(); }
 
     int recordIndex(const char* name) {
         for (size_t i = 0; i < records.size(); ++i)
@@ -1792,6 +1863,8 @@ struct KernelImpl {
 
     template <typename Launch>
     void timed(const char* name, double bytes, hipStream_t s, Launch&& launch) {
+        if (failAfterLaunches > 0 && --failAfterLaunches == 0)
+            throw Failure{WV_FFT_EXECUTION_FAILURE, std::string("Injected failure before launch: ") + name};
         if (!instrument) {
             launch();
             hipCheck(hipGetLastError(), name);
@@ -1935,7 +2008,13 @@ struct KernelImpl {
         NH = hydro ? 10 : 12;
         targetCode = hydro ? (0 | (1 << 4) | (3 << 8)) : (0 | (1 << 4) | (2 << 8) | (3 << 12));
         const size_t pairs = static_cast<size_t>(NT == 4 ? XMap<4>::pairs : XMap<3>::pairs);
-        if (pairs * cfg.Nx > static_cast<size_t>(kFftRegs) * 256 || static_cast<size_t>(KT) * cfg.Ny > static_cast<size_t>(kFftRegs) * 256)
+        // In-LDS FFTs hold at most kFftRegs * kFftThreads = 4096 points per
+        // workgroup: the fused x kernel `pairs` transforms of one row, the
+        // generic x transforms (forward, inverse, tracer) 8 of them, the y
+        // transforms KT columns of Ny.
+        constexpr size_t kXBatch = 8;
+        const size_t fftPoints = static_cast<size_t>(kFftRegs) * kFftThreads;
+        if (std::max(pairs, kXBatch) * cfg.Nx > fftPoints || static_cast<size_t>(KT) * cfg.Ny > fftPoints)
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx <= 512, Ny <= 4096)."};
         int maxLds = 0;
         hipCheck(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
@@ -2095,10 +2174,6 @@ struct KernelImpl {
                              d.stratified ? synFh.as<T>() : nullptr, d.stratified ? synEz.as<T>() : nullptr}};
     }
 
-    FieldTables fieldTables() const {
-        return FieldTables{omega.as<double>(), UAp.as<cx<double>>(), VAp.as<cx<double>>(), WAp.as<cx<double>>(),
-                           NAp.as<double>(),   UA0.as<cx<double>>(), VA0.as<cx<double>>(), NA0.as<double>()};
-    }
     ProjTables projTables() const {
         return ProjTables{omega.as<double>(), A0Vort.as<double>(),     A0Buoy.as<double>(),
                           NA0.as<double>(),   ApmN.as<double>(),       ApmDScaled.as<double>(),
@@ -2108,12 +2183,13 @@ struct KernelImpl {
     template <typename T>
     void launchFields(const double* Ap, const double* Am, const double* A0, double elapsed, hipStream_t s) {
         const int M = static_cast<int>(d.M);
-        // reads Ap, Am, A0 (48 B) and omega, UAp, VAp, WAp, NAp, UA0, VA0, NA0
-        // (104 B) per coefficient, writes four field spectra
-        timed("fields", 152.0 * M + 4.0 * M * sizeof(cx<T>), s, [&] {
+        // reads Ap, Am, A0 (48 B) per coefficient (the tables are formed from
+        // per-mode and per-j values), writes four field spectra
+        timed("fields", 48.0 * M + 4.0 * M * sizeof(cx<T>), s, [&] {
             hipLaunchKernelGGL(k_fields<T>, dim3((M + 255) / 256), dim3(256), 0, s,
                                reinterpret_cast<const cx<double>*>(Ap), reinterpret_cast<const cx<double>*>(Am),
-                               reinterpret_cast<const cx<double>*>(A0), fieldTables(), elapsed, M, FV.as<cx<T>>());
+                               reinterpret_cast<const cx<double>*>(A0), modeTab.as<ModeDev>(),
+                               static_cast<int>(d.c.Nj), coefConsts, elapsed, M, FV.as<cx<T>>());
         });
     }
 
@@ -2234,7 +2310,7 @@ struct KernelImpl {
         hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>((nrows + RB - 1) / RB)),
                            dim3(kFftThreads), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, log2Exact(d.c.Nx), Ny, Nz,
                            static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), kxv.as<T>(),
-                           zeroRow.as<cx<T>>(), es);
+                           zeroRow.as<cx<T>>(), es, static_cast<const double*>(nullptr));
     }
 
     // Nx = 64 ... 512 run kernels compiled for that length; others the
@@ -2255,11 +2331,29 @@ struct KernelImpl {
             launchXfusedKernel<T, NTc, 1, 0>(s);
     }
 
+    // The products with the caller's advecting fields (runtime-length
+    // kernel, any Nx).
+    template <typename T, int NTc>
+    void launchXfusedExt(const double* adv, hipStream_t s) {
+        constexpr int pairs = XMap<NTc>::pairs;
+        const int Nx = static_cast<int>(d.c.Nx), Ny = static_cast<int>(d.c.Ny), Nz = static_cast<int>(d.c.Nz);
+        const size_t lds = xfusedLdsBytes<T>(Nx, pairs, 1);
+        if (lds > maxLdsPerBlock) throw Failure{WV_UNSUPPORTED_OPERATION, "x transforms exceed the LDS of a workgroup."};
+        const T es = static_cast<T>(std::ldexp(1.0, static_cast<int>(std::lround(std::log2(d.c.N0 > 0 ? d.c.N0 : 1.0)))));
+        hipLaunchKernelGGL((k_xfused<T, NTc, 1, 0, false, true>), dim3(static_cast<unsigned>(Ny * Nz)),
+                           dim3(kFftThreads), lds, s, G.as<cx<T>>(), P.as<cx<T>>(), Nx, log2Exact(d.c.Nx), Ny, Nz,
+                           static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), kxv.as<T>(),
+                           zeroRow.as<cx<T>>(), es, adv);
+    }
+
     template <typename T>
-    void launchXfused(hipStream_t s) {
-        const double bytes = (static_cast<double>(NH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>);
+    void launchXfused(hipStream_t s, const double* adv = nullptr) {
+        const double bytes = (static_cast<double>(NH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>) +
+                             (adv ? 3.0 * d.R * sizeof(double) : 0.0);
         timed("xfused", bytes, s, [&] {
-            if (NT == 4)
+            if (adv)
+                NT == 4 ? launchXfusedExt<T, 4>(adv, s) : launchXfusedExt<T, 3>(adv, s);
+            else if (NT == 4)
                 launchXfusedFor<T, 4>(s);
             else
                 launchXfusedFor<T, 3>(s);
@@ -2338,9 +2432,10 @@ struct KernelImpl {
 
     template <typename T>
     // fieldsReady: FV already holds the spectra of (Ap, Am, A0) at t (the
-    // previous RK stage formed them).
+    // previous RK stage formed them).  adv: the caller's advecting fields
+    // [3][Nz][Ny][Nx] (nonlinearFluxUsingAdvectionFields), else synthesized.
     void rhs(const double* Ap, const double* Am, const double* A0, double t, double t0, const Epilogue& ep,
-             hipStream_t s, bool fieldsReady = false) {
+             hipStream_t s, bool fieldsReady = false, const double* adv = nullptr) {
         const double elapsed = t - t0;
         if (!fieldsReady) launchFields<T>(Ap, Am, A0, elapsed, s);
         SynthPlan lean;
@@ -2354,10 +2449,20 @@ struct KernelImpl {
             launchSynth<T>(fluxPlan(), s);
             launchYfft<T>(H, G, NH, +1, s, "yfft_inv", true, false);
         }
-        launchXfused<T>(s);
+        launchXfused<T>(s, adv);
         launchYfft<T>(P, Q, NT, -1, s, "yfft_fwd", false, true);
         launchProj<T>(s);
         launchStage<T>(elapsed, ep, s);
+        // the logical transforms of the reference's schedule (one velocity
+        // c2r, per target one derivative c2r and one r2c; vertical: three
+        // velocity channels, three derivative channels and one analysis per
+        // target), counted as its plan executions
+        ++metrics.nonlinearFluxCallCount;
+        metrics.nonlinearFluxPhaseEvaluationCount += d.M;
+        if (!adv) ++metrics.advectionVelocityReconstructionCount;
+        metrics.horizontalExecutionCount += (adv ? 0 : 1) + 2 * static_cast<size_t>(NT);
+        metrics.verticalExecutionCount += (adv ? 0 : 3) + 4 * static_cast<size_t>(NT);
+        metrics.executionCount = metrics.horizontalExecutionCount + metrics.verticalExecutionCount;
     }
 
     // The generic x transforms take rows in groups of about 16 real signals
@@ -2439,7 +2544,10 @@ struct KernelImpl {
         scNxh = static_cast<int>(Nx / 2 + 1);
         scNxhP = (scNxh + KT - 1) / KT * KT;
         const size_t yLds = (static_cast<size_t>(KT) + 1) * Ny * sizeof(cx<double>);
-        const size_t xLds = 2 * static_cast<size_t>(ldsLength(static_cast<int>(Nx))) * sizeof(cx<double>);
+        // the generic x transforms hold (rows per block x signals + 1) / 2
+        // complex transforms: 8 for one signal (r2c) and for three (c2r)
+        const size_t xTransforms = static_cast<size_t>((xRowsPerBlock(1) + 1) / 2);
+        const size_t xLds = xTransforms * static_cast<size_t>(ldsLength(static_cast<int>(Nx))) * sizeof(cx<double>);
         const size_t zLds = Nz * 8 * sizeof(cx<double>);
         if (yLds > maxLdsPerBlock || xLds > maxLdsPerBlock || zLds > maxLdsPerBlock)
             throw Failure{WV_UNSUPPORTED_OPERATION, "Scalar advection needs more LDS per workgroup than the device provides."};
@@ -2459,6 +2567,7 @@ struct KernelImpl {
         upload(scKeep, keep);
         upload(scTwX, d.twX);
         upload(scTwY, d.twY);
+        scalarPrepared = true;
     }
 
     void advectScalar(const double* scalar, const double* adv, bool antialias, double* rhs, hipStream_t s) {
@@ -2527,12 +2636,48 @@ struct KernelImpl {
             xinv(rhs, 1);
         }
         hipCheck(hipGetLastError(), "advectFGridScalar");
+        // the reference's plan executions (...Kernel.cpp:1059-1129): r2c,
+        // DCT-I and DST-I of the z derivative, c2r of the three gradients;
+        // the antialias adds an r2c and a c2r
+        ++metrics.scalarAdvectionCount;
+        metrics.horizontalExecutionCount += antialias ? 4 : 2;
+        metrics.verticalExecutionCount += 2;
+        if (antialias) ++metrics.scalarAntialiasCount;
+        metrics.executionCount = metrics.horizontalExecutionCount + metrics.verticalExecutionCount;
     }
 
     void rhsAny(const double* Ap, const double* Am, const double* A0, double t, double t0, const Epilogue& ep,
-                hipStream_t s, bool fieldsReady = false) {
-        if (precision == WV_PRECISION_FP64) rhs<double>(Ap, Am, A0, t, t0, ep, s, fieldsReady);
-        else rhs<float>(Ap, Am, A0, t, t0, ep, s, fieldsReady);
+                hipStream_t s, bool fieldsReady = false, const double* adv = nullptr) {
+        if (precision == WV_PRECISION_FP64) rhs<double>(Ap, Am, A0, t, t0, ep, s, fieldsReady, adv);
+        else rhs<float>(Ap, Am, A0, t, t0, ep, s, fieldsReady, adv);
+    }
+
+    // transformStateFieldDerivatives (...Kernel.cpp:719-742): d/dx, d/dy and
+    // d/dz of field f (0 u, 1 v, 2 w, 3 eta) at time t, out [3][Nz][Ny][Nx].
+    // u and v are of the cosine (F) family, w and eta of the sine (G) family;
+    // d/dz turns one family into the other (the DST-I / DCT-I with -/+ m_j / 2,
+    // or for an arbitrary stratification -(N^2/g) G and F/h).
+    template <typename T>
+    void stateDerivatives(const double* Ap, const double* Am, const double* A0, double t, double t0, int f,
+                          double* out, hipStream_t s) {
+        launchFields<T>(Ap, Am, A0, t - t0, s);
+        const bool cosFamily = f < 2;
+        SynthPlan sp{};
+        sp.valueScale = d.stratified ? 1.0 : 0.5;
+        sp.ngroups = 2;
+        sp.g[0] = {cosFamily ? 0 : 1, 1, {{f, 0, -1, 0, 1}, {}, {}, {}}};
+        if (d.stratified)
+            sp.g[1] = {cosFamily ? 2 : 3, 1, {{f, 0, 2, -1, -1}, {}, {}, {}}};
+        else
+            sp.g[1] = {cosFamily ? 1 : 0, 1, {{f, 1, 2, -1, -1}, {}, {}, {}}};
+        launchSynth<T>(sp, s);
+        launchYfft<T>(H, G, 3, +1, s, "yfft_inv", true, false);
+        timed("xinv", 3.0 * d.R * (8.0 + 0.67 * sizeof(cx<T>)), s, [&] {
+            launchXinvRows<T>(G.as<cx<T>>(), out, 3, static_cast<int>(d.nkx), static_cast<int>(d.nkxP), twX.as<cx<T>>(), s);
+        });
+        metrics.horizontalExecutionCount += 1;
+        metrics.verticalExecutionCount += 3;
+        metrics.executionCount = metrics.horizontalExecutionCount + metrics.verticalExecutionCount;
     }
 
     template <typename T>
@@ -2870,6 +3015,16 @@ struct KernelImpl {
     // (with the same times as WVFixedStepRK4::step: t + h/2, t + h/2, t + h),
     // so only the first evaluation of the call starts from the coefficients.
     void rk4Advance(cx<double>* const y[3], double* t, double t0, double h, int nsteps, hipStream_t s) {
+        rk4AdvanceSteps(y, t, t0, std::vector<double>(static_cast<size_t>(std::max(nsteps, 0)), h), s);
+    }
+
+    // Classic RK4 steps of the given sizes.  A step's stages 1-3 write only
+    // the workspace; its stage 4 forms y + h/6 weighted in place, so every
+    // step is committed by its last launch: when a launch fails, y holds the
+    // state after the last whole step and *t that step's end time.
+    void rk4AdvanceSteps(cx<double>* const y[3], double* t, double t0, const std::vector<double>& steps,
+                         hipStream_t s) {
+        const int nsteps = static_cast<int>(steps.size());
         const size_t M = d.M;
         cx<double>* st = stage.as<cx<double>>();
         cx<double>* w = weighted.as<cx<double>>();
@@ -2890,6 +3045,7 @@ struct KernelImpl {
         const double* y2 = reinterpret_cast<const double*>(y[2]);
         for (int n = 0; n < nsteps; ++n) {
             const double ti = *t;
+            const double h = steps[static_cast<size_t>(n)];
             ep.mode = kRk1;
             ep.scale = 0.5 * h;
             ep.nextElapsed = (ti + 0.5 * h) - t0;
@@ -2906,6 +3062,8 @@ struct KernelImpl {
             rhsAny(sd, sd + 2 * M, sd + 4 * M, ti + h, t0, ep, s, true);
             ep.emitFields = 1;
             *t = ti + h;
+            ++rkStepCount;
+            lastStepSize = h;
         }
     }
 
@@ -2998,6 +3156,26 @@ struct ExecGuard {
     }
 };
 
+hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+
+// One call on a handle: the reentrancy guard (ExecutionGuard, ...Kernel.cpp:76-84),
+// the handle's device, ordering after the handle's previous call, the work,
+// and the event that orders the next call.  body(s) throws Failure on error.
+template <typename F>
+wv_status execute(KernelImpl& k, void* stream, const char* busy, F&& body) {
+    ExecGuard g(k.executing);
+    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, busy);
+    return guarded([&] {
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        const hipStream_t s = S(stream);
+        k.orderAfterLastCall(s);
+        body(s);
+        k.recordCall(s);
+        return WV_SUCCESS;
+    });
+}
+constexpr const char* kNotReentrant = "Kernel operations are not reentrant.";
+
 bool overlaps(const void* a, size_t an, const void* b, size_t bn) {
     if (!a || !b || an == 0 || bn == 0) return false;
     const auto x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
@@ -3023,7 +3201,20 @@ wv_status validateFlux(const KernelImpl& k, double t, double t0, const void* Ap,
     return WV_SUCCESS;
 }
 
-hipStream_t S(void* s) { return static_cast<hipStream_t>(s); }
+// validateInverseOwnership: a spatial bundle of nf fields may not overlap the
+// state's coefficient arrays.
+wv_status validateBundleOwnership(const KernelImpl& k, const void* fields, int nf, const void* Ap, const void* Am,
+                                  const void* A0, const char* what) {
+    const size_t cb = k.d.M * 16, fb = k.d.R * static_cast<size_t>(nf) * sizeof(double);
+    for (const void* in : {Ap, Am, A0})
+        if (overlaps(fields, fb, in, cb)) return fail(WV_OVERLAPPING_ARRAYS, std::string(what) + " must not overlap coefficient inputs.");
+    return WV_SUCCESS;
+}
+
+// The three arrays of a state or cotangent must not overlap one another.
+bool overlapping3(const void* a, const void* b, const void* c, size_t bytes) {
+    return overlaps(a, bytes, b, bytes) || overlaps(a, bytes, c, bytes) || overlaps(b, bytes, c, bytes);
+}
 
 }  // namespace
 
@@ -3094,12 +3285,65 @@ void wv_destroy(wv_kernel* kernel) {
         (void)hipEventDestroy(pd.a);
         (void)hipEventDestroy(pd.b);
     }
+    if (kernel->impl.lastCall) (void)hipEventDestroy(kernel->impl.lastCall);
     delete kernel;
 }
 
 size_t wv_nkl(const wv_kernel* kernel) { return kernel ? kernel->impl.d.Nkl : 0; }
 double wv_coriolis_frequency(const wv_kernel* kernel) { return kernel ? kernel->impl.d.f : 0.0; }
 size_t wv_persistent_bytes(const wv_kernel* kernel) { return kernel ? kernel->impl.persistentBytes() : 0; }
+size_t wv_scratch_bytes(const wv_kernel* kernel) { return kernel ? kernel->impl.scratchBytes() : 0; }
+
+wv_status wv_metrics(const wv_kernel* kernel, wv_kernel_metrics* out) {
+    if (!kernel || !out) return fail(WV_INVALID_POINTER, "wv_metrics requires a kernel and an output.");
+    const KernelImpl& k = kernel->impl;
+    wv_kernel_metrics m = k.metrics;
+    m.descriptorBytes = k.descriptorBytes();
+    m.planBytes = k.planBytes();
+    m.engineBytes = 0;
+    m.kernelManagementBytes = sizeof(wv_kernel);
+    m.halfSpectrumScratchCapacityBytes = k.scratchBytes();
+    m.realScratchCapacityBytes = 0;
+    m.scratchCapacityBytes = m.halfSpectrumScratchCapacityBytes + m.realScratchCapacityBytes;
+    m.scratchHighWaterBytes = m.scratchCapacityBytes;
+    m.planCount = 17 + (k.scalarPrepared ? 1 : 0);
+    auto mean = [&](std::initializer_list<const char*> names) {
+        double sec = 0.0;
+        for (const char* n : names)
+            for (const auto& r : k.records)
+                if (r.name == n && r.launches) sec += 1e-3 * r.totalMs / static_cast<double>(r.launches);
+        return sec;
+    };
+    m.phaseSeconds = mean({"fields"});
+    m.reconstructionSeconds = mean({"synth", "yfft_inv"});
+    m.productSeconds = mean({"xfused"});
+    m.projectionSeconds = mean({"yfft_fwd", "proj", "stage"});
+    *out = m;
+    return WV_SUCCESS;
+}
+
+const char* wv_engine_identifier(const wv_kernel* kernel) { return kernel ? "wvamd-hip-gfx950" : nullptr; }
+const char* wv_engine_library_identity(const wv_kernel* kernel) { return kernel ? "libwvamd.so (HIP, gfx950)" : nullptr; }
+const char* wv_nonlinear_flux_schedule_identifier(const wv_kernel* kernel) {
+    return kernel ? "mfma-synthesis/y-fft/fused-x-products/y-fft/mfma-analysis/stage" : nullptr;
+}
+const char* wv_phase_implementation_identifier(const wv_kernel* kernel) { return kernel ? "device-sincos-fp64" : nullptr; }
+const char* wv_coefficient_arithmetic_mode_identifier(const wv_kernel* kernel) {
+    return kernel ? "natural-dimensional-prescaled" : nullptr;
+}
+const char* wv_inverse_normalization_placement_identifier(const wv_kernel* kernel) {
+    return kernel ? "coefficient-production" : nullptr;
+}
+const char* wv_optimization_implementation_identifier(const wv_kernel* kernel) {
+    if (!kernel) return nullptr;
+    return kernel->impl.precision == WV_PRECISION_FP64 ? "hip-gfx950-fp64" : "hip-gfx950-mixed-fp32";
+}
+size_t wv_coefficient_worker_count(const wv_kernel* kernel) { (void)kernel; return 0; }
+size_t wv_phase_reservation_bytes(const wv_kernel* kernel) {
+    if (!kernel) return 0;
+    const auto& c = kernel->impl.d.c;
+    return kernel->impl.d.NxHalf * c.Ny * c.Nz * 16;
+}
 
 wv_status wv_descriptor_modes(const wv_kernel* kernel, int64_t* kMode, int64_t* lMode, double* k, double* l,
                               double* Kh, double* cosAlpha, double* sinAlpha, int64_t* storageRow,
@@ -3130,17 +3374,13 @@ wv_status wv_nonlinear_flux_device(wv_kernel* kernel, double t, double t0, const
     if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
     KernelImpl& k = kernel->impl;
     if (auto st = validateFlux(k, t, t0, Ap, Am, A0, Fp, Fm, F0); st != WV_SUCCESS) return st;
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
         wvamd::Epilogue ep{};
         ep.mode = wvamd::kFluxOut;
         ep.out0 = reinterpret_cast<wvamd::cx<double>*>(Fp);
         ep.out1 = reinterpret_cast<wvamd::cx<double>*>(Fm);
         ep.out2 = reinterpret_cast<wvamd::cx<double>*>(F0);
-        k.rhsAny(Ap, Am, A0, t, t0, ep, S(stream));
-        return WV_SUCCESS;
+        k.rhsAny(Ap, Am, A0, t, t0, ep, s);
     });
 }
 
@@ -3166,6 +3406,7 @@ wv_status wv_nonlinear_flux(wv_kernel* kernel, double t, double t0, const double
         hipCheck(hipMemcpy(Fp, o, bytes, hipMemcpyDeviceToHost), "copy");
         hipCheck(hipMemcpy(Fm, o + n, bytes, hipMemcpyDeviceToHost), "copy");
         hipCheck(hipMemcpy(F0, o + 2 * n, bytes, hipMemcpyDeviceToHost), "copy");
+        k.metrics.bytesCopied += 6 * bytes;
         return WV_SUCCESS;
     });
 }
@@ -3174,10 +3415,52 @@ wv_status wv_nonlinear_flux_with_advection_fields_device(wv_kernel* kernel, doub
                                                          const double* Am, const double* A0, double* Fp,
                                                          double* Fm, double* F0, double* uvw, void* stream) {
     if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (auto st = validateFlux(k, t, t0, Ap, Am, A0, Fp, Fm, F0); st != WV_SUCCESS) return st;
     if (!uvw) return fail(WV_INVALID_POINTER, "Advection fields has a null data pointer.");
-    wv_status st = wv_transform_wave_vortex_to_uvw_device(kernel, t, t0, Ap, Am, A0, uvw, stream);
-    if (st != WV_SUCCESS) return st;
-    return wv_nonlinear_flux_device(kernel, t, t0, Ap, Am, A0, Fp, Fm, F0, stream);
+    if (auto st = validateBundleOwnership(k, uvw, 3, Ap, Am, A0, "Advection fields"); st != WV_SUCCESS) return st;
+    const size_t cb = k.d.M * 16, fb = 3 * k.d.R * sizeof(double);
+    for (const void* o : {static_cast<const void*>(Fp), static_cast<const void*>(Fm), static_cast<const void*>(F0)})
+        if (overlaps(uvw, fb, o, cb)) return fail(WV_OVERLAPPING_ARRAYS, "Advection fields must not overlap flux outputs.");
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
+        if (k.precision == WV_PRECISION_FP64) k.inverse<double>(Ap, Am, A0, t, t0, 3, uvw, s);
+        else k.inverse<float>(Ap, Am, A0, t, t0, 3, uvw, s);
+        wvamd::Epilogue ep{};
+        ep.mode = wvamd::kFluxOut;
+        ep.out0 = reinterpret_cast<wvamd::cx<double>*>(Fp);
+        ep.out1 = reinterpret_cast<wvamd::cx<double>*>(Fm);
+        ep.out2 = reinterpret_cast<wvamd::cx<double>*>(F0);
+        k.rhsAny(Ap, Am, A0, t, t0, ep, s);
+    });
+}
+
+wv_status wv_nonlinear_flux_using_advection_fields_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                          const double* Am, const double* A0, const double* uvw,
+                                                          double* Fp, double* Fm, double* F0, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (auto st = validateFlux(k, t, t0, Ap, Am, A0, Fp, Fm, F0); st != WV_SUCCESS) return st;
+    if (!uvw) return fail(WV_INVALID_POINTER, "Prepared advection fields has a null data pointer.");
+    if (auto st = validateBundleOwnership(k, uvw, 3, Ap, Am, A0, "Prepared advection fields"); st != WV_SUCCESS)
+        return st;
+    const size_t cb = k.d.M * 16, fb = 3 * k.d.R * sizeof(double);
+    for (const void* o : {static_cast<const void*>(Fp), static_cast<const void*>(Fm), static_cast<const void*>(F0)})
+        if (overlaps(uvw, fb, o, cb))
+            return fail(WV_OVERLAPPING_ARRAYS, "Prepared advection fields must not overlap flux outputs.");
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
+        wvamd::Epilogue ep{};
+        ep.mode = wvamd::kFluxOut;
+        ep.out0 = reinterpret_cast<wvamd::cx<double>*>(Fp);
+        ep.out1 = reinterpret_cast<wvamd::cx<double>*>(Fm);
+        ep.out2 = reinterpret_cast<wvamd::cx<double>*>(F0);
+        k.rhsAny(Ap, Am, A0, t, t0, ep, s, false, uvw);
+    });
+}
+
+wv_status wv_prepare_scalar_advection(wv_kernel* kernel) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    return execute(k, nullptr, kNotReentrant, [&](hipStream_t) { k.ensureScalarScratch(); });
 }
 
 wv_status wv_advect_f_grid_scalar_device(wv_kernel* kernel, const double* scalar, const double* advectionFields,
@@ -3191,12 +3474,8 @@ wv_status wv_advect_f_grid_scalar_device(wv_kernel* kernel, const double* scalar
     if (overlaps(scalar, bytes, rightHandSide, bytes) || overlaps(advectionFields, 3 * bytes, scalar, bytes) ||
         overlaps(advectionFields, 3 * bytes, rightHandSide, bytes))
         return fail(WV_OVERLAPPING_ARRAYS, "Tracer scalar, velocity, and RHS arrays must not overlap.");
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
-        k.advectScalar(scalar, advectionFields, shouldAntialias != 0, rightHandSide, S(stream));
-        return WV_SUCCESS;
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
+        k.advectScalar(scalar, advectionFields, shouldAntialias != 0, rightHandSide, s);
     });
 }
 
@@ -3205,16 +3484,12 @@ static wv_status inverseCommon(wv_kernel* kernel, double t, double t0, const dou
     if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
     KernelImpl& k = kernel->impl;
     if (!Ap || !Am || !A0 || !fields) return fail(WV_INVALID_POINTER, "Inverse transform has a null data pointer.");
-    const size_t cb = k.d.M * 16, fb = k.d.R * nf * sizeof(double);
-    for (const void* in : {static_cast<const void*>(Ap), static_cast<const void*>(Am), static_cast<const void*>(A0)})
-        if (overlaps(fields, fb, in, cb)) return fail(WV_OVERLAPPING_ARRAYS, "Inverse spatial output must not overlap coefficient inputs.");
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
-        if (k.precision == WV_PRECISION_FP64) k.inverse<double>(Ap, Am, A0, t, t0, nf, fields, S(stream));
-        else k.inverse<float>(Ap, Am, A0, t, t0, nf, fields, S(stream));
-        return WV_SUCCESS;
+    if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "State times must be finite.");
+    if (auto st = validateBundleOwnership(k, fields, nf, Ap, Am, A0, "Inverse spatial output"); st != WV_SUCCESS)
+        return st;
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
+        if (k.precision == WV_PRECISION_FP64) k.inverse<double>(Ap, Am, A0, t, t0, nf, fields, s);
+        else k.inverse<float>(Ap, Am, A0, t, t0, nf, fields, s);
     });
 }
 
@@ -3229,6 +3504,25 @@ wv_status wv_transform_wave_vortex_to_uvw_device(wv_kernel* kernel, double t, do
     return inverseCommon(kernel, t, t0, Ap, Am, A0, fields, 3, stream);
 }
 
+wv_status wv_transform_state_field_derivatives_device(wv_kernel* kernel, double t, double t0, const double* Ap,
+                                                      const double* Am, const double* A0, int field,
+                                                      double* derivatives, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (field < 0 || field > 3) return fail(WV_INVALID_CONFIGURATION, "Unknown dynamical field derivative target.");
+    if (!derivatives) return fail(WV_INVALID_POINTER, "Field derivatives has a null data pointer.");
+    if (!Ap) return fail(WV_INVALID_POINTER, "Ap has a null data pointer.");
+    if (!Am) return fail(WV_INVALID_POINTER, "Am has a null data pointer.");
+    if (!A0) return fail(WV_INVALID_POINTER, "A0 has a null data pointer.");
+    if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "State times must be finite.");
+    if (auto st = validateBundleOwnership(k, derivatives, 3, Ap, Am, A0, "Field derivatives"); st != WV_SUCCESS)
+        return st;
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
+        if (k.precision == WV_PRECISION_FP64) k.stateDerivatives<double>(Ap, Am, A0, t, t0, field, derivatives, s);
+        else k.stateDerivatives<float>(Ap, Am, A0, t, t0, field, derivatives, s);
+    });
+}
+
 static wv_status forwardCommon(wv_kernel* kernel, const double* fields, double t, double t0, double* Ap,
                                double* Am, double* A0, int nf, void* stream) {
     if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
@@ -3238,6 +3532,7 @@ static wv_status forwardCommon(wv_kernel* kernel, const double* fields, double t
     if (nf == 3 && !hydro) return fail(WV_INVALID_CONFIGURATION, "transformUVEtaToWaveVortex requires a hydrostatic kernel.");
     if (!fields) return fail(WV_INVALID_POINTER, "Fields has a null data pointer.");
     if (!Ap || !Am || !A0) return fail(WV_INVALID_POINTER, "Coefficient output has a null data pointer.");
+    if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "State times must be finite.");
     const size_t cb = k.d.M * 16, fb = k.d.R * nf * sizeof(double);
     void* outs[3] = {Ap, Am, A0};
     for (int i = 0; i < 3; ++i) {
@@ -3245,13 +3540,9 @@ static wv_status forwardCommon(wv_kernel* kernel, const double* fields, double t
         for (int j = i + 1; j < 3; ++j)
             if (overlaps(outs[i], cb, outs[j], cb)) return fail(WV_OVERLAPPING_ARRAYS, "Forward coefficient outputs must not overlap each other.");
     }
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
-        if (k.precision == WV_PRECISION_FP64) k.forward<double>(fields, t, t0, nf, Ap, Am, A0, S(stream));
-        else k.forward<float>(fields, t, t0, nf, Ap, Am, A0, S(stream));
-        return WV_SUCCESS;
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
+        if (k.precision == WV_PRECISION_FP64) k.forward<double>(fields, t, t0, nf, Ap, Am, A0, s);
+        else k.forward<float>(fields, t, t0, nf, Ap, Am, A0, s);
     });
 }
 
@@ -3275,13 +3566,9 @@ static wv_status fgCommon(wv_kernel* kernel, const double* Apm, const double* A0
     const size_t cb = k.d.M * 16, fb = k.d.R * 4 * sizeof(double);
     if (overlaps(fields, fb, Apm, cb) || overlaps(fields, fb, A0, cb))
         return fail(WV_OVERLAPPING_ARRAYS, "Derivative output must not overlap coefficient inputs.");
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
-        if (k.precision == WV_PRECISION_FP64) k.fgAll<double>(Apm, A0, fFamily, fields, S(stream));
-        else k.fgAll<float>(Apm, A0, fFamily, fields, S(stream));
-        return WV_SUCCESS;
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
+        if (k.precision == WV_PRECISION_FP64) k.fgAll<double>(Apm, A0, fFamily, fields, s);
+        else k.fgAll<float>(Apm, A0, fFamily, fields, s);
     });
 }
 
@@ -3297,23 +3584,65 @@ wv_status wv_transform_to_spatial_domain_with_g_all_derivatives_device(wv_kernel
     return fgCommon(kernel, Apm, A0, fields, false, stream);
 }
 
+namespace {
+// State checks shared by the RK4 entry points (WVFixedStepRK4::step,
+// WVRungeKutta.cpp:335-344): non-null, pairwise non-overlapping arrays.
+wv_status validateRk4State(const KernelImpl& k, const double* Ap, const double* Am, const double* A0, const double* t) {
+    if (!Ap || !Am || !A0 || !t) return fail(WV_INVALID_POINTER, "RK4 state has a null data pointer.");
+    if (overlapping3(Ap, Am, A0, k.d.M * 16)) return fail(WV_OVERLAPPING_ARRAYS, "RK4 state arrays must not overlap.");
+    if (!std::isfinite(*t)) return fail(WV_INVALID_CONFIGURATION, "State times must be finite.");
+    return WV_SUCCESS;
+}
+}  // namespace
+
 wv_status wv_rk4_advance_device(wv_kernel* kernel, double* Ap, double* Am, double* A0, double* t, double t0,
                                 double h, int nsteps, void* stream) {
     if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
-    if (!Ap || !Am || !A0 || !t) return fail(WV_INVALID_POINTER, "RK4 state has a null data pointer.");
+    KernelImpl& k = kernel->impl;
+    if (auto st = validateRk4State(k, Ap, Am, A0, t); st != WV_SUCCESS) return st;
+    if (!std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "State times must be finite.");
     if (!(h > 0.0) || !std::isfinite(h)) return fail(WV_INVALID_CONFIGURATION, "RK4 step size must be finite and positive.");
     if (nsteps < 0) return fail(WV_INVALID_CONFIGURATION, "RK4 step count must be nonnegative.");
-    KernelImpl& k = kernel->impl;
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "RK4 stepping is not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+    return execute(k, stream, "RK4 stepping is not reentrant.", [&](hipStream_t s) {
         using wvamd::cx;
         cx<double>* const y[3] = {reinterpret_cast<cx<double>*>(Ap), reinterpret_cast<cx<double>*>(Am),
                                   reinterpret_cast<cx<double>*>(A0)};
-        k.rk4Advance(y, t, t0, h, nsteps, S(stream));
-        return WV_SUCCESS;
+        k.rk4Advance(y, t, t0, h, nsteps, s);
     });
+}
+
+wv_status wv_rk4_advance_to_time_device(wv_kernel* kernel, double* Ap, double* Am, double* A0, double* t, double t0,
+                                        double finalTime, double h, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (auto st = validateRk4State(k, Ap, Am, A0, t); st != WV_SUCCESS) return st;
+    if (!std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "State times must be finite.");
+    if (finalTime < *t || !std::isfinite(finalTime))
+        return fail(WV_INVALID_CONFIGURATION, "RK4 cannot advance backward or to a nonfinite time.");
+    if (!(h > 0.0) || !std::isfinite(h)) return fail(WV_INVALID_CONFIGURATION, "RK4 step size must be finite and positive.");
+    // the step sizes advanceToTime takes, with its time arithmetic
+    std::vector<double> steps;
+    for (double tt = *t; tt < finalTime;) {
+        const double stepSize = std::min(h, finalTime - tt);
+        if (!(stepSize > 0.0)) break;
+        steps.push_back(stepSize);
+        tt = tt + stepSize;
+    }
+    const wv_status st = execute(k, stream, "RK4 stepping is not reentrant.", [&](hipStream_t s) {
+        using wvamd::cx;
+        cx<double>* const y[3] = {reinterpret_cast<cx<double>*>(Ap), reinterpret_cast<cx<double>*>(Am),
+                                  reinterpret_cast<cx<double>*>(A0)};
+        k.rk4AdvanceSteps(y, t, t0, steps, s);
+    });
+    if (st == WV_SUCCESS) *t = finalTime;
+    return st;
+}
+
+wv_status wv_rk4_counters(const wv_kernel* kernel, uint64_t* stepCount, double* lastStepSize) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    if (stepCount) *stepCount = kernel->impl.rkStepCount;
+    if (lastStepSize) *lastStepSize = kernel->impl.lastStepSize;
+    return WV_SUCCESS;
 }
 
 namespace {
@@ -3328,18 +3657,15 @@ wv_status wv_nonlinear_flux_vjp_device(wv_kernel* kernel, double t, double t0, c
     if (!Ap || !Am || !A0 || !Lp || !Lm || !L0 || !Gp || !Gm || !G0)
         return fail(WV_INVALID_POINTER, "Flux adjoint has a null data pointer.");
     if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "Times must be finite.");
+    KernelImpl& k = kernel->impl;
+    const size_t cb = k.d.M * 16;
     for (const double* o : {Gp, Gm, G0})
         for (const double* i : {Ap, Am, A0, Lp, Lm, L0})
-            if (o == i) return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs may not alias the inputs.");
-    if (!distinct3(Gp, Gm, G0)) return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs must be distinct.");
-    KernelImpl& k = kernel->impl;
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
-        k.fluxVjpAny(Ap, Am, A0, t, t0, Lp, Lm, L0, Gp, Gm, G0, S(stream));
-        return WV_SUCCESS;
-    });
+            if (overlaps(o, cb, i, cb)) return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs may not alias the inputs.");
+    if (!distinct3(Gp, Gm, G0) || overlapping3(Gp, Gm, G0, cb))
+        return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs must be distinct.");
+    return execute(k, stream, kNotReentrant,
+                   [&](hipStream_t s) { k.fluxVjpAny(Ap, Am, A0, t, t0, Lp, Lm, L0, Gp, Gm, G0, s); });
 }
 
 wv_status wv_transform_wave_vortex_to_uvweta_vjp_device(wv_kernel* kernel, const double* fieldsBar, double t,
@@ -3347,16 +3673,15 @@ wv_status wv_transform_wave_vortex_to_uvweta_vjp_device(wv_kernel* kernel, const
     if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
     if (!fieldsBar || !Gp || !Gm || !G0) return fail(WV_INVALID_POINTER, "Transform adjoint has a null data pointer.");
     if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "Times must be finite.");
-    if (!distinct3(Gp, Gm, G0)) return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs must be distinct.");
     KernelImpl& k = kernel->impl;
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+    const size_t cb = k.d.M * 16;
+    if (!distinct3(Gp, Gm, G0) || overlapping3(Gp, Gm, G0, cb))
+        return fail(WV_OVERLAPPING_ARRAYS, "Adjoint outputs must be distinct.");
+    if (auto st = validateBundleOwnership(k, fieldsBar, 4, Gp, Gm, G0, "Field cotangents"); st != WV_SUCCESS) return st;
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
         k.ensureAdjoint();
-        if (k.precision == WV_PRECISION_FP64) k.inverseVjp<double>(fieldsBar, t, t0, Gp, Gm, G0, S(stream));
-        else k.inverseVjp<float>(fieldsBar, t, t0, Gp, Gm, G0, S(stream));
-        return WV_SUCCESS;
+        if (k.precision == WV_PRECISION_FP64) k.inverseVjp<double>(fieldsBar, t, t0, Gp, Gm, G0, s);
+        else k.inverseVjp<float>(fieldsBar, t, t0, Gp, Gm, G0, s);
     });
 }
 
@@ -3389,19 +3714,17 @@ wv_status wv_rk4_adjoint_device(wv_kernel* kernel, const double* Ap, const doubl
     if (!(h > 0.0) || !std::isfinite(h) || !std::isfinite(t) || !std::isfinite(t0))
         return fail(WV_INVALID_CONFIGURATION, "RK4 step size must be finite and positive, times finite.");
     if (nsteps < 0 || interval < 1) return fail(WV_INVALID_CONFIGURATION, "Step count must be nonnegative, interval positive.");
-    if (!distinct3(Lp, Lm, L0)) return fail(WV_OVERLAPPING_ARRAYS, "Cotangent arrays must be distinct.");
+    KernelImpl& k = kernel->impl;
+    const size_t cb = k.d.M * 16;
+    if (!distinct3(Lp, Lm, L0) || overlapping3(Lp, Lm, L0, cb))
+        return fail(WV_OVERLAPPING_ARRAYS, "Cotangent arrays must be distinct.");
     for (const double* o : {Lp, Lm, L0})
         for (const double* i : {Ap, Am, A0})
-            if (o == i) return fail(WV_OVERLAPPING_ARRAYS, "Cotangents may not alias the state.");
-    KernelImpl& k = kernel->impl;
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+            if (overlaps(o, cb, i, cb)) return fail(WV_OVERLAPPING_ARRAYS, "Cotangents may not alias the state.");
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
         using wvamd::cx;
         k.ensureAdjoint();
         const size_t M = k.d.M;
-        const hipStream_t s = S(stream);
         wvamd::DeviceBuffer y0, ybar;
         y0.alloc(3 * M * sizeof(cx<double>));
         ybar.alloc(3 * M * sizeof(cx<double>));
@@ -3410,21 +3733,15 @@ wv_status wv_rk4_adjoint_device(wv_kernel* kernel, const double* Ap, const doubl
         k.rk4Adjoint(y0.as<cx<double>>(), t, t0, h, nsteps, interval, ybar.as<cx<double>>(), s);
         unpackTriple(ybar.as<cx<double>>(), Lp, Lm, L0, M, s);
         hipCheck(hipStreamSynchronize(s), "adjoint");
-        return WV_SUCCESS;
     });
 }
 
 wv_status wv_total_energy_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0,
                                  double* energy, void* stream) {
-    if (!kernel || !energy) return fail(WV_INVALID_POINTER, "null pointer");
+    if (!kernel || !energy) return fail(WV_INVALID_POINTER, "Total energy requires a kernel and an output.");
+    if (!Ap || !Am || !A0) return fail(WV_INVALID_POINTER, "Total energy state has a null data pointer.");
     KernelImpl& k = kernel->impl;
-    ExecGuard g(k.executing);
-    if (!g.entered) return fail(WV_REENTRANT_EXECUTION, "Kernel operations are not reentrant.");
-    return guarded([&] {
-        hipCheck(hipSetDevice(k.device), "hipSetDevice");
-        *energy = k.energy(Ap, Am, A0, S(stream));
-        return WV_SUCCESS;
-    });
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) { *energy = k.energy(Ap, Am, A0, s); });
 }
 
 wv_status wv_set_stage_instrumentation(wv_kernel* kernel, int enabled) {
@@ -3462,7 +3779,15 @@ void wv_reset_stage_timings(wv_kernel* kernel) {
     for (auto& r : kernel->impl.records) {
         r.totalMs = 0.0;
         r.launches = 0;
+        r.totalBytes = 0.0;
     }
+}
+
+wv_status wv_debug_inject_launch_failure(wv_kernel* kernel, long n) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    if (n < 0) return fail(WV_INVALID_CONFIGURATION, "Launch count must be nonnegative.");
+    kernel->impl.failAfterLaunches = n;
+    return WV_SUCCESS;
 }
 
 }  // extern "C"

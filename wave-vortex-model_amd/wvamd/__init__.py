@@ -98,7 +98,35 @@ EXPORTED_SYMBOLS = [
     "wv_rk4_advance_device", "wv_total_energy_device", "wv_set_stage_instrumentation", "wv_stage_timings",
     "wv_reset_stage_timings", "wv_create_hydrostatic", "wv_nonlinear_flux_vjp_device",
     "wv_transform_wave_vortex_to_uvweta_vjp_device", "wv_rk4_step_vjp_device", "wv_rk4_adjoint_device",
+    "wv_scratch_bytes", "wv_metrics", "wv_engine_identifier", "wv_engine_library_identity",
+    "wv_nonlinear_flux_schedule_identifier", "wv_phase_implementation_identifier",
+    "wv_coefficient_arithmetic_mode_identifier", "wv_inverse_normalization_placement_identifier",
+    "wv_optimization_implementation_identifier", "wv_coefficient_worker_count", "wv_phase_reservation_bytes",
+    "wv_nonlinear_flux_using_advection_fields_device", "wv_prepare_scalar_advection",
+    "wv_transform_state_field_derivatives_device", "wv_rk4_advance_to_time_device", "wv_rk4_counters",
+    "wv_debug_inject_launch_failure",
 ]
+
+IDENTIFIERS = ("engine_identifier", "engine_library_identity", "nonlinear_flux_schedule_identifier",
+               "phase_implementation_identifier", "coefficient_arithmetic_mode_identifier",
+               "inverse_normalization_placement_identifier", "optimization_implementation_identifier")
+
+# WVDynamicalField (...Kernel.hpp:10)
+DYNAMICAL_FIELDS = {"u": 0, "v": 1, "w": 2, "eta": 3}
+
+
+class KernelMetrics(C.Structure):
+    """wv_kernel_metrics / WVKernelMetrics (...Kernel.hpp:12-45), same members and order."""
+    _fields_ = [(n, C.c_size_t) for n in (
+        "descriptorBytes", "planBytes", "engineBytes", "kernelManagementBytes", "scratchCapacityBytes",
+        "scratchHighWaterBytes", "halfSpectrumScratchCapacityBytes", "realScratchCapacityBytes", "planCount",
+        "executionCount", "horizontalExecutionCount", "verticalExecutionCount", "nonlinearFluxCallCount",
+        "nonlinearFluxPhaseEvaluationCount", "advectionVelocityReconstructionCount", "scalarAdvectionCount",
+        "scalarAntialiasCount", "bytesCopied")] + [(n, C.c_double) for n in (
+        "phaseSeconds", "reconstructionSeconds", "derivativeReconstructionSeconds", "productSeconds",
+        "projectionSeconds", "coefficientAssemblySeconds", "derivativeCoefficientAssemblySeconds",
+        "coefficientProjectionSeconds", "scalarForwardSeconds", "scalarDerivativeAssemblySeconds",
+        "scalarVerticalDerivativeSeconds", "scalarInverseSeconds", "scalarProductSeconds", "scalarAntialiasSeconds")]
 
 
 class VerticalOperators(C.Structure):
@@ -169,6 +197,29 @@ def library() -> C.CDLL:
     lib.wv_rk4_adjoint_device.restype = C.c_int
     lib.wv_rk4_adjoint_device.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                           vp, vp, vp, vp]
+    lib.wv_scratch_bytes.restype = C.c_size_t
+    lib.wv_scratch_bytes.argtypes = [vp]
+    lib.wv_metrics.restype = C.c_int
+    lib.wv_metrics.argtypes = [vp, C.POINTER(KernelMetrics)]
+    for name in IDENTIFIERS:
+        getattr(lib, "wv_" + name).restype = C.c_char_p
+        getattr(lib, "wv_" + name).argtypes = [vp]
+    for fn in ("wv_coefficient_worker_count", "wv_phase_reservation_bytes"):
+        getattr(lib, fn).restype = C.c_size_t
+        getattr(lib, fn).argtypes = [vp]
+    lib.wv_nonlinear_flux_using_advection_fields_device.restype = C.c_int
+    lib.wv_nonlinear_flux_using_advection_fields_device.argtypes = [vp, C.c_double, C.c_double] + [vp] * 7 + [vp]
+    lib.wv_prepare_scalar_advection.restype = C.c_int
+    lib.wv_prepare_scalar_advection.argtypes = [vp]
+    lib.wv_transform_state_field_derivatives_device.restype = C.c_int
+    lib.wv_transform_state_field_derivatives_device.argtypes = [vp, C.c_double, C.c_double, vp, vp, vp, C.c_int,
+                                                                vp, vp]
+    lib.wv_rk4_advance_to_time_device.restype = C.c_int
+    lib.wv_rk4_advance_to_time_device.argtypes = [vp, vp, vp, vp, dp, C.c_double, C.c_double, C.c_double, vp]
+    lib.wv_rk4_counters.restype = C.c_int
+    lib.wv_rk4_counters.argtypes = [vp, C.POINTER(C.c_uint64), dp]
+    lib.wv_debug_inject_launch_failure.restype = C.c_int
+    lib.wv_debug_inject_launch_failure.argtypes = [vp, C.c_long]
     _lib = lib
     return lib
 
@@ -258,6 +309,24 @@ class WVTransformConstantStratificationKernel:
     def persistent_bytes(self) -> int:
         return int(library().wv_persistent_bytes(self._h))
 
+    def scratch_bytes(self) -> int:
+        """scratchBytes (...Kernel.hpp:70)."""
+        return int(library().wv_scratch_bytes(self._h))
+
+    def metrics(self) -> dict:
+        """metrics() (...Kernel.cpp:522-537) as a dict of WVKernelMetrics members."""
+        m = KernelMetrics()
+        _check(library().wv_metrics(self._h, C.byref(m)))
+        return {name: getattr(m, name) for name, _ in KernelMetrics._fields_}
+
+    def identifiers(self) -> dict:
+        """The identifier accessors (...Kernel.hpp:59-67)."""
+        lib = library()
+        out = {name: getattr(lib, "wv_" + name)(self._h).decode() for name in IDENTIFIERS}
+        out["coefficient_worker_count"] = int(lib.wv_coefficient_worker_count(self._h))
+        out["phase_reservation_bytes"] = int(lib.wv_phase_reservation_bytes(self._h))
+        return out
+
     # --------------------------------------------------------- allocation --
     def empty_coefficients(self, n=3):
         torch = _torch()
@@ -327,6 +396,47 @@ class WVTransformConstantStratificationKernel:
             self._dev(out, shape, "torch.float64"), self._stream(stream)))
         return out
 
+    def nonlinear_flux_with_advection_fields(self, Ap, Am, A0, t=0.0, t0=0.0, out=None, uvw=None, stream=None):
+        """nonlinearFluxWithAdvectionFields (...Kernel.cpp:1148-1158): ((Fp, Fm, F0), uvw [3, Nz, Ny, Nx])."""
+        torch = _torch()
+        if out is None:
+            out = self.empty_coefficients()
+        shape = (3,) + self.spatial_shape
+        if uvw is None:
+            uvw = torch.zeros(shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        _check(library().wv_nonlinear_flux_with_advection_fields_device(
+            self._h, float(t), float(t0), self._coef(Ap), self._coef(Am), self._coef(A0),
+            *[self._coef(o) for o in out], self._dev(uvw, shape, "torch.float64"), self._stream(stream)))
+        return tuple(out), uvw
+
+    def nonlinear_flux_using_advection_fields(self, Ap, Am, A0, uvw, t=0.0, t0=0.0, out=None, stream=None):
+        """nonlinearFluxUsingAdvectionFields (...Kernel.cpp:1160-1171): the flux
+        with the given advecting fields uvw [3, Nz, Ny, Nx] in the products."""
+        if out is None:
+            out = self.empty_coefficients()
+        _check(library().wv_nonlinear_flux_using_advection_fields_device(
+            self._h, float(t), float(t0), self._coef(Ap), self._coef(Am), self._coef(A0),
+            self._dev(uvw, (3,) + self.spatial_shape, "torch.float64"), *[self._coef(o) for o in out],
+            self._stream(stream)))
+        return tuple(out)
+
+    def transform_state_field_derivatives(self, Ap, Am, A0, field, t=0.0, t0=0.0, out=None, stream=None):
+        """transformStateFieldDerivatives (...Kernel.cpp:719-742): (d/dx, d/dy,
+        d/dz) of field "u", "v", "w" or "eta" as a (3, Nz, Ny, Nx) tensor."""
+        torch = _torch()
+        shape = (3,) + self.spatial_shape
+        if out is None:
+            out = torch.zeros(shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        f = DYNAMICAL_FIELDS[field] if isinstance(field, str) else int(field)
+        _check(library().wv_transform_state_field_derivatives_device(
+            self._h, float(t), float(t0), self._coef(Ap), self._coef(Am), self._coef(A0), f,
+            self._dev(out, shape, "torch.float64"), self._stream(stream)))
+        return out
+
+    def prepare_scalar_advection(self):
+        """prepareScalarAdvection (...Kernel.cpp:1004-1006)."""
+        _check(library().wv_prepare_scalar_advection(self._h))
+
     def advect_f_grid_scalar(self, scalar, advection_fields, should_antialias=True, out=None, stream=None):
         """advectFGridScalar (...Kernel.cpp:1032-1138): -(u, v, w) . grad(scalar)
         of an F-grid scalar [Nz, Ny, Nx] (the reference's [Nx, Ny, Nz]) with the
@@ -373,6 +483,23 @@ class WVTransformConstantStratificationKernel:
         _check(library().wv_rk4_advance_device(self._h, self._coef(Ap), self._coef(Am), self._coef(A0), C.byref(tt),
                                                float(t0), float(h), int(nsteps), self._stream(stream)))
         return tt.value
+
+    def rk4_advance_to_time(self, Ap, Am, A0, t, final_time, h, t0=0.0, stream=None) -> float:
+        """WVFixedStepRK4::advanceToTime (WVRungeKutta.cpp:464-480) in place; returns final_time."""
+        tt = C.c_double(float(t))
+        _check(library().wv_rk4_advance_to_time_device(self._h, self._coef(Ap), self._coef(Am), self._coef(A0),
+                                                       C.byref(tt), float(t0), float(final_time), float(h),
+                                                       self._stream(stream)))
+        return tt.value
+
+    def rk4_counters(self) -> dict:
+        n, h = C.c_uint64(), C.c_double()
+        _check(library().wv_rk4_counters(self._h, C.byref(n), C.byref(h)))
+        return {"stepCount": int(n.value), "lastStepSize": float(h.value)}
+
+    def inject_launch_failure(self, n: int):
+        """Test hook: the n-th next flux-pipeline launch fails (0 disarms)."""
+        _check(library().wv_debug_inject_launch_failure(self._h, int(n)))
 
     def total_energy(self, Ap, Am, A0, stream=None) -> float:
         e = C.c_double()

@@ -7,7 +7,8 @@ not reproducible outside MATLAB):
 * every draw comes from a counter-based SplitMix64 hash of
   (seed, array id, coefficient index), so the state is identical on any
   machine and independent of evaluation order;
-* coefficient magnitudes are ``(Kh/Kh_min)**slope`` (slope -2 by default; the
+* coefficient magnitudes are ``(Kh/Kh_min)**slope`` (slope -2 by default,
+  ``geo_slope`` for the geostrophic A0, default the same; the
   Kh = 0 column uses the Kh_min value), with uniform random phases, for
   vertical modes ``j <= jmax`` and only where the mode exists (its
   total-energy factor is nonzero);
@@ -18,7 +19,8 @@ not reproducible outside MATLAB):
   energy, with the closed-form total-energy factors;
 * ``scale_to_rms_speed`` then rescales all three arrays so that the
   volume-averaged horizontal speed ``sqrt(<u^2 + v^2>)`` (trapezoidal
-  weights in z, uniform in x and y) equals the requested value.
+  weights in z, or the given quadrature weights, uniform in x and y) equals
+  the requested value.
 
 Arrays are numpy complex128 with shape (Nkl, Nj), i.e. the [Nj, Nkl]
 column-major coefficient layout of CompiledKernel.
@@ -48,7 +50,7 @@ def uniform(seed: int, stream: int, n: int) -> np.ndarray:
 
 
 def random_flow_shape(Kh: np.ndarray, Nj: int, ApmTE: np.ndarray, A0TE: np.ndarray,
-                      seed: int, jmax: int, slope: float = -2.0):
+                      seed: int, jmax: int, slope: float = -2.0, geo_slope: float | None = None):
     """Unit-scaled Ap, Am, A0 with equal wave and geostrophic total energy.
 
     Kh: [Nkl] horizontal wavenumber magnitude per WV mode.
@@ -58,12 +60,13 @@ def random_flow_shape(Kh: np.ndarray, Nj: int, ApmTE: np.ndarray, A0TE: np.ndarr
     M = Nkl * Nj
     kh = np.asarray(Kh, np.float64)
     khmin = kh[kh > 0].min() if np.any(kh > 0) else 1.0
-    amp_mode = np.where(kh > 0, (np.maximum(kh, khmin) / khmin) ** slope, 1.0)
-    amp = np.repeat(amp_mode[:, None], Nj, axis=1)
     jj = np.repeat(np.arange(Nj)[None, :], Nkl, axis=0)
     active = jj <= jmax
     out = []
     for stream, te in ((1, ApmTE), (2, ApmTE), (3, A0TE)):
+        sl = slope if stream < 3 or geo_slope is None else geo_slope
+        amp_mode = np.where(kh > 0, (np.maximum(kh, khmin) / khmin) ** sl, 1.0)
+        amp = np.repeat(amp_mode[:, None], Nj, axis=1)
         ph = 2.0 * np.pi * uniform(seed, stream, M).reshape(Nkl, Nj)
         a = amp * np.exp(1j * ph)
         a[(~active) | (np.asarray(te).reshape(Nkl, Nj) <= 0)] = 0.0
@@ -91,17 +94,18 @@ def trapezoid_weights(Nz: int, Lz: float) -> np.ndarray:
     return w
 
 
-def rms_speed(fields: np.ndarray, Lz: float) -> float:
-    """sqrt(<u^2+v^2>) of fields shaped (nfield>=2, Nz, Ny, Nx)."""
+def rms_speed(fields: np.ndarray, Lz: float, weights: np.ndarray | None = None) -> float:
+    """sqrt(<u^2+v^2>) of fields shaped (nfield>=2, Nz, Ny, Nx); weights: the
+    vertical quadrature weights (summing to Lz), trapezoidal by default."""
     u, v = fields[0], fields[1]
     Nz = u.shape[0]
-    w = trapezoid_weights(Nz, Lz)
+    w = trapezoid_weights(Nz, Lz) if weights is None else np.asarray(weights, np.float64)
     s = ((u * u + v * v).mean(axis=(1, 2)) * w).sum() / Lz
     return float(np.sqrt(s))
 
 
-def scale_factor_for_rms(fields: np.ndarray, Lz: float, target: float) -> float:
-    r = rms_speed(fields, Lz)
+def scale_factor_for_rms(fields: np.ndarray, Lz: float, target: float, weights: np.ndarray | None = None) -> float:
+    r = rms_speed(fields, Lz, weights)
     return target / r if r > 0 else 1.0
 
 
