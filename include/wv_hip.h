@@ -285,6 +285,18 @@ wv_status wv_rk4_counters(const wv_kernel* kernel, uint64_t* stepCount, double* 
 wv_status wv_total_energy_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0,
                                  double* energy, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate or
+ * synchronize (work under HIP graph capture, serving loops): the reduction of
+ * wv_total_energy_device with its partial sums in a library-wide static device
+ * array (1024 doubles per GPU, held until the process ends) instead of the
+ * handle's workspace, and the result written to the device pointer `energy`.
+ * Nothing is allocated, copied to the host or synchronized; the work is
+ * enqueued on `stream`.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (different streams, no ordering between them) race.
+ * Coefficient counts above INT_MAX - 2^18 return WV_SIZE_OVERFLOW. */
+wv_status wv_total_energy_no_alloc_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0,
+                                          double* energy, void* stream);
+
 /* --- adjoint (vector-Jacobian products) ---------------------------------------
  * No counterpart in the reference, which has no adjoint.  Cotangents use the
  * real inner product <a, b> = Re sum conj(a) b over the coefficient arrays (and

@@ -31,10 +31,12 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -1708,6 +1710,14 @@ __global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+// Scratch of the allocation-free energy entry point (wv_total_energy_no_alloc_device):
+// the partial sums of its reduction, one per workgroup of k_energy_partial.
+// A file-scope device array, so each GPU holds one copy for the life of the
+// process; host code reaches it only through the per-device address taken
+// with hipGetSymbolAddress at create.
+constexpr int kEnergyPartials = 1024;
+static __device__ __align__(256) double scratch[kEnergyPartials];
+
 #include "wv_adjoint_kernels.hpp"
 
 // ------------------------------------------------------------------- host --
@@ -2117,7 +2127,7 @@ struct KernelImpl {
         hipCheck(hipMemset(P.p, 0, P.bytes), "hipMemset");
         stage.alloc(3 * d.M * sizeof(cx<double>));
         weighted.alloc(3 * d.M * sizeof(cx<double>));
-        partial.alloc(1024 * sizeof(double));
+        partial.alloc(kEnergyPartials * sizeof(double));
         scalar.alloc(16 * sizeof(double));
         hipCheck(hipDeviceSynchronize(), "setup");
     }
@@ -2758,15 +2768,26 @@ struct KernelImpl {
         });
     }
 
-    double energy(const double* Ap, const double* Am, const double* A0, hipStream_t s) {
+    // Workgroups of the energy reduction: one partial sum each.
+    int energyBlocks() const {
+        return std::max(1, std::min(kEnergyPartials, static_cast<int>((d.M + 255) / 256)));
+    }
+    // The total-energy reduction with its partial sums in `partials`
+    // (energyBlocks() of them) and the result in `out`, both device memory.
+    void energyInto(const double* Ap, const double* Am, const double* A0, double* partials, double* out,
+                    hipStream_t s) const {
         const int M = static_cast<int>(d.M);
-        const int blocks = std::max(1, std::min(1024, (M + 255) / 256));
+        const int blocks = energyBlocks();
         hipLaunchKernelGGL(k_energy_partial, dim3(blocks), dim3(256), 0, s, reinterpret_cast<const cx<double>*>(Ap),
                            reinterpret_cast<const cx<double>*>(Am), reinterpret_cast<const cx<double>*>(A0),
-                           ApmTE.as<double>(), A0TE.as<double>(), M, partial.as<double>());
+                           ApmTE.as<double>(), A0TE.as<double>(), M, partials);
         hipCheck(hipGetLastError(), "energy");
-        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, partial.as<double>(), blocks, scalar.as<double>());
+        hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, s, partials, blocks, out);
         hipCheck(hipGetLastError(), "energy");
+    }
+
+    double energy(const double* Ap, const double* Am, const double* A0, hipStream_t s) {
+        energyInto(Ap, Am, A0, partial.as<double>(), scalar.as<double>(), s);
         double e = 0.0;
         hipCheck(hipMemcpyAsync(&e, scalar.p, sizeof(double), hipMemcpyDeviceToHost, s), "energy copy");
         hipCheck(hipStreamSynchronize(s), "energy sync");
@@ -3176,6 +3197,29 @@ wv_status execute(KernelImpl& k, void* stream, const char* busy, F&& body) {
 }
 constexpr const char* kNotReentrant = "Kernel operations are not reentrant.";
 
+// Device address of `scratch` on each GPU, taken once per device at create
+// with that device current (the array's name in host code is a host-side
+// stand-in, never a device address).
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+double* gScratch[kMaxDevices] = {};
+
+void registerScratch(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        throw Failure{WV_UNSUPPORTED_OPERATION, "Device index beyond the library's per-device scratch table."};
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (gScratch[device]) return;
+    void* ptr = nullptr;
+    hipCheck(hipSetDevice(device), "hipSetDevice");
+    hipCheck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(wvamd::scratch)), "hipGetSymbolAddress(scratch)");
+    gScratch[device] = static_cast<double*>(ptr);
+}
+
+double* scratchOf(int device) {
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    return device >= 0 && device < kMaxDevices ? gScratch[device] : nullptr;
+}
+
 bool overlaps(const void* a, size_t an, const void* b, size_t bn) {
     if (!a || !b || an == 0 || bn == 0) return false;
     const auto x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
@@ -3235,6 +3279,7 @@ wv_status wv_create(const wv_config* config, wv_precision precision, int device,
         if (device < 0 || device >= count) return fail(WV_INVALID_CONFIGURATION, "Invalid HIP device index.");
         std::unique_ptr<wv_kernel> k(new wv_kernel());
         k->impl.setup(*config, precision, device);
+        registerScratch(device);
         *kernel = k.release();
         return WV_SUCCESS;
     });
@@ -3272,6 +3317,7 @@ wv_status wv_create_hydrostatic(const wv_config* config, const wv_vertical_opera
         st.QG.assign(ops->QG, ops->QG + nj * nz);
         std::unique_ptr<wv_kernel> k(new wv_kernel());
         k->impl.setup(*config, precision, device, &st);
+        registerScratch(device);
         *kernel = k.release();
         return WV_SUCCESS;
     });
@@ -3742,6 +3788,25 @@ wv_status wv_total_energy_device(wv_kernel* kernel, const double* Ap, const doub
     if (!Ap || !Am || !A0) return fail(WV_INVALID_POINTER, "Total energy state has a null data pointer.");
     KernelImpl& k = kernel->impl;
     return execute(k, stream, kNotReentrant, [&](hipStream_t s) { *energy = k.energy(Ap, Am, A0, s); });
+}
+
+wv_status wv_total_energy_no_alloc_device(wv_kernel* kernel, const double* Ap, const double* Am, const double* A0,
+                                          double* energy, void* stream) {
+    if (!kernel || !energy) return fail(WV_INVALID_POINTER, "Total energy requires a kernel and a device output.");
+    if (!Ap || !Am || !A0) return fail(WV_INVALID_POINTER, "Total energy state has a null data pointer.");
+    const KernelImpl& k = kernel->impl;
+    // k_energy_partial strides an int index over M coefficients
+    if (k.d.M > static_cast<size_t>(INT_MAX) - static_cast<size_t>(256 * wvamd::kEnergyPartials))
+        return fail(WV_SIZE_OVERFLOW, "Coefficient count exceeds the allocation-free energy reduction's maximum.");
+    if (k.energyBlocks() > wvamd::kEnergyPartials)
+        return fail(WV_SIZE_OVERFLOW, "Energy reduction needs more partial sums than the static scratch holds (1024).");
+    return guarded([&] {
+        double* partials = scratchOf(k.device);
+        if (!partials) return fail(WV_INVALID_CONFIGURATION, "No scratch address for this kernel's device.");
+        hipCheck(hipSetDevice(k.device), "hipSetDevice");
+        k.energyInto(Ap, Am, A0, partials, energy, S(stream));
+        return WV_SUCCESS;
+    });
 }
 
 wv_status wv_set_stage_instrumentation(wv_kernel* kernel, int enabled) {

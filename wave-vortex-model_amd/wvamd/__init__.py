@@ -104,7 +104,7 @@ EXPORTED_SYMBOLS = [
     "wv_optimization_implementation_identifier", "wv_coefficient_worker_count", "wv_phase_reservation_bytes",
     "wv_nonlinear_flux_using_advection_fields_device", "wv_prepare_scalar_advection",
     "wv_transform_state_field_derivatives_device", "wv_rk4_advance_to_time_device", "wv_rk4_counters",
-    "wv_debug_inject_launch_failure",
+    "wv_debug_inject_launch_failure", "wv_total_energy_no_alloc_device",
 ]
 
 IDENTIFIERS = ("engine_identifier", "engine_library_identity", "nonlinear_flux_schedule_identifier",
@@ -218,6 +218,8 @@ def library() -> C.CDLL:
     lib.wv_rk4_advance_to_time_device.argtypes = [vp, vp, vp, vp, dp, C.c_double, C.c_double, C.c_double, vp]
     lib.wv_rk4_counters.restype = C.c_int
     lib.wv_rk4_counters.argtypes = [vp, C.POINTER(C.c_uint64), dp]
+    lib.wv_total_energy_no_alloc_device.restype = C.c_int
+    lib.wv_total_energy_no_alloc_device.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.wv_debug_inject_launch_failure.restype = C.c_int
     lib.wv_debug_inject_launch_failure.argtypes = [vp, C.c_long]
     _lib = lib
@@ -543,6 +545,13 @@ class WVTransformConstantStratificationKernel:
         _check(library().wv_rk4_adjoint_device(
             self._h, *[self._coef(a) for a in (Ap, Am, A0)], float(t), float(t0), float(h), int(nsteps),
             int(interval), *[self._coef(a) for a in (Lp, Lm, L0)], self._stream(stream)))
+
+    def total_energy_into(self, Ap, Am, A0, out, stream=None):
+        """Total energy written to the one-element float64 device tensor out,
+        allocating nothing (wv_total_energy_no_alloc_device)."""
+        _check(library().wv_total_energy_no_alloc_device(self._h, self._coef(Ap), self._coef(Am), self._coef(A0),
+                                                         self._dev(out, (1,), "torch.float64"), self._stream(stream)))
+        return out
 
     def set_stage_instrumentation(self, enabled: bool):
         _check(library().wv_set_stage_instrumentation(self._h, int(bool(enabled))))
