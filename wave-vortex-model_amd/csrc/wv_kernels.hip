@@ -779,6 +779,7 @@ template <int NT>
 struct XMap;
 template <>
 struct XMap<4> {
+    static constexpr int targets = 4;
     static constexpr int pairs = 8;
     static constexpr int srcA[8] = {0, 0, 1, 2, 3, 8, 2, 11};
     static constexpr int ikA[8] = {0, 1, 1, 1, 1, 0, 0, 0};
@@ -788,12 +789,31 @@ struct XMap<4> {
 };
 template <>
 struct XMap<3> {
+    static constexpr int targets = 3;
     static constexpr int pairs = 7;
     static constexpr int srcA[7] = {0, 0, 1, 3, 7, 2, 9};
     static constexpr int ikA[7] = {0, 1, 1, 1, 0, 0, 0};
     static constexpr int srcB[7] = {1, 4, 5, 6, 8, -1, -1};
     static constexpr int uvw[3] = {0, 1, 10};
     static constexpr int prod[3][3] = {{2, 3, 8}, {4, 5, 9}, {6, 7, 12}};
+};
+// Nonhydrostatic with d/dz w from continuity (the mixed-precision flux of the
+// constant-N transform): the wave-vortex modes are divergence-free one by one
+// (i (k UAp + l VAp) + m WAp = 0, i (k UA0 + l VA0) = 0), so d/dz w =
+// -(d/dx u + d/dy v) and the d/dz w channel is neither synthesized nor
+// transformed.  Half-spectrum channels u v w eta, dy u..eta, dz u, dz v,
+// dz eta:
+//   (U, V) (dxu, dyu) (dxv, dyv) (dxw, dyw) (dxeta, dyeta) (dzu, dzv) (W, dzeta)
+// prod entry -1: -(dxu + dyv) (spatial channels 2 and 5).
+template <>
+struct XMap<5> {
+    static constexpr int targets = 4;
+    static constexpr int pairs = 7;
+    static constexpr int srcA[7] = {0, 0, 1, 2, 3, 8, 2};
+    static constexpr int ikA[7] = {0, 1, 1, 1, 1, 0, 0};
+    static constexpr int srcB[7] = {1, 4, 5, 6, 7, 9, 10};
+    static constexpr int uvw[3] = {0, 1, 12};
+    static constexpr int prod[4][3] = {{2, 3, 10}, {4, 5, 11}, {6, 7, -1}, {8, 9, 13}};
 };
 
 // Per (y, z) row: c2r of the spatial channels along x (two real transforms
@@ -931,8 +951,10 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
         }
         T F[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
-            F[t] = -(U * ch[XM::prod[t][0]] + V * ch[XM::prod[t][1]] + W * ch[XM::prod[t][2]]);
+        for (int t = 0; t < XM::targets; ++t) {
+            const T dz = XM::prod[t][2] >= 0 ? ch[XM::prod[t][2] >= 0 ? XM::prod[t][2] : 0] : -(ch[2] + ch[5]);
+            F[t] = -(U * ch[XM::prod[t][0]] + V * ch[XM::prod[t][1]] + W * dz);
+        }
         // F_eta shares a transform with F_w: scaled by etaScale (a power of two
         // near N0) to the size of an acceleration, removed exactly below
         rb[xl] = {F[0], F[1]};
@@ -957,8 +979,8 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
         const cx<T> Zb = rb[ldsIndex(k == 0 ? 0 : Nx - k, 1)];
         const T h = T(0.5), hb = q == 1 ? h / etaScale : h;
         const size_t o = static_cast<size_t>(row) * nkxP + k;
-        if (2 * q < NT) P[(2 * q) * chStride + o] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
-        if (2 * q + 1 < NT) P[(2 * q + 1) * chStride + o] = {(Za.im + Zb.im) * hb, (Zb.re - Za.re) * hb};
+        if (2 * q < XM::targets) P[(2 * q) * chStride + o] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
+        if (2 * q + 1 < XM::targets) P[(2 * q + 1) * chStride + o] = {(Za.im + Zb.im) * hb, (Zb.re - Za.re) * hb};
     }
 }
 
@@ -1496,7 +1518,15 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
     cx<double>* outs[3] = {ep.out0, ep.out1, ep.out2};
     cx<double>* ys[3] = {ep.y0, ep.y1, ep.y2};
     cx<double>* sts[3] = {ep.st0, ep.st1, ep.st2};
-    cx<double>* ws[3] = {ep.w0, ep.w1, ep.w2};
+    // The weighted flux sum k1 + 2 k2 + 2 k3 + k4 is stored in the flux's own
+    // precision T (the fluxes of the mixed path carry FP32 accuracy; the
+    // state update y + h/6 w below is FP64 in both).
+    cx<T>* ws[3] = {reinterpret_cast<cx<T>*>(ep.w0), reinterpret_cast<cx<T>*>(ep.w1), reinterpret_cast<cx<T>*>(ep.w2)};
+    auto wload = [&](int a) {
+        const cx<T> v = ws[a][idx];
+        return cx<double>{static_cast<double>(v.re), static_cast<double>(v.im)};
+    };
+    auto wstore = [&](int a, cx<double> v) { ws[a][idx] = {static_cast<T>(v.re), static_cast<T>(v.im)}; };
     // RK4 stage arithmetic of WVFixedStepRK4::step with scaledSum
     // (value + scale*increment, WVRungeKutta.cpp:15-19), rounded per
     // operation as on the CPU.
@@ -1510,18 +1540,18 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
         switch (ep.mode) {
             case kFluxOut: outs[a][idx] = kv; break;
             case kRk1:
-                ws[a][idx] = kv;
+                wstore(a, kv);
                 next[a] = axpy(ys[a][idx], kv, ep.scale);
                 if (!ep.emitFields) sts[a][idx] = next[a];
                 break;
             case kRk2:
             case kRk3:
-                ws[a][idx] = axpy(ws[a][idx], kv, 2.0);
+                wstore(a, axpy(wload(a), kv, 2.0));
                 next[a] = axpy(ys[a][idx], kv, ep.scale);
                 if (!ep.emitFields) sts[a][idx] = next[a];
                 break;
             default: {
-                const cx<double> w = axpy(ws[a][idx], kv, 1.0);
+                const cx<double> w = axpy(wload(a), kv, 1.0);
                 next[a] = axpy(ys[a][idx], w, ep.scale);
                 ys[a][idx] = next[a];
                 break;
@@ -2137,11 +2167,16 @@ struct KernelImpl {
     //   u v w eta, d/dy of each target, d/dz of each target
     // (the d/dx channels are formed in the x transform).  For an arbitrary
     // stratification the eta target's z channel is d/dz eta + eta dLnN2.
-    SynthPlan fluxPlan() const {
+    // derived: d/dz w from continuity (XMap<5>; u v w eta, dy, dz u, dz v,
+    // dz eta: 11 channels).
+    SynthPlan fluxPlan(bool derived = false) const {
         SynthPlan sp{};
         sp.valueScale = 0.5;
         sp.ngroups = 2;
-        if (d.stratified) {
+        if (derived) {
+            sp.g[0] = {0, 3, {{0, 0, 0, -1, 4}, {1, 0, 1, -1, 5}, {3, 1, 10, -1, -1}, {}}};
+            sp.g[1] = {1, 4, {{2, 0, 2, -1, 6}, {3, 0, 3, -1, 7}, {0, 1, 8, -1, -1}, {1, 1, 9, -1, -1}}};
+        } else if (d.stratified) {
             sp.valueScale = 1.0;
             sp.ngroups = 4;
             sp.g[0] = {0, 2, {{0, 0, 0, -1, 4}, {1, 0, 1, -1, 5}, {}, {}}};
@@ -2161,9 +2196,9 @@ struct KernelImpl {
     // The flux plan without the y-derivative outputs, which the inverse y
     // transform forms instead (plan in *yp; false when the storage rows'
     // y wavenumbers are not one per row).
-    bool fluxPlanLean(SynthPlan& sp, YDerivPlan& yp) const {
+    bool fluxPlanLean(SynthPlan& sp, YDerivPlan& yp, bool derived = false) const {
         if (!lrowValid) return false;
-        sp = fluxPlan();
+        sp = fluxPlan(derived);
         yp = YDerivPlan{};
         int n = 0;
         for (int g = 0; g < sp.ngroups; ++g)
@@ -2356,19 +2391,27 @@ struct KernelImpl {
                            zeroRow.as<cx<T>>(), es, adv);
     }
 
+    // derived: the d/dz w channel is not in G (XMap<5>).
     template <typename T>
-    void launchXfused(hipStream_t s, const double* adv = nullptr) {
-        const double bytes = (static_cast<double>(NH) + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>) +
-                             (adv ? 3.0 * d.R * sizeof(double) : 0.0);
+    void launchXfused(hipStream_t s, const double* adv = nullptr, bool derived = false) {
+        const double nh = derived ? NH - 1 : NH;
+        const double bytes = (nh + NT) * d.c.Nz * d.c.Ny * d.nkx * sizeof(cx<T>) + (adv ? 3.0 * d.R * sizeof(double) : 0.0);
         timed("xfused", bytes, s, [&] {
             if (adv)
-                NT == 4 ? launchXfusedExt<T, 4>(adv, s) : launchXfusedExt<T, 3>(adv, s);
+                derived ? launchXfusedExt<T, 5>(adv, s) : NT == 4 ? launchXfusedExt<T, 4>(adv, s) : launchXfusedExt<T, 3>(adv, s);
+            else if (derived)
+                launchXfusedFor<T, 5>(s);
             else if (NT == 4)
                 launchXfusedFor<T, 4>(s);
             else
                 launchXfusedFor<T, 3>(s);
         });
     }
+
+    // The mixed-precision flux of the constant-N nonhydrostatic transform
+    // forms d/dz w from continuity (XMap<5>); the FP64 flux synthesizes every
+    // derivative, as the reference's dataflow does.
+    bool deriveDzW() const { return precision == WV_PRECISION_MIXED && NT == 4 && !d.stratified && lrowValid; }
 
     // Vertical analysis of the transformed products into Tg.
     template <typename T>
@@ -2419,11 +2462,13 @@ struct KernelImpl {
         // 144 B); stages 2, 3 read w, y and write w (and the stage: 144 or
         // 192 B); stage 4 reads w, y and writes y (144 B); with the next
         // input's spectra four spectra
-        const double stBytes = ep.emitFields ? 0.0 : 48.0;
+        // (y and the stage state are FP64 triples, 48 B; the weighted sum is
+        // a triple in T, 3 sizeof(cx<T>))
+        const double stBytes = ep.emitFields ? 0.0 : 48.0, wB = 3.0 * sizeof(cx<T>);
         const double epBytes = ep.mode == kFluxOut ? 48.0
-                               : ep.mode == kRk1   ? 96.0 + stBytes
-                               : ep.mode == kRk4   ? 144.0
-                                                   : 144.0 + stBytes;
+                               : ep.mode == kRk1   ? 48.0 + wB + stBytes
+                               : ep.mode == kRk4   ? 96.0 + wB
+                                                   : 48.0 + 2.0 * wB + stBytes;
         const double fieldBytes = (ep.mode != kFluxOut && ep.emitFields) ? 4.0 * sizeof(cx<T>) : 0.0;
         const double bytes = static_cast<double>(M) * (NT * sizeof(cx<T>) + epBytes + fieldBytes);
         // the kernel reads y and w in every mode
@@ -2450,7 +2495,8 @@ struct KernelImpl {
         if (!fieldsReady) launchFields<T>(Ap, Am, A0, elapsed, s);
         SynthPlan lean;
         YDerivPlan yp;
-        if (fluxPlanLean(lean, yp)) {
+        const bool derived = deriveDzW();
+        if (fluxPlanLean(lean, yp, derived)) {
             int nsrc = 0;
             for (int g = 0; g < lean.ngroups; ++g) nsrc += lean.g[g].nch;
             launchSynth<T>(lean, s);
@@ -2459,7 +2505,7 @@ struct KernelImpl {
             launchSynth<T>(fluxPlan(), s);
             launchYfft<T>(H, G, NH, +1, s, "yfft_inv", true, false);
         }
-        launchXfused<T>(s, adv);
+        launchXfused<T>(s, adv, derived);
         launchYfft<T>(P, Q, NT, -1, s, "yfft_fwd", false, true);
         launchProj<T>(s);
         launchStage<T>(elapsed, ep, s);
