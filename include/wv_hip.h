@@ -250,6 +250,18 @@ wv_status wv_transform_state_field_derivatives_device(wv_kernel* kernel, double 
                                                       const double* Am, const double* A0, int field,
                                                       double* derivatives, void* stream);
 
+/* Available potential vorticity of the state at time t, apv [Nx,Ny,Nz]:
+ *   zeta_z - zeta_x d/dx eta - zeta_y d/dy eta - (zeta_z + f) d/dz eta
+ * (Operations/APVOperation.m:15-26; zeta as in
+ * @WVTransform/classDefinedOperationForKnownVariable.m:126-137).  The
+ * reference forms it from eta_true (Operations/EtaTrueOperation.m), which for
+ * the constant-N background used as the no-motion profile (the default,
+ * @WVTransformConstantStratification/WVTransformConstantStratification.m:110)
+ * is eta itself; arbitrary-stratification kernels return
+ * WV_UNSUPPORTED_OPERATION. */
+wv_status wv_apv_device(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                        const double* A0, double* apv, void* stream);
+
 /* transformToSpatialDomainWithFAllDerivatives / ...GAllDerivatives (...Kernel.cpp:819-863):
  * the F-family (cosine) or G-family (sine) field Apm*waveScale + A0*geostrophicScale
  * and its x, y, z derivatives, fields [Nx,Ny,Nz,4] = (value, d/dx, d/dy, d/dz). */

@@ -194,6 +194,20 @@ class _Lib:
             raise RuntimeError(f"state_field_derivatives failed: {rc}")
         return out
 
+    def apv(self, Ap, Am, A0, t=0.0, t0=0.0):
+        """Available potential vorticity, Operations/APVOperation.m:15-26:
+        zeta_z - zeta_x eta_x - zeta_y eta_y - (zeta_z + f) eta_z with
+        zeta_x = w_y - v_z, zeta_y = u_z - w_x, zeta_z = v_x - u_y
+        (@WVTransform/classDefinedOperationForKnownVariable.m:126-137) and
+        eta_true = eta (the constant-N background as the no-motion profile,
+        for which EtaTrueOperation.m's inversion returns z - eta), from this
+        library's state-field derivatives.  Returns (Nz, Ny, Nx)."""
+        du, dv, dw, de = (self.state_field_derivatives(Ap, Am, A0, f, t, t0) for f in range(4))
+        zx = dw[1] - dv[2]
+        zy = du[2] - dw[0]
+        zz = dv[0] - du[1]
+        return zz - zx * de[0] - zy * de[1] - (zz + self.coriolis()) * de[2]
+
     def inverse_uvweta(self, Ap, Am, A0, t=0.0, t0=0.0):
         Ap, Am, A0 = self._coef(Ap), self._coef(Am), self._coef(A0)
         out = self._fields(4)

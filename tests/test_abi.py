@@ -102,6 +102,7 @@ def test_new_entry_points_reject_null_kernel_without_gpu_work():
     assert lib.wv_debug_inject_launch_failure(None, 1) == 3
     e = ctypes.c_double()
     assert lib.wv_total_energy_device(None, None, None, None, ctypes.byref(e), None) == 3
+    assert lib.wv_apv_device(None, 0.0, 0.0, None, None, None, None, None) == 3
 
 
 def test_metrics_struct_mirrors_reference_members():

@@ -988,13 +988,21 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
 // transforms).  A workgroup takes rpb consecutive (y, z) rows: its real
 // signals s = (row, channel) = (s / nch, s % nch) are paired two per complex
 // transform (2q real part, 2q + 1 imaginary part), so that small channel
-// counts still give the radix-16 passes several transforms.  PRODUCT (nch =
-// 3, the scalar advection's d/dx, d/dy, d/dz): out = -(u, v, w) . (the three
-// fields) of each row, with adv = [u, v, w] ([3][R]), instead of the fields.
-template <typename T, bool PRODUCT = false, int L2N = 0>
+// counts still give the radix-16 passes several transforms.  OP selects what
+// is written instead of the fields themselves:
+//   kXinvAdvect (nch = 3, the scalar advection's d/dx, d/dy, d/dz):
+//     out = -(u, v, w) . (the three fields) of each row, adv = [u, v, w] ([3][R]);
+//   kXinvApv (nch = 9, the derivative channels of kApvChannels): the
+//     available potential vorticity of each row, coriolis = f.
+enum XinvOp : int { kXinvFields = 0, kXinvAdvect = 1, kXinvApv = 2 };
+// Channels of the APV x transform: d/dy u, d/dz u, d/dx v, d/dz v, d/dx w,
+// d/dy w, d/dx eta, d/dy eta, d/dz eta.
+constexpr int kApvChannels = 9;
+template <typename T, int OP = kXinvFields, int L2N = 0>
 __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, double* __restrict__ out, int nch, int rpb,
                                               int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
-                                              const cx<T>* __restrict__ twX, const double* __restrict__ adv = nullptr) {
+                                              const cx<T>* __restrict__ twX, const double* __restrict__ adv = nullptr,
+                                              double coriolis = 0.0) {
     const int Nx = L2N > 0 ? 1 << L2N : NxArg;  // L2N > 0: the length fixed at compile time
     const int log2Nx = L2N > 0 ? L2N : log2NxArg;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1016,11 +1024,27 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
     __syncthreads();
     fftLds<T, +1, L2N>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
-    if constexpr (PRODUCT) {
-        auto field = [&](int sg, int x) {
-            const cx<T> v = buf[(sg / 2) * NxP + ldsIndex(x, 1)];
-            return static_cast<double>((sg & 1) ? v.im : v.re);
-        };
+    auto field = [&](int sg, int x) {
+        const cx<T> v = buf[(sg / 2) * NxP + ldsIndex(x, 1)];
+        return static_cast<double>((sg & 1) ? v.im : v.re);
+    };
+    if constexpr (OP == kXinvApv) {
+        // APVOperation.m:15-26 with eta_true = eta: zeta_z - zeta_x eta_x -
+        // zeta_y eta_y - (zeta_z + f) eta_z, zeta_x = w_y - v_z, zeta_y =
+        // u_z - w_x, zeta_z = v_x - u_y (classDefinedOperationForKnownVariable.m:126-137)
+        for (int e = threadIdx.x; e < rpb * Nx; e += blockDim.x) {
+            const int r = e / Nx, x = e % Nx, row = row0 + r;
+            if (row >= nrows) continue;
+            const int c0 = kApvChannels * r;
+            const double uy = field(c0, x), uz = field(c0 + 1, x), vx = field(c0 + 2, x), vz = field(c0 + 3, x);
+            const double wx = field(c0 + 4, x), wy = field(c0 + 5, x);
+            const double ex = field(c0 + 6, x), ey = field(c0 + 7, x), ez = field(c0 + 8, x);
+            const double zx = wy - vz, zy = uz - wx, zz = vx - uy;
+            out[static_cast<size_t>(row) * Nx + x] = zz - zx * ex - zy * ey - (zz + coriolis) * ez;
+        }
+        return;
+    }
+    if constexpr (OP == kXinvAdvect) {
         for (int e = threadIdx.x; e < rpb * Nx; e += blockDim.x) {
             const int r = e / Nx, x = e % Nx, row = row0 + r;
             if (row >= nrows) continue;
@@ -2544,20 +2568,25 @@ struct KernelImpl {
     }
     template <typename T>
     void launchXinvRows(const cx<T>* G, double* out, int nch, int nkx, int nkxP, const cx<T>* tw, hipStream_t s) {
+        launchXinvOp<T, kXinvFields>(G, out, nch, nkx, nkxP, tw, s);
+    }
+    template <typename T, int OP>
+    void launchXinvOp(const cx<T>* G, double* out, int nch, int nkx, int nkxP, const cx<T>* tw, hipStream_t s,
+                      double coriolis = 0.0) {
         const int rpb = xRowsPerBlock(nch), ntr = (rpb * nch + 1) / 2;
         const int nrows = static_cast<int>(d.c.Ny * d.c.Nz);
         const size_t lds = static_cast<size_t>(ntr) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
-        auto fn = k_xinv<T, false, 0>;
+        auto fn = k_xinv<T, OP, 0>;
         switch (log2Exact(d.c.Nx)) {  // lengths 64 ... 512 compiled as such
-            case 6: fn = k_xinv<T, false, 6>; break;
-            case 7: fn = k_xinv<T, false, 7>; break;
-            case 8: fn = k_xinv<T, false, 8>; break;
-            case 9: fn = k_xinv<T, false, 9>; break;
+            case 6: fn = k_xinv<T, OP, 6>; break;
+            case 7: fn = k_xinv<T, OP, 7>; break;
+            case 8: fn = k_xinv<T, OP, 8>; break;
+            case 9: fn = k_xinv<T, OP, 9>; break;
             default: break;
         }
         hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256), lds, s, G, out,
                            nch, rpb, static_cast<int>(d.c.Nx), log2Exact(d.c.Nx), static_cast<int>(d.c.Ny),
-                           static_cast<int>(d.c.Nz), nkx, nkxP, tw, static_cast<const double*>(nullptr));
+                           static_cast<int>(d.c.Nz), nkx, nkxP, tw, static_cast<const double*>(nullptr), coriolis);
     }
 
     // ---- advectFGridScalar (FP64 in either precision, as the reference's)
@@ -2665,17 +2694,17 @@ struct KernelImpl {
             // c2r of d/dx, d/dy, d/dz with the product -(u, v, w) . grad in the epilogue
             const int rpb = xRowsPerBlock(3), ntr = (rpb * 3 + 1) / 2;
             const int nrows = Ny * Nz;
-            auto pfn = k_xinv<double, true, 0>;
+            auto pfn = k_xinv<double, kXinvAdvect, 0>;
             switch (log2Exact(d.c.Nx)) {
-                case 6: pfn = k_xinv<double, true, 6>; break;
-                case 7: pfn = k_xinv<double, true, 7>; break;
-                case 8: pfn = k_xinv<double, true, 8>; break;
-                case 9: pfn = k_xinv<double, true, 9>; break;
+                case 6: pfn = k_xinv<double, kXinvAdvect, 6>; break;
+                case 7: pfn = k_xinv<double, kXinvAdvect, 7>; break;
+                case 8: pfn = k_xinv<double, kXinvAdvect, 8>; break;
+                case 9: pfn = k_xinv<double, kXinvAdvect, 9>; break;
                 default: break;
             }
             hipLaunchKernelGGL(pfn, dim3(static_cast<unsigned>((nrows + rpb - 1) / rpb)), dim3(256),
                                static_cast<size_t>(ntr) * ldsLength(Nx) * sizeof(cx<double>), s, C, rhs, 3, rpb, Nx,
-                               log2Exact(d.c.Nx), Ny, Nz, scNxh, scNxhP, twx, adv);
+                               log2Exact(d.c.Nx), Ny, Nz, scNxh, scNxhP, twx, adv, 0.0);
         }
         if (antialias) {
             xfwd(rhs);
@@ -2733,6 +2762,38 @@ struct KernelImpl {
         });
         metrics.horizontalExecutionCount += 1;
         metrics.verticalExecutionCount += 3;
+        metrics.executionCount = metrics.horizontalExecutionCount + metrics.verticalExecutionCount;
+    }
+
+    // Available potential vorticity (Operations/APVOperation.m:15-26) of the
+    // state at time t, out [Nz][Ny][Nx].  eta_true = eta here: with the
+    // constant-N background as the no-motion profile (the default,
+    // shouldUseTrueNoMotionProfile = false) the density is linear in z, so
+    // the inversion of EtaTrueOperation.m returns z - eta exactly.  One
+    // synthesis of the nine derivative channels (kApvChannels order), the y
+    // inverse, and the x inverse forming the APV products in LDS.
+    template <typename T>
+    void apv(const double* Ap, const double* Am, const double* A0, double t, double t0, double* out,
+             hipStream_t s) {
+        launchFields<T>(Ap, Am, A0, t - t0, s);
+        SynthPlan sp{};
+        sp.valueScale = 0.5;
+        sp.ngroups = 4;
+        // cosine values: u -> d/dy (0), v -> d/dx (2); sine values: w -> d/dx,
+        // d/dy (4, 5), eta -> d/dx, d/dy (6, 7); z derivatives of the cosine
+        // fields on the sine operator (1, 3) and of eta on the cosine one (8)
+        sp.g[0] = {0, 2, {{0, 0, -1, -1, 0}, {1, 0, -1, 2, -1}, {}, {}}};
+        sp.g[1] = {1, 2, {{2, 0, -1, 4, 5}, {3, 0, -1, 6, 7}, {}, {}}};
+        sp.g[2] = {1, 2, {{0, 1, 1, -1, -1}, {1, 1, 3, -1, -1}, {}, {}}};
+        sp.g[3] = {0, 1, {{3, 1, 8, -1, -1}, {}, {}, {}}};
+        launchSynth<T>(sp, s);
+        launchYfft<T>(H, G, kApvChannels, +1, s, "yfft_inv", true, false);
+        timed("xapv", d.R * (8.0 + kApvChannels * 0.67 * sizeof(cx<T>)), s, [&] {
+            launchXinvOp<T, kXinvApv>(G.as<cx<T>>(), out, kApvChannels, static_cast<int>(d.nkx),
+                                      static_cast<int>(d.nkxP), twX.as<cx<T>>(), s, d.f);
+        });
+        metrics.horizontalExecutionCount += 1;
+        metrics.verticalExecutionCount += 4;
         metrics.executionCount = metrics.horizontalExecutionCount + metrics.verticalExecutionCount;
     }
 
@@ -3612,6 +3673,25 @@ wv_status wv_transform_state_field_derivatives_device(wv_kernel* kernel, double 
     return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
         if (k.precision == WV_PRECISION_FP64) k.stateDerivatives<double>(Ap, Am, A0, t, t0, field, derivatives, s);
         else k.stateDerivatives<float>(Ap, Am, A0, t, t0, field, derivatives, s);
+    });
+}
+
+wv_status wv_apv_device(wv_kernel* kernel, double t, double t0, const double* Ap, const double* Am,
+                        const double* A0, double* apv, void* stream) {
+    if (!kernel) return fail(WV_INVALID_POINTER, "null kernel");
+    KernelImpl& k = kernel->impl;
+    if (k.d.stratified)
+        return fail(WV_UNSUPPORTED_OPERATION,
+                    "APV needs eta_true, which equals eta only for the constant-N background profile.");
+    if (!apv) return fail(WV_INVALID_POINTER, "APV has a null data pointer.");
+    if (!Ap) return fail(WV_INVALID_POINTER, "Ap has a null data pointer.");
+    if (!Am) return fail(WV_INVALID_POINTER, "Am has a null data pointer.");
+    if (!A0) return fail(WV_INVALID_POINTER, "A0 has a null data pointer.");
+    if (!std::isfinite(t) || !std::isfinite(t0)) return fail(WV_INVALID_CONFIGURATION, "State times must be finite.");
+    if (auto st = validateBundleOwnership(k, apv, 1, Ap, Am, A0, "APV"); st != WV_SUCCESS) return st;
+    return execute(k, stream, kNotReentrant, [&](hipStream_t s) {
+        if (k.precision == WV_PRECISION_FP64) k.apv<double>(Ap, Am, A0, t, t0, apv, s);
+        else k.apv<float>(Ap, Am, A0, t, t0, apv, s);
     });
 }
 

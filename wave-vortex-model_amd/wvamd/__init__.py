@@ -104,7 +104,7 @@ EXPORTED_SYMBOLS = [
     "wv_optimization_implementation_identifier", "wv_coefficient_worker_count", "wv_phase_reservation_bytes",
     "wv_nonlinear_flux_using_advection_fields_device", "wv_prepare_scalar_advection",
     "wv_transform_state_field_derivatives_device", "wv_rk4_advance_to_time_device", "wv_rk4_counters",
-    "wv_debug_inject_launch_failure", "wv_total_energy_no_alloc_device",
+    "wv_debug_inject_launch_failure", "wv_total_energy_no_alloc_device", "wv_apv_device",
 ]
 
 IDENTIFIERS = ("engine_identifier", "engine_library_identity", "nonlinear_flux_schedule_identifier",
@@ -220,6 +220,8 @@ def library() -> C.CDLL:
     lib.wv_rk4_counters.argtypes = [vp, C.POINTER(C.c_uint64), dp]
     lib.wv_total_energy_no_alloc_device.restype = C.c_int
     lib.wv_total_energy_no_alloc_device.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.wv_apv_device.restype = C.c_int
+    lib.wv_apv_device.argtypes = [vp, C.c_double, C.c_double, vp, vp, vp, vp, vp]
     lib.wv_debug_inject_launch_failure.restype = C.c_int
     lib.wv_debug_inject_launch_failure.argtypes = [vp, C.c_long]
     _lib = lib
@@ -421,6 +423,19 @@ class WVTransformConstantStratificationKernel:
             self._dev(uvw, (3,) + self.spatial_shape, "torch.float64"), *[self._coef(o) for o in out],
             self._stream(stream)))
         return tuple(out)
+
+    def apv(self, Ap, Am, A0, t=0.0, t0=0.0, out=None, stream=None):
+        """Available potential vorticity (Operations/APVOperation.m:15-26,
+        eta_true = eta for the constant-N background) as an (Nz, Ny, Nx)
+        tensor."""
+        torch = _torch()
+        shape = self.spatial_shape
+        if out is None:
+            out = torch.zeros(shape, dtype=torch.float64, device=f"cuda:{self.device}")
+        _check(library().wv_apv_device(self._h, float(t), float(t0), self._coef(Ap), self._coef(Am),
+                                       self._coef(A0), self._dev(out, shape, "torch.float64"),
+                                       self._stream(stream)))
+        return out
 
     def transform_state_field_derivatives(self, Ap, Am, A0, field, t=0.0, t0=0.0, out=None, stream=None):
         """transformStateFieldDerivatives (...Kernel.cpp:719-742): (d/dx, d/dy,
