@@ -157,7 +157,31 @@ def host_flux_rate(k, state, calls=5):
 
 
 def cpu_baseline(cfg, state, dt, threads, steps=4):
-    """Oracle FP64 RK4 steps at the full configuration on the host cores."""
+    """The reference's CPU path where its kernel is built (oracle/_ref): one
+    classic RK4 step of the reference kernel (CompiledKernel compiled
+    unchanged, over the build-authored radix-2 OpenMP FFT engine; FFTW, its
+    production engine, is not installed) -- four nonlinearFlux calls and
+    WVFixedStepRK4's stage arithmetic (WVRungeKutta.cpp:335-462) in numpy.
+    Otherwise the oracle restatement's RK4 steps."""
+    import numpy as np
+    from oracle import ReferenceFast
+    if ReferenceFast.available():
+        ReferenceFast.set_threads(threads)
+        r = ReferenceFast(cfg)
+        y = [np.asarray(a, np.complex128).reshape(r.Nkl, r.Nj) for a in state]
+        r.nonlinear_flux(*y, t=0.0)  # warm-up: first touch of the scratch arena
+        t0 = time.perf_counter()
+        h, t = dt, 0.0
+        k1 = r.nonlinear_flux(*y, t=t)
+        k2 = r.nonlinear_flux(*[a + (h / 2) * b for a, b in zip(y, k1)], t=t + h / 2)
+        k3 = r.nonlinear_flux(*[a + (h / 2) * b for a, b in zip(y, k2)], t=t + h / 2)
+        k4 = r.nonlinear_flux(*[a + h * b for a, b in zip(y, k3)], t=t + h)
+        y = [a + (h / 6) * (((b1 + 2 * b2) + 2 * b3) + b4) for a, b1, b2, b3, b4 in zip(y, k1, k2, k3, k4)]
+        sec = time.perf_counter() - t0
+        return {"value": round(1.0 / sec, 5), "unit": "steps/s", "cores": threads, "kind": "reference",
+                "sample": f"1 FP64 RK4 step (4 nonlinearFlux calls) of the reference kernel at 256x256x129 "
+                          f"from the benchmark state, {sec:.1f} s; engine: build-authored radix-2 FFTs on "
+                          f"{threads} OpenMP threads (FFTW, the reference's production engine, is not installed)"}
     from oracle import Oracle
     Oracle.set_threads(threads)
     o = Oracle(cfg)

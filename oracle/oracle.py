@@ -18,6 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libwvref.so")
+REF_FAST_SO = os.path.join(HERE, "_ref", "libwvref_fast.so")
 
 
 class Config(C.Structure):
@@ -110,6 +111,10 @@ class _Lib:
             if hasattr(lib, p + "advect_f_grid_scalar"):
                 getattr(lib, p + "advect_f_grid_scalar").restype = C.c_int
                 getattr(lib, p + "advect_f_grid_scalar").argtypes = [C.c_void_p, _P, _P, C.c_int, _P]
+            if hasattr(lib, p + "engine_compare"):
+                lib.wvr_engine_compare.restype = C.c_double
+                lib.wvr_engine_compare.argtypes = [C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint]
+                lib.wvr_set_threads.argtypes = [C.c_int]
             if p == "wvo_":
                 lib.wvo_rk4_advance.restype = C.c_int
                 lib.wvo_rk4_advance.argtypes = [C.c_void_p, _P, _P, _P, _P, C.c_double, C.c_double, C.c_int]
@@ -293,3 +298,25 @@ class Reference(_Lib):
     @staticmethod
     def available() -> bool:
         return os.path.exists(REF_SO)
+
+
+class ReferenceFast(Reference):
+    """The reference kernel over the build-authored fast engine
+    (fast_fft_engine.cpp): the reference's CPU path at production sizes."""
+    path = REF_FAST_SO
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_FAST_SO)
+
+    @classmethod
+    def set_threads(cls, n: int):
+        """Threads of the fast engine for kernels created afterwards (0: OpenMP default)."""
+        cls._load().wvr_set_threads(int(n))
+
+    @classmethod
+    def engine_compare(cls, kind: str, n0: int, n1: int = 1, batch: int = 3, seed: int = 1) -> float:
+        """max|fast - direct| / max|direct| of one plan of the given kind
+        ("r2c", "c2r", "dct1", "dst1") against WVReferenceFFTEngine."""
+        k = {"r2c": 0, "c2r": 1, "dct1": 2, "dst1": 3}[kind]
+        return float(cls._load().wvr_engine_compare(k, n0, n1, batch, seed))

@@ -9,8 +9,20 @@
 // (O(N^4) transforms) and therefore only meant for tiny grids.
 //
 // The API mirrors oracle/wv_oracle.h (prefix wvr_ instead of wvo_).
+//
+// Built a second time with WVR_FAST_ENGINE defined (oracle/_ref/libwvref_fast.so):
+// the same reference kernel over the build-authored fast engine of
+// fast_fft_engine.cpp, to time the reference's CPU path at production sizes;
+// wvr_engine_compare then checks that engine against WVReferenceFFTEngine.
 #include "WaveVortexKernel/WVTransformConstantStratificationKernel.hpp"
 #include "WVReferenceFFTEngine.hpp"
+#ifdef WVR_FAST_ENGINE
+#include "fast_fft_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <random>
+#endif
 
 #include <cstdio>
 #include <cstring>
@@ -43,7 +55,76 @@ int report(const WVKernelStatus& st, char* msg, size_t len) {
 
 }  // namespace
 
+#ifdef WVR_FAST_ENGINE
+int gThreads = 0;  // threads of the fast engine (0: OpenMP's default)
+#endif
+
 extern "C" {
+
+#ifdef WVR_FAST_ENGINE
+void wvr_set_threads(int n) { gThreads = n; }
+
+// The fast engine against WVReferenceFFTEngine on one plan of the given kind
+// (0 r2c 2-D [n0 = Ny, n1 = Nx], 1 c2r 2-D, 2 DCT-I and 3 DST-I of length n0),
+// `batch` contiguous batches, seeded random input (c2r: a Hermitian-consistent
+// half spectrum, the r2c of random data).  Returns max|fast - ref| / max|ref|.
+double wvr_engine_compare(int kind, size_t n0, size_t n1, size_t batch, unsigned seed) {
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd;
+    WVFFTPlanSpecification sp;
+    size_t inCount = 0, outCount = 0;  // doubles
+    if (kind <= 1) {
+        const size_t Ny = n0, Nx = n1, H = Nx / 2 + 1;
+        const std::ptrdiff_t real = static_cast<std::ptrdiff_t>(Nx * Ny), half = static_cast<std::ptrdiff_t>(H * Ny);
+        sp.kind = kind == 0 ? WVFFTPlanKind::horizontalRealToComplex2D : WVFFTPlanKind::horizontalComplexToReal2D;
+        if (kind == 0) {
+            sp.transformDimensions = {{Ny, static_cast<std::ptrdiff_t>(Nx), static_cast<std::ptrdiff_t>(H)}, {Nx, 1, 1}};
+            sp.batchDimensions = {{batch, real, half}};
+            inCount = batch * real;
+            outCount = 2 * batch * half;
+        } else {
+            sp.transformDimensions = {{Ny, static_cast<std::ptrdiff_t>(H), static_cast<std::ptrdiff_t>(Nx)}, {Nx, 1, 1}};
+            sp.batchDimensions = {{batch, half, real}};
+            inCount = 2 * batch * half;
+            outCount = batch * real;
+        }
+    } else {
+        const size_t N = n0;
+        sp.kind = kind == 2 ? WVFFTPlanKind::verticalDCTI : WVFFTPlanKind::verticalDSTI;
+        sp.transformDimensions = {{N, 2, 2}};
+        sp.batchDimensions = {{2, 1, 1}, {batch, static_cast<std::ptrdiff_t>(2 * N), static_cast<std::ptrdiff_t>(2 * N)}};
+        inCount = outCount = 2 * N * batch;
+    }
+    sp.inputBytes = inCount * sizeof(double);
+    sp.outputBytes = outCount * sizeof(double);
+    std::vector<double> in(inCount);
+    for (auto& v : in) v = nd(rng);
+    if (kind == 1) {  // a consistent half spectrum: the r2c of random real data
+        WVFFTPlanSpecification f = sp;
+        f.kind = WVFFTPlanKind::horizontalRealToComplex2D;
+        for (auto& d : f.transformDimensions) std::swap(d.inputStride, d.outputStride);
+        for (auto& d : f.batchDimensions) std::swap(d.inputStride, d.outputStride);
+        std::vector<double> real(outCount);
+        for (auto& v : real) v = nd(rng);
+        std::unique_ptr<WVFFTPlan> p;
+        wvfast::FastFFTEngine(gThreads).createPlan(f, p);
+        p->execute(real.data(), in.data());
+    }
+    std::vector<double> a(outCount), b(outCount);
+    std::unique_ptr<WVFFTPlan> pf, pr;
+    wvfast::FastFFTEngine fast(gThreads);
+    WVReferenceFFTEngine ref;
+    if (!fast.createPlan(sp, pf) || !ref.createPlan(sp, pr)) return -1.0;
+    std::vector<double> in2 = in;
+    if (!pf->execute(in.data(), a.data()) || !pr->execute(in2.data(), b.data())) return -1.0;
+    double num = 0.0, den = 0.0;
+    for (size_t i = 0; i < outCount; ++i) {
+        num = std::max(num, std::fabs(a[i] - b[i]));
+        den = std::max(den, std::fabs(b[i]));
+    }
+    return den > 0.0 ? num / den : num;
+}
+#endif
 
 void* wvr_create(const RefConfig* c, char* msg, size_t msglen) {
     WVTransformConstantStratificationConfiguration cfg;
@@ -53,7 +134,12 @@ void* wvr_create(const RefConfig* c, char* msg, size_t msglen) {
     cfg.isHydrostatic = c->isHydrostatic != 0;
     cfg.shouldAntialias = c->shouldAntialias != 0;
     auto h = std::make_unique<Handle>();
-    const auto st = WVTransformConstantStratificationKernel::create(cfg, std::make_unique<WVReferenceFFTEngine>(), h->kernel);
+#ifdef WVR_FAST_ENGINE
+    auto engine = std::make_unique<wvfast::FastFFTEngine>(gThreads);
+#else
+    auto engine = std::make_unique<WVReferenceFFTEngine>();
+#endif
+    const auto st = WVTransformConstantStratificationKernel::create(cfg, std::move(engine), h->kernel);
     if (!st) { report(st, msg, msglen); return nullptr; }
     return h.release();
 }
