@@ -35,6 +35,8 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_rk4_256x256x129_mi
 # per-dispatch SQ counters (MFMA instructions among them) of the same kernels
 COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "sq_counters_rk4_256x256x129_mixed.md")
 MFMA_PEAK_TFLOPS_F32 = 157.3
+VALU_SIMDS = 256 * 4          # CUs x SIMDs
+PEAK_CLOCK_HZ = 2.4e9         # peak engine clock
 MFMA_FLOPS_F32_16X16X4 = 2 * 16 * 16 * 4  # the only MFMA shape of the mixed RHS
 # stage name -> kernel of the mixed-precision RHS at 256x256x129
 STAGE_KERNEL = {"fields": "k_fields<float", "synth": "k_synth<float", "yfft_inv": "k_yfft<float, 8, 4, true",
@@ -333,6 +335,15 @@ def main():
                 "traffic": round(measured["traffic_bytes"]) if measured else None,
                 "kernel": dom["name"], "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": round(dom["bytes_per_launch"])}
+    # What limits that kernel: its VALU issue rate, against one wave64 VALU
+    # instruction per 4 cycles per SIMD (1024 SIMDs) at the 2.4 GHz peak
+    # engine clock, from the per-dispatch instruction count of the counter
+    # summary; the HBM fraction above is the roofline the contract names.
+    from wvamd.rocprof_summary import kernel_counters
+    c = kernel_counters(COUNTER_SUMMARY, STAGE_KERNEL.get(dom["name"], "?"))
+    if c and c.get("SQ_INSTS_VALU"):
+        roofline["limiter"] = "valu"
+        roofline["valu_issue_frac"] = round(c["SQ_INSTS_VALU"] * 4 / (VALU_SIMDS * avg_ms * 1e-3 * PEAK_CLOCK_HZ), 4)
     b32 = step_model_bytes(cfg["Nx"], cfg["Ny"], cfg["Nz"], k32.Nkl * k32.Nj, 8, 4)
     b64 = step_model_bytes(cfg["Nx"], cfg["Ny"], cfg["Nz"], k64.Nkl * k64.Nj, 16, 8)
     # The reference's unfused dataflow (per-pass arrays read and written) at

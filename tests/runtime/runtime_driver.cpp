@@ -19,6 +19,8 @@
 #include "WaveVortexRuntime/WVConstantStratificationIntegrationSystem.hpp"
 #include "WaveVortexRuntime/WVExtensionCatalog.hpp"
 #include "WaveVortexRuntime/WVForcingEngine.hpp"
+#include "WaveVortexRuntime/WVIntegrationState.hpp"
+#include "WaveVortexRuntime/WVObserverContracts.hpp"
 #include "WaveVortexRuntime/WVRungeKutta.hpp"
 #include "WVReferenceFFTEngine.hpp"
 
@@ -223,6 +225,106 @@ void forced(Output& out, bool hydrostatic) {
     out.put(p + "state", s2.values);
 }
 
+// Observers coupled to the right-hand side
+// (WVConstantStratificationIntegrationSystem.cpp:294-406): Lagrangian floats
+// sampled with spline interpolation in 3-D, surface drifters (x, y only) with
+// linear interpolation, and an antialiased F-grid dye; every RHS shares the
+// kernel's advection fields (nonlinearFluxWithAdvectionFields) among them.
+// Fixed-step RK4 over 0 -> 1800 s.
+void observers(Output& out) {
+    const auto c = configuration(16, 16, 9, 5, false);
+    WVTransformConstantStratificationDescriptor td;
+    require(WVTransformConstantStratificationDescriptor::create(c, td), "transform descriptor");
+    WVPortableObserverRecord rec;
+    auto block = [&](const char* id, WVStateScalarType type, std::vector<std::size_t> dims, WVToleranceKind kind,
+                     double tol) {
+        rec.stateBlocks.push_back({id, type, std::move(dims), kind, tol, WVStateOwnership::integratorOwned,
+                                   WVRestartRequirement::requiredDynamicState});
+    };
+    for (const char* id : {"Ap", "Am", "A0"})
+        block(id, WVStateScalarType::complex64, {c.Nj, td.Nkl()}, WVToleranceKind::coefficientEnergyScaled, 1e-6);
+    for (const char* id : {"float-x", "float-y"}) block(id, WVStateScalarType::real64, {4}, WVToleranceKind::uniformAbsolute, 1e-5);
+    block("float-z", WVStateScalarType::real64, {4}, WVToleranceKind::uniformAbsolute, 1e-6);
+    for (const char* id : {"drifter-x", "drifter-y"}) block(id, WVStateScalarType::real64, {3}, WVToleranceKind::uniformAbsolute, 1e-5);
+    block("dye", WVStateScalarType::real64, {c.Nx, c.Ny, c.Nz}, WVToleranceKind::uniformAbsolute, 1e-8);
+
+    WVObserverRecord coefficients;
+    coefficients.identifier = "coefficients";
+    coefficients.name = "coefficients";
+    coefficients.typeIdentifier = "WVCoefficients";
+    coefficients.stateBlockIdentifiers = {"Ap", "Am", "A0"};
+    rec.observers.push_back(coefficients);
+    WVObserverRecord floats;
+    floats.identifier = "floats";
+    floats.name = "floats";
+    floats.typeIdentifier = "WVLagrangianParticles";
+    floats.stateBlockIdentifiers = {"float-x", "float-y", "float-z"};
+    floats.x = {1200.0, 17500.0, 33100.0, 48800.0};
+    floats.y = {2500.0, 39000.0, 11000.0, 26700.0};
+    floats.z = {-80.0, -420.0, -777.0, -1210.0};
+    floats.advectionInterpolation = WVPositionInterpolation::spline;
+    floats.horizontalAbsoluteTolerance = 1e-5;
+    floats.verticalAbsoluteTolerance = 1e-6;
+    rec.observers.push_back(floats);
+    WVObserverRecord drifters;
+    drifters.identifier = "drifters";
+    drifters.name = "drifters";
+    drifters.typeIdentifier = "WVLagrangianParticles";
+    drifters.stateBlockIdentifiers = {"drifter-x", "drifter-y"};
+    drifters.x = {-150.0, 20000.0, c.Lx + 40.0};
+    drifters.y = {700.0, -35.0, 31000.0};
+    drifters.z = {0.0, -10.0, -650.0};
+    drifters.isXYOnly = true;
+    drifters.advectionInterpolation = WVPositionInterpolation::linear;
+    drifters.horizontalAbsoluteTolerance = 1e-5;
+    rec.observers.push_back(drifters);
+    WVObserverRecord dye;
+    dye.identifier = "dye";
+    dye.name = "dye";
+    dye.typeIdentifier = "WVTracer";
+    dye.stateBlockIdentifiers = {"dye"};
+    dye.shouldAntialias = true;
+    rec.observers.push_back(dye);
+    const auto cat = catalog();  // the descriptor and the system share one catalog
+    WVPortableObserverDescriptor desc;
+    require(WVPortableObserverDescriptor::create(rec, cat, desc), "observer descriptor");
+
+    std::unique_ptr<WVConstantStratificationIntegrationSystem> sys;
+    require(WVConstantStratificationIntegrationSystem::create(c, nonlinearSchedule(), desc, cat,
+                                                              std::make_unique<WVReferenceFFTEngine>(), sys),
+            "integration system with observers");
+    State st(sys->kernel().descriptor(), 0.01);
+    WVAdditionalStateStorage storage;
+    require(storage.initialize(sys->stateLayout()), "additional state storage");
+    WVMutableIntegrationState state = st.integration();
+    state.additionalBlocks = storage.mutableBlocks();
+    state.additionalBlockCount = storage.blockCount();
+    require(sys->initializeParticleState(state), "particle initial positions");
+    auto find = [&](const std::string& id) -> WVAdditionalStateBlockView& {
+        for (std::size_t b = 0; b < state.additionalBlockCount; ++b)
+            if (state.additionalBlocks[b].layout->identifier == id) return state.additionalBlocks[b];
+        throw std::runtime_error("missing state block " + id);
+    };
+    // dye: a smooth, band-limited blob pattern
+    double* d = find("dye").realData;
+    const double pi = std::acos(-1.0);
+    for (std::size_t z = 0; z < c.Nz; ++z)
+        for (std::size_t y = 0; y < c.Ny; ++y)
+            for (std::size_t x = 0; x < c.Nx; ++x)
+                d[x + c.Nx * (y + c.Ny * z)] = std::cos(2.0 * pi * x / c.Nx) * std::sin(2.0 * pi * y / c.Ny) +
+                                               0.3 * std::cos(pi * z / (c.Nz - 1.0));
+    WVFixedStepRK4 rk4(*sys);
+    require(rk4.prepareStateAfterRestart(state), "observer rk4 restart preparation");
+    require(rk4.advanceToTime(state, 1800.0, 600.0), "observer rk4");
+    out.put("obs_state", st.values);
+    for (const char* id : {"float-x", "float-y", "float-z", "drifter-x", "drifter-y"}) {
+        const auto& b = find(id);
+        out.put(std::string("obs_") + id, b.realData, b.layout->elementCount);
+    }
+    out.put("obs_dye", d, c.Nx * c.Ny * c.Nz);
+    out.put("obs_rhs_count", static_cast<double>(sys->metrics().rightHandSideEvaluationCount));
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -246,6 +348,7 @@ int main(int argc, char** argv) {
         run("rk23", [&] { adaptive(out); });
         run("forced-nh", [&] { forced(out, false); });
         run("forced-h", [&] { forced(out, true); });
+        run("observers", [&] { observers(out); });
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "runtime driver failed: %s\n", e.what());

@@ -6,8 +6,10 @@ tests/golden/runtime_driver.npz) and over the drop-in adapter
 (wave-vortex-model_amd/adapter/_build/runtime_driver_hip: the contract's
 class name and header over libwvamd).  The runtime sources -- the forcing
 engine, the constant-stratification integration system, WVFixedStepRK4
-(advanceToTime with a clipped last step) and WVAdaptiveRK23 -- are compiled
-unchanged in both.  Results must agree: states and right-hand sides at 1e-12
+(advanceToTime with a clipped last step), WVAdaptiveRK23 and the observers
+coupled to the right-hand side (Lagrangian floats with spline and drifters
+with linear velocity sampling, an antialiased F-grid dye, all fed by the
+kernel's advection fields) -- are compiled unchanged in both.  Results must agree: states and right-hand sides at 1e-12
 relative (max|diff| / max|ref|), times and step counts exactly.
 """
 import os
@@ -26,7 +28,7 @@ pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(ROOT, "wave-vortex-model_amd", "adapter", "_build", "runtime_driver_hip")
 # scalars compared exactly: end times, step counts and sizes, RHS evaluations
 EXACT = {"rk4_nh_t", "rk4_nh_steps", "rk4_nh_last_step", "rk4_h_t", "rk4_h_steps", "rk4_h_last_step", "rk23_t",
-         "rk23_accepted", "rk23_rejected", "rk23_rhs"}
+         "rk23_accepted", "rk23_rejected", "rk23_rhs", "obs_rhs_count"}
 
 
 def test_runtime_over_hip_kernel_matches_reference_kernel(torch_gpu, tmp_path):
