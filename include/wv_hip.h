@@ -59,10 +59,17 @@ typedef enum wv_status {
     WV_REENTRANT_EXECUTION
 } wv_status;
 
-/* Arithmetic used inside the flux.  FP64 everywhere, or FP32 spatial fields,
- * spectra, FFTs and vertical transforms with FP64 coefficients, RK4 state and
- * reductions. */
-typedef enum wv_precision { WV_PRECISION_FP64 = 0, WV_PRECISION_MIXED = 1 } wv_precision;
+/* Arithmetic used inside the flux.  FP64 everywhere (WV_PRECISION_FP64), or
+ * FP32 spatial fields, spectra and FFTs with FP64 coefficients, RK4 state and
+ * reductions, the vertical transforms on FP32 MFMA (WV_PRECISION_MIXED) or,
+ * for the constant-N synthesis, on split-FP16 MFMA products: operands scaled
+ * by powers of two and held as FP16 hi/lo halves, three products summed in
+ * FP32 (WV_PRECISION_MIXED_SPLIT16). */
+typedef enum wv_precision {
+    WV_PRECISION_FP64 = 0,
+    WV_PRECISION_MIXED = 1,
+    WV_PRECISION_MIXED_SPLIT16 = 2
+} wv_precision;
 
 /* WVTransformConstantStratificationConfiguration, WVKernelTypes.hpp:129-146. */
 typedef struct wv_config {

@@ -36,7 +36,7 @@ def test_apv_matches_oracle(torch_gpu, case):
     for seed in (3, 4):
         A = _state(o, seed)
         ref = o.apv(*A, t=77.0, t0=5.0)
-        for prec, tol in (("fp64", 1e-12), ("mixed", 1e-4)):
+        for prec, tol in (("fp64", 1e-12), ("mixed", 1e-4), ("mixed-split16", 1e-4)):
             print(f"[gpu] apv {case} seed {seed} {prec}", flush=True)
             k = wvamd.WVTransformConstantStratificationKernel(cfg, prec)
             dev = [k.to_device(a) for a in A]

@@ -45,7 +45,7 @@ def _rand_state(k, seed, scale=1e-3):
 def test_state_derivatives_and_flux_using_fields_match_reference(torch_gpu, path):
     g = load_golden(path)
     t, t0 = float(g["t"]), float(g["t0"])
-    for prec, tol in (("fp64", TOL), ("mixed", 1e-4)):
+    for prec, tol in (("fp64", TOL), ("mixed", 1e-4), ("mixed-split16", 1e-4)):
         k = _kernel(config_from_golden(g), prec)
         A = [k.to_device(a) for a in (g["Ap"], g["Am"], g["A0"])]
         for f, name in enumerate(("u", "v", "w", "eta")):
@@ -87,7 +87,7 @@ def test_midpoint_row_derivatives_at_nz_33(torch_gpu):
     import wvamd
     cfg = wvamd.make_config(32, 32, 33, 21)
     o = Oracle(cfg)
-    for prec, tol in (("fp64", TOL), ("mixed", 1e-4)):
+    for prec, tol in (("fp64", TOL), ("mixed", 1e-4), ("mixed-split16", 1e-4)):
         k = _kernel(cfg, prec)
         for seed in (11, 12):
             A = _rand_state(k, seed)
@@ -262,7 +262,7 @@ def test_flux_triad_energy_flux_matches_closed_form(torch_gpu, hydrostatic):
     import triad_kat
     import wvamd
     cfg = triad_kat.config(wvamd.make_config, hydrostatic)
-    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-4)):
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-4), ("mixed-split16", 1e-4)):
         k = _kernel(cfg, prec)
         Ap, idx, E = triad_kat.setup(cfg, k.modes(), k.coriolis_frequency())
         Z = np.zeros_like(Ap)

@@ -78,13 +78,14 @@ def test_flux_fp64_matches_reference_golden(torch_gpu, path):
     assert err <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed-split16"])
 @pytest.mark.parametrize("path", GOLDEN_TINY, ids=IDS)
-def test_flux_mixed_matches_reference_golden(torch_gpu, path):
+def test_flux_mixed_matches_reference_golden(torch_gpu, path, precision):
     g = load_golden(path)
-    k = _kernel(config_from_golden(g), "mixed")
+    k = _kernel(config_from_golden(g), precision)
     F = _host(k.nonlinear_flux(*_dev(k, (g["Ap"], g["Am"], g["A0"])), t=float(g["t"]), t0=float(g["t0"])))
     err = rel_err_joint(F, (g["Fp"], g["Fm"], g["F0"]))
-    print(f"[gpu] mixed flux rel err {err:.2e}", flush=True)
+    print(f"[gpu] {precision} flux rel err {err:.2e}", flush=True)
     assert err <= MIXED_TOL
 
 
@@ -101,7 +102,7 @@ def test_inverse_transforms_match_reference_golden(torch_gpu, path):
 
 def test_zero_state_gives_exactly_zero_flux(torch_gpu):
     import wvamd
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "mixed", "mixed-split16"):
         k = _kernel(wvamd.make_config(16, 16, 9, 5), prec)
         Z = k.empty_coefficients()
         F = _host(k.nonlinear_flux(*Z, t=0.5))
@@ -145,9 +146,10 @@ def test_flux_fp64_64x64x33_flow_matches_reference(torch_gpu):
     k = _kernel(config_from_golden(g))
     F = _host(k.nonlinear_flux(*_dev(k, (g["Ap"], g["Am"], g["A0"])), t=float(g["t"]), t0=float(g["t0"])))
     assert rel_err_joint(F, (g["Fp"], g["Fm"], g["F0"])) <= FP64_TOL
-    km = _kernel(config_from_golden(g), "mixed")
-    Fm_ = _host(km.nonlinear_flux(*_dev(km, (g["Ap"], g["Am"], g["A0"])), t=float(g["t"]), t0=float(g["t0"])))
-    assert rel_err_joint(Fm_, (g["Fp"], g["Fm"], g["F0"])) <= MIXED_TOL
+    for prec in ("mixed", "mixed-split16"):
+        km = _kernel(config_from_golden(g), prec)
+        Fm_ = _host(km.nonlinear_flux(*_dev(km, (g["Ap"], g["Am"], g["A0"])), t=float(g["t"]), t0=float(g["t0"])))
+        assert rel_err_joint(Fm_, (g["Fp"], g["Fm"], g["F0"])) <= MIXED_TOL, prec
 
 
 @pytest.mark.parametrize("path", [p for p in GOLDEN_TINY if "fwd_fields" in np.load(p).files],

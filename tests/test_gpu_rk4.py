@@ -166,7 +166,7 @@ def test_rk4_fp64_256x256x129_one_step_matches_oracle(torch_gpu):
     assert err <= 1e-12
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-split16"])
 def test_flux_conserves_energy_256x256x129(torch_gpu, precision):
     """sum TE*Re(conj(A) F) = 0 for the antialiased flux (UnitTests/TestNonlinearFlux.m:7-18);
     FP64 to 100 eps of the term magnitudes, mixed to its FP32 level."""
@@ -183,9 +183,10 @@ def test_flux_conserves_energy_256x256x129(torch_gpu, precision):
     assert ratio <= (100 * np.finfo(float).eps if precision == "fp64" else 1e-5)
 
 
-def test_mixed_energy_tracks_fp64_256x256x129(torch_gpu):
+@pytest.mark.parametrize("precision", ["mixed", "mixed-split16"])
+def test_mixed_energy_tracks_fp64_256x256x129(torch_gpu, precision):
     cfg = _cfg(256, 129, 200e3)
-    k64, k32 = _kernel(cfg, "fp64"), _kernel(cfg, "mixed")
+    k64, k32 = _kernel(cfg, "fp64"), _kernel(cfg, precision)
     A = _flow(k64, seed=4004, jmax=64)
     h = _dt(k64, A)
     d64 = [k64.to_device(a) for a in A]

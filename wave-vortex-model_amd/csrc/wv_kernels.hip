@@ -215,6 +215,68 @@ __device__ __forceinline__ double pairSwap(double v) {
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
 
+// Stores of one folded synthesis task (rows z = 16 zt + row(kq, i) of the
+// tile and their mirrors L - z; z <= L/2 only) for the NQ channel slots of
+// group g from slot q0: the value (E + O at z, E - O or O - E at L - z, times
+// scale[q]) and i k / i l times it, whose other part the partner lane n ^ 1
+// holds.
+template <typename T, int NQ>
+__device__ __forceinline__ void storeFoldedTask(const SynthGroup& g, int q0, int zt, int kq, int L, int zStride,
+                                                size_t chStride, int colOff, bool store, const T (&scale)[NQ], T ks,
+                                                T ls, int part, cx<T>* __restrict__ H,
+                                                const typename Mfma<T>::acc (&accE)[NQ],
+                                                const typename Mfma<T>::acc (&accO)[NQ]) {
+    using MF = Mfma<T>;
+    const int half = L / 2, mat = g.mat, nch = g.nch;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        if (q0 + q >= nch) continue;
+        const SynthChannel& sc = g.ch[q0 + q];
+        const int oV = sc.outV, oX = sc.outX, oY = sc.outY;
+        // rows z = zr0 + i dzr and L - z of this lane, as byte offsets
+        // within a channel (uniform channel bases, 32-bit lane offsets)
+        const int zr0 = 16 * zt + MF::row(kq, 0);
+        constexpr int dzr = sizeof(T) == 4 ? 1 : 4;  // MF::row(kq, i + 1) - MF::row(kq, i)
+        const unsigned offZ = (static_cast<unsigned>(zr0) * zStride + colOff) * sizeof(T);
+        const unsigned offL = (static_cast<unsigned>(L - zr0) * zStride + colOff) * sizeof(T);
+        const unsigned stepB = static_cast<unsigned>(dzr * zStride) * sizeof(T);
+        char* const hb = reinterpret_cast<char*>(H);
+        const size_t chBytes = chStride * sizeof(T);
+        char* const hV = hb + (oV >= 0 ? oV : 0) * chBytes;
+        const bool xy = oX >= 0 || oY >= 0;  // the i k and i l outputs (uniform)
+        const T sq = scale[q];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int z = zr0 + i * dzr;
+            const T e = accE[q][i], o = accO[q][i];
+            const T v1 = (e + o) * sq, v2 = (mat == 0 ? e - o : o - e) * sq;  // at z and at L - z
+            const unsigned o1 = offZ + i * stepB, o2 = offL - i * stepB;
+            const bool lo = z <= half && store, hi = lo && L - z != z;
+            if (oV >= 0) {
+                if (lo) *reinterpret_cast<T*>(hV + o1) = v1;
+                if (hi) *reinterpret_cast<T*>(hV + o2) = v2;
+            }
+            if (xy) {
+                // partner values with every lane active (a DPP read from an
+                // inactive lane returns nothing)
+                const T w1 = pairSwap(v1), w2 = pairSwap(v2);
+                const T r1 = part ? w1 : -w1;  // part of i * value
+                const T r2 = part ? w2 : -w2;
+                char* const hX = hb + (oX >= 0 ? oX : 0) * chBytes;
+                char* const hY = hb + (oY >= 0 ? oY : 0) * chBytes;
+                if (oX >= 0) {
+                    if (lo) *reinterpret_cast<T*>(hX + o1) = ks * r1;
+                    if (hi) *reinterpret_cast<T*>(hX + o2) = ks * r2;
+                }
+                if (oY >= 0) {
+                    if (lo) *reinterpret_cast<T*>(hY + o1) = ls * r1;
+                    if (hi) *reinterpret_cast<T*>(hY + o2) = ls * r2;
+                }
+            }
+        }
+    }
+}
+
 // Spectral assembly + vertical synthesis (MFMA).  One workgroup owns 8
 // consecutive columns: it stages their evolved field spectra (all j < Nj) and
 // m_j in LDS once, then its waves loop over (group, z-chunk) tasks, each
@@ -333,7 +395,7 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
     // lane reads the real (part 0) or imaginary (part 1) word of fv[.][c][j]
     const T* fvw = reinterpret_cast<const T*>(fv) + 2 * c * Jp + part;
     if constexpr (FOLD) {
-        const int L = Nz - 1, half = L / 2;
+        const int L = Nz - 1;
         // channel slots of a group in parts of NQ
         int maxch = 0;
         for (int g = 0; g < plan.ngroups; ++g) maxch = max(maxch, plan.g[g].nch);
@@ -367,7 +429,7 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
             const int gi = task % plan.ngroups;
             const int pt = (task / plan.ngroups) % parts;
             const int zt = task / (plan.ngroups * parts);
-            const int mat = plan.g[gi].mat, nch = plan.g[gi].nch;
+            const int nch = plan.g[gi].nch;
             const int q0 = pt * NQ;  // first channel of this part
             // B is the staged spectrum itself; the channel's factor (the value
             // scale, or 1 for z derivatives, whose -/+ m_j / 2 is in the
@@ -484,52 +546,11 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
                     }
                 }
             }
+            T scale[NQ];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                if (q0 + q >= nch) continue;
-                const SynthChannel& sc = plan.g[gi].ch[q0 + q];
-                const int oV = sc.outV, oX = sc.outX, oY = sc.outY;
-                // rows z = zr0 + i dzr and L - z of this lane, as byte offsets
-                // within a channel (uniform channel bases, 32-bit lane offsets)
-                const int zr0 = 16 * zt + MF::row(kq, 0);
-                constexpr int dzr = sizeof(T) == 4 ? 1 : 4;  // MF::row(kq, i + 1) - MF::row(kq, i)
-                const unsigned offZ = (static_cast<unsigned>(zr0) * zStride + colOff) * sizeof(T);
-                const unsigned offL = (static_cast<unsigned>(L - zr0) * zStride + colOff) * sizeof(T);
-                const unsigned stepB = static_cast<unsigned>(dzr * zStride) * sizeof(T);
-                char* const hb = reinterpret_cast<char*>(H);
-                const size_t chBytes = chStride * sizeof(T);
-                char* const hV = hb + (oV >= 0 ? oV : 0) * chBytes;
-                const bool xy = oX >= 0 || oY >= 0;  // the i k and i l outputs (uniform)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int z = zr0 + i * dzr;
-                    const T e = accE[q][i], o = accO[q][i];
-                    const T v1 = (e + o) * oscale, v2 = (mat == 0 ? e - o : o - e) * oscale;  // at z and at L - z
-                    const unsigned o1 = offZ + i * stepB, o2 = offL - i * stepB;
-                    const bool lo = z <= half && store, hi = lo && L - z != z;
-                    if (oV >= 0) {
-                        if (lo) *reinterpret_cast<T*>(hV + o1) = v1;
-                        if (hi) *reinterpret_cast<T*>(hV + o2) = v2;
-                    }
-                    if (xy) {
-                        // partner values with every lane active (a DPP read
-                        // from an inactive lane returns nothing)
-                        const T w1 = pairSwap(v1), w2 = pairSwap(v2);
-                        const T r1 = part ? w1 : -w1;  // part of i * value
-                        const T r2 = part ? w2 : -w2;
-                        char* const hX = hb + (oX >= 0 ? oX : 0) * chBytes;
-                        char* const hY = hb + (oY >= 0 ? oY : 0) * chBytes;
-                        if (oX >= 0) {
-                            if (lo) *reinterpret_cast<T*>(hX + o1) = ks * r1;
-                            if (hi) *reinterpret_cast<T*>(hX + o2) = ks * r2;
-                        }
-                        if (oY >= 0) {
-                            if (lo) *reinterpret_cast<T*>(hY + o1) = ls * r1;
-                            if (hi) *reinterpret_cast<T*>(hY + o2) = ls * r2;
-                        }
-                    }
-                }
-            }
+            for (int q = 0; q < NQ; ++q) scale[q] = oscale;
+            storeFoldedTask<T, NQ>(plan.g[gi], q0, zt, kq, L, zStride, chStride, colOff, store, scale, ks, ls, part, H,
+                                   accE, accO);
         }
         return;
     }
@@ -633,6 +654,207 @@ __global__ __launch_bounds__(512) void k_synth(const cx<T>* __restrict__ FV, con
                 }
             }
         }
+    }
+}
+
+// The folded synthesis with split-FP16 MFMA products (the mixed-precision
+// form with compensated half-precision vertical GEMMs): every operand x is
+// scaled by a power of two s and held as hi = fp16(x s) and lo = fp16(x s -
+// hi), so x s = hi + lo to 22 bits, and each product is summed in FP32 as
+// lo_a hi_b + hi_a lo_b + hi_a hi_b (the lo_a lo_b term, 2^-22 of the
+// product, is dropped).  The operators (SynthMats16, prepared on the host)
+// have one scale per matrix pair; the staged spectra have one per (field,
+// column), chosen from the column's largest entry so that it lies in
+// [2^14, 2^15): every hi part is a normal FP16 number and the columns of
+// small modes keep their relative precision.  The outputs are scaled back
+// exactly.  Same tasks, tiles and outputs as the folded k_synth, with the
+// midpoint row z = L/2 in the last MFMA tile (no separate dot product).
+// K is the folded contraction length (Jh = Js / 2 even- or odd-j entries)
+// in N32 steps of 32 (v_mfma_f32_16x16x32_f16: lane l holds k = 8 (l/16) +
+// t, t < 8) and TAIL steps of 16 (v_mfma_f32_16x16x16_f16: k = 4 (l/16) + t,
+// t < 4).
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+struct SynthMats16 {
+    const _Float16* m[kSynthMatSlots];  // per matrix: [tile][N32 steps x (hi, lo) x 64 x 8 | TAIL x (hi, lo) x 64 x 4]
+    float inv[kSynthMatSlots];          // 1 / (the matrix's scale)
+};
+// Power-of-two scale that puts a nonzero magnitude m in [2^14, 2^15), as an
+// exponent clamped so that the scale stays a normal float.
+__host__ __device__ inline int split16Exponent(float m) {
+    if (!(m > 0.0f)) return 0;
+    int ex;
+    frexpf(m, &ex);  // m = f 2^ex, f in [0.5, 1)
+    const int e = 15 - ex;
+    return e > 100 ? 100 : e < -100 ? -100 : e;
+}
+__device__ __forceinline__ void splitHalf(float v, _Float16& hi, _Float16& lo) {
+    hi = static_cast<_Float16>(v);
+    lo = static_cast<_Float16>(v - static_cast<float>(hi));
+}
+
+template <int NQ, int N32, int TAIL>
+__global__ __launch_bounds__(256) void k_synth16(const cx<float>* __restrict__ FV, const ColumnDev* __restrict__ cols,
+                                                 int ncol, int M, SynthMats16 mats, int Js, int Nz, int Nj, int Ny,
+                                                 int nkxP, int ztiles, SynthPlan plan, cx<float>* __restrict__ H) {
+    using MF = Mfma<float>;
+    constexpr int NC8 = 4 * N32 + 2 * TAIL;          // 8-entry chunks of the folded contraction
+    constexpr int kTileHalves = N32 * 2 * 64 * 8 + TAIL * 2 * 64 * 4;
+    // staged spectra: half8 [field 4][parity 2][hl 2][chunk NC8][column c 8 x part 2]
+    extern __shared__ __align__(16) unsigned char smem[];
+    half8* sb = reinterpret_cast<half8*>(smem);
+    __shared__ ColumnDev scol[8];
+    __shared__ unsigned rowMax[32];  // per (field, column): largest |entry| as float bits
+    __shared__ float rowInv[32];     // 1 / (its scale)
+    const int lane = threadIdx.x & 63;
+    const int waveInBlock = waveIndex();
+    const int nwaves = blockDim.x >> 6;
+    const int cg = xcdBlock(blockIdx.x, gridDim.x);
+    if (threadIdx.x < 8) {
+        const int col = cg * 8 + threadIdx.x;
+        if (col < ncol) scol[threadIdx.x] = cols[col];
+        else scol[threadIdx.x] = ColumnDev{0, 0, -1, 0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    }
+    if (threadIdx.x < 32) rowMax[threadIdx.x] = 0u;
+    __syncthreads();
+    // staging: item (field, chunk g, column c, part) takes one part of the
+    // entries j = 16 g ... 16 g + 15 of the column (even and odd j of chunk
+    // g); chunks past Js are zeros.  Sixteen consecutive items write the 16
+    // column-part slots of one 256-byte row of the staged layout (no bank
+    // conflicts).  The values stay in registers across the barrier that
+    // completes the row maxima.
+    constexpr int kItems = 4 * NC8 * 16;
+    constexpr int kItemsPerThread = (kItems + 255) / 256;
+    float v[kItemsPerThread][16];
+#pragma unroll
+    for (int it = 0; it < kItemsPerThread; ++it) {
+        const int item = threadIdx.x + 256 * it;
+        const int part = item & 1, c = (item >> 1) & 7, g = (item >> 4) % NC8, field = (item >> 4) / NC8;
+        if (field < 4) {
+            const int wv = scol[c].wv;
+            const cx<float>* src = FV + static_cast<size_t>(field) * M + static_cast<size_t>(Nj) * (wv >= 0 ? wv : 0);
+            float mx = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int j = 16 * g + t;
+                const cx<float> x = src[j < Nj ? j : Nj - 1];
+                v[it][t] = (wv >= 0 && j < Nj && j < Js) ? (part ? x.im : x.re) : 0.0f;
+                mx = fmaxf(mx, fabsf(v[it][t]));
+            }
+            atomicMax(&rowMax[field * 8 + c], __float_as_uint(mx));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kItemsPerThread; ++it) {
+        const int item = threadIdx.x + 256 * it;
+        const int part = item & 1, c = (item >> 1) & 7, g = (item >> 4) % NC8, field = (item >> 4) / NC8;
+        if (field >= 4) continue;
+        const int r = field * 8 + c;
+        const int e = split16Exponent(__uint_as_float(rowMax[r]));
+        const float sc = ldexpf(1.0f, e);
+        if (g == 0 && part == 0) rowInv[r] = ldexpf(1.0f, -e);
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+            half8 hi, lo;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                _Float16 h, l;
+                splitHalf(v[it][2 * t + par] * sc, h, l);
+                hi[t] = h;
+                lo[t] = l;
+            }
+            const int base = ((field * 2 + par) * 2) * NC8 + g;
+            sb[base * 16 + 2 * c + part] = hi;
+            sb[(base + NC8) * 16 + 2 * c + part] = lo;
+        }
+    }
+    __syncthreads();
+    const int n = lane & 15, kq = lane >> 4;
+    const int c = n >> 1, part = n & 1;
+    const ColumnDev cd = scol[c];
+    const bool colValid = cd.wv >= 0;
+    const float rowSign = (cd.flags & kColConj) ? -1.0f : 1.0f;
+    const float partWeight = part ? ((cd.flags & kColSelfConj) ? 0.0f : rowSign) : 1.0f;
+    const float ks = rowSign * static_cast<float>(cd.k), ls = rowSign * static_cast<float>(cd.l);
+    const bool store = colValid && !(plan.skipCompletion && (cd.flags & kColCompletion));
+    const int colOff = 2 * (cd.iy * nkxP + cd.kx) + part;
+    const int zStride = 2 * Ny * nkxP;
+    const size_t chStride = static_cast<size_t>(Nz) * zStride;
+    const int L = Nz - 1;
+    int maxch = 0;
+    for (int gg = 0; gg < plan.ngroups; ++gg) maxch = max(maxch, plan.g[gg].nch);
+    const int parts = (maxch + NQ - 1) / NQ;
+    const int ntasks = plan.ngroups * parts * ztiles;
+    for (int task = waveInBlock; task < ntasks; task += nwaves) {
+        const int gi = task % plan.ngroups;
+        const int pt = (task / plan.ngroups) % parts;
+        const int zt = task / (plan.ngroups * parts);
+        const SynthGroup& grp = plan.g[gi];
+        const int q0 = pt * NQ;
+        const bool dzPart = grp.ch[q0].dz != 0;
+        const int m = grp.mat + (dzPart ? 4 : 0);
+        const _Float16* AE = mats.m[m] + static_cast<size_t>(zt) * kTileHalves;
+        const _Float16* AO = mats.m[m + 2] + static_cast<size_t>(zt) * kTileHalves;
+        int field[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) field[q] = q0 + q < grp.nch ? grp.ch[q0 + q].field : 0;
+        typename MF::acc accE[NQ], accO[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) accE[q] = accO[q] = MF::zero();
+#pragma unroll
+        for (int s = 0; s < N32; ++s) {
+            const half8 aEh = reinterpret_cast<const half8*>(AE + (2 * s) * 512)[lane];
+            const half8 aEl = reinterpret_cast<const half8*>(AE + (2 * s + 1) * 512)[lane];
+            const half8 aOh = reinterpret_cast<const half8*>(AO + (2 * s) * 512)[lane];
+            const half8 aOl = reinterpret_cast<const half8*>(AO + (2 * s + 1) * 512)[lane];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int bE = ((field[q] * 2 + 0) * 2) * NC8 + 4 * s + kq, bO = ((field[q] * 2 + 1) * 2) * NC8 + 4 * s + kq;
+                const half8 bEh = sb[bE * 16 + n], bEl = sb[(bE + NC8) * 16 + n];
+                const half8 bOh = sb[bO * 16 + n], bOl = sb[(bO + NC8) * 16 + n];
+                accE[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aEl, bEh, accE[q], 0, 0, 0);
+                accE[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aEh, bEl, accE[q], 0, 0, 0);
+                accE[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aEh, bEh, accE[q], 0, 0, 0);
+                accO[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aOl, bOh, accO[q], 0, 0, 0);
+                accO[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aOh, bOl, accO[q], 0, 0, 0);
+                accO[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aOh, bOh, accO[q], 0, 0, 0);
+            }
+        }
+        if constexpr (TAIL) {
+            // k = 32 N32 + 4 (l/16) + t: chunk 4 N32 + kq/2, entries 4 (kq & 1) ...
+            const _Float16* tE = AE + N32 * 2 * 512;
+            const _Float16* tO = AO + N32 * 2 * 512;
+            const half4 aEh = reinterpret_cast<const half4*>(tE)[lane];
+            const half4 aEl = reinterpret_cast<const half4*>(tE + 256)[lane];
+            const half4 aOh = reinterpret_cast<const half4*>(tO)[lane];
+            const half4 aOl = reinterpret_cast<const half4*>(tO + 256)[lane];
+            const int ch8 = 4 * N32 + (kq >> 1), sub = 4 * (kq & 1);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int bE = ((field[q] * 2 + 0) * 2) * NC8 + ch8, bO = ((field[q] * 2 + 1) * 2) * NC8 + ch8;
+                const _Float16* pEh = reinterpret_cast<const _Float16*>(sb + bE * 16 + n) + sub;
+                const _Float16* pEl = reinterpret_cast<const _Float16*>(sb + (bE + NC8) * 16 + n) + sub;
+                const _Float16* pOh = reinterpret_cast<const _Float16*>(sb + bO * 16 + n) + sub;
+                const _Float16* pOl = reinterpret_cast<const _Float16*>(sb + (bO + NC8) * 16 + n) + sub;
+                const half4 bEh = *reinterpret_cast<const half4*>(pEh), bEl = *reinterpret_cast<const half4*>(pEl);
+                const half4 bOh = *reinterpret_cast<const half4*>(pOh), bOl = *reinterpret_cast<const half4*>(pOl);
+                accE[q] = __builtin_amdgcn_mfma_f32_16x16x16f16(aEl, bEh, accE[q], 0, 0, 0);
+                accE[q] = __builtin_amdgcn_mfma_f32_16x16x16f16(aEh, bEl, accE[q], 0, 0, 0);
+                accE[q] = __builtin_amdgcn_mfma_f32_16x16x16f16(aEh, bEh, accE[q], 0, 0, 0);
+                accO[q] = __builtin_amdgcn_mfma_f32_16x16x16f16(aOl, bOh, accO[q], 0, 0, 0);
+                accO[q] = __builtin_amdgcn_mfma_f32_16x16x16f16(aOh, bOl, accO[q], 0, 0, 0);
+                accO[q] = __builtin_amdgcn_mfma_f32_16x16x16f16(aOh, bOh, accO[q], 0, 0, 0);
+            }
+        }
+        // the channel's factor (value scale, or 1 for z derivatives), the
+        // column's part weight, and the operand scales undone
+        const float oscale = (dzPart ? 1.0f : static_cast<float>(plan.valueScale)) * partWeight * mats.inv[m];
+        float scale[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) scale[q] = oscale * rowInv[field[q] * 8 + c];
+        storeFoldedTask<float, NQ>(grp, q0, zt, kq, L, zStride, chStride, colOff, store, scale, ks, ls, part, H, accE,
+                                   accO);
     }
 }
 
@@ -1382,6 +1604,179 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
     }
 }
 
+// The folded vertical analysis with split-FP16 MFMA products (the
+// mixed-precision form of k_proj<float, NT, ., true>; the scheme of
+// k_synth16): the analysis operators (ana.m[fam + 2 P], prepared on the
+// host: per row tile [N32 x (hi, lo) x 64 x 8 | TAIL x (hi, lo) x 64 x 4],
+// lane l holding rows j = 16 tile + l%16 and folded columns z = 32 s + 8
+// (l/16) + t, resp. 32 N32 + 4 (l/16) + t) with one scale per matrix, the
+// staged sums and differences with one scale per (target, sums or
+// differences, column) from their largest entry.  The contraction covers
+// every folded row z <= L/2 (the midpoint row included).  Same waves (one
+// per 16-row tile of j, even j first) and the same transposed stores of Tg.
+template <int NT, int N32, int TAIL>
+__global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q, const ColumnDev* __restrict__ cols,
+                                                 int ncol, SynthMats16 ana, int Ja, int Nz, int Nj, int Ny, int nkxP,
+                                                 int M, cx<float>* __restrict__ Tg) {
+    using MF = Mfma<float>;
+    constexpr int NC8 = 4 * N32 + 2 * TAIL;  // 8-row chunks of the contraction
+    constexpr int kTileHalves = N32 * 1024 + TAIL * 512;
+    constexpr int kItems = NT * 16 * NC8;    // staging items (target, chunk, column, part)
+    extern __shared__ __align__(16) unsigned char smem[];
+    half8* sb = reinterpret_cast<half8*>(smem);  // [target][sums, diffs][hl][chunk][column c x part]
+    __shared__ ColumnDev scol[8];
+    __shared__ unsigned colMax[NT * 16];  // per (target, sums or diffs, column): largest |entry| as float bits
+    __shared__ float colInv[NT * 16];
+    const int lane = threadIdx.x & 63;
+    const int jt = waveIndex();
+    const int cg = xcdBlock(blockIdx.x, gridDim.x);
+    if (threadIdx.x < 8) {
+        const int col = cg * 8 + threadIdx.x;
+        if (col < ncol) scol[threadIdx.x] = cols[col];
+        else scol[threadIdx.x] = ColumnDev{0, 0, -1, kColCompletion, 0.0, 0.0, 0.0, 0.0, 0.0};
+    }
+    if (threadIdx.x < NT * 16) colMax[threadIdx.x] = 0u;
+    __syncthreads();
+    const size_t tStride = static_cast<size_t>(Nz) * Ny * nkxP, plane = static_cast<size_t>(Ny) * nkxP;
+    const int L = Nz - 1;
+    // staging: item (target t, chunk g, column c, part) forms one part of the
+    // folded sums and differences of rows z = 8 g ... 8 g + 7; sixteen
+    // consecutive items write the 16 column-part slots of one 256-byte row
+    // (launched with at least kItems / 2 threads: two items per thread at
+    // most; a wave whose items are all past the end skips the second)
+    float sv[2][8], dv[2][8];
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        if (static_cast<int>(threadIdx.x - lane) + it * static_cast<int>(blockDim.x) >= kItems) break;
+        const int item = threadIdx.x + it * blockDim.x;
+        const int part = item & 1, c = (item >> 1) & 7, g = (item >> 4) % NC8, t = (item >> 4) / NC8;
+        const bool live = item < kItems;
+        const ColumnDev& cc = scol[c];
+        const size_t colBase = static_cast<size_t>(cc.iy) * nkxP + cc.kx;
+        const bool colLive = live && cc.wv >= 0;
+        const int tt = t < NT ? t : NT - 1;
+        float mxs = 0.0f, mxd = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int z = 8 * g + i;
+            const int zc = z < Nz ? z : Nz - 1;  // clamped: loads are unconditional
+            const int zm = L - zc >= 0 ? L - zc : 0;
+            const cx<float> v = Q[tt * tStride + zc * plane + colBase];
+            const cx<float> w = Q[tt * tStride + zm * plane + colBase];
+            const float a = part ? v.im : v.re, b = part ? w.im : w.re;
+            const bool ok = colLive && 2 * z <= L, pair = z < L - z;
+            sv[it][i] = !ok ? 0.0f : pair ? a + b : a;
+            dv[it][i] = !ok || !pair ? 0.0f : a - b;
+            mxs = fmaxf(mxs, fabsf(sv[it][i]));
+            mxd = fmaxf(mxd, fabsf(dv[it][i]));
+        }
+        if (live) {
+            atomicMax(&colMax[(t * 2 + 0) * 8 + c], __float_as_uint(mxs));
+            atomicMax(&colMax[(t * 2 + 1) * 8 + c], __float_as_uint(mxd));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int item = threadIdx.x + it * blockDim.x;
+        if (item >= kItems) continue;
+        const int part = item & 1, c = (item >> 1) & 7, g = (item >> 4) % NC8, t = (item >> 4) / NC8;
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+            const int slot = (t * 2 + sd) * 8 + c;
+            const int e = split16Exponent(__uint_as_float(colMax[slot]));
+            const float sc = ldexpf(1.0f, e);
+            if (g == 0 && part == 0) colInv[slot] = ldexpf(1.0f, -e);
+            half8 hi, lo;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                _Float16 h, l;
+                splitHalf((sd ? dv[it][i] : sv[it][i]) * sc, h, l);
+                hi[i] = h;
+                lo[i] = l;
+            }
+            const int base = ((t * 2 + sd) * 2) * NC8 + g;
+            sb[base * 16 + 2 * c + part] = hi;
+            sb[(base + NC8) * 16 + 2 * c + part] = lo;
+        }
+    }
+    __syncthreads();
+    const int n = lane & 15, kq = lane >> 4;
+    const int nEven = Ja / 32;
+    const int P = jt >= nEven ? 1 : 0;  // parity of this wave's j
+    const int jh = jt - P * nEven;      // row tile within the parity
+    typename MF::acc acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = MF::zero();
+    float scale[NT];
+#pragma unroll
+    for (int fam = 0; fam < 2; ++fam) {
+        const _Float16* A = ana.m[fam + 2 * P] + static_cast<size_t>(jh) * kTileHalves;
+        const int sd = fam ^ P;  // even j of the cosine family: sums, ...
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+            if ((projTarget<NT>(t) >= 2) == (fam == 1)) scale[t] = ana.inv[fam + 2 * P] * colInv[(t * 2 + sd) * 8 + (n >> 1)];
+#pragma unroll
+        for (int s = 0; s < N32; ++s) {
+            const half8 aH = reinterpret_cast<const half8*>(A + (2 * s) * 512)[lane];
+            const half8 aL = reinterpret_cast<const half8*>(A + (2 * s + 1) * 512)[lane];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                if ((projTarget<NT>(t) >= 2) == (fam == 1)) {
+                    const int b = ((t * 2 + sd) * 2) * NC8 + 4 * s + kq;
+                    const half8 bH = sb[b * 16 + n], bL = sb[(b + NC8) * 16 + n];
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aL, bH, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aH, bL, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aH, bH, acc[t], 0, 0, 0);
+                }
+        }
+        if constexpr (TAIL) {
+            const _Float16* tA = A + N32 * 1024;
+            const half4 aH = reinterpret_cast<const half4*>(tA)[lane];
+            const half4 aL = reinterpret_cast<const half4*>(tA + 256)[lane];
+            const int ch8 = 4 * N32 + (kq >> 1), sub = 4 * (kq & 1);
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                if ((projTarget<NT>(t) >= 2) == (fam == 1)) {
+                    const int b = ((t * 2 + sd) * 2) * NC8 + ch8;
+                    const half4 bH = *reinterpret_cast<const half4*>(reinterpret_cast<const _Float16*>(sb + b * 16 + n) + sub);
+                    const half4 bL =
+                        *reinterpret_cast<const half4*>(reinterpret_cast<const _Float16*>(sb + (b + NC8) * 16 + n) + sub);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(aL, bH, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(aH, bL, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(aH, bH, acc[t], 0, 0, 0);
+                }
+        }
+    }
+    // the results through LDS as res[t][j][n], stores walking j (as k_proj)
+    constexpr int kResStride = 17;
+    __syncthreads();  // every wave has finished reading the staged operands
+    float* res = reinterpret_cast<float*>(smem);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int j = 2 * (16 * jh + MF::row(kq, i)) + P;
+            res[(t * Ja + j) * kResStride + n] = acc[t][i] * scale[t];
+        }
+    __syncthreads();
+    const int ec = lane >> 3, jj = lane & 7;
+    const ColumnDev cd = scol[ec];
+    if (cd.wv < 0 || (cd.flags & kColCompletion)) return;
+    const bool conj = (cd.flags & kColConj) != 0;
+#pragma unroll
+    for (int ii = 0; ii < 2; ++ii) {
+        const int j = 16 * jt + 8 * ii + jj;
+        if (j >= Nj) continue;
+        const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const float* rt = res + (t * Ja + j) * kResStride + 2 * ec;
+            Tg[t * static_cast<size_t>(M) + idx] = cx<float>{rt[0], conj ? -rt[1] : rt[1]};
+        }
+    }
+}
+
 // Per-mode wavenumbers of the projection.
 struct ModeDev {
     double k, l, Kh, cosA, sinA;
@@ -1844,6 +2239,15 @@ struct KernelImpl {
     // of the analysis (columns z <= (Nz-1)/2): constant N
     DeviceBuffer synCE, synSE, synCO, synSO, anaCE, anaCO, anaSE, anaSO;
     DeviceBuffer synCmE, synSmE, synCmO, synSmO;  // folded, columns times +/- m_j / 2
+    // split-FP16 (hi, lo) forms of the eight folded synthesis operators, in
+    // the order of the folded launch's slots, and their scales
+    DeviceBuffer syn16[kSynthMatSlots];
+    float syn16Inv[kSynthMatSlots] = {};
+    int syn16N32 = 0, syn16Tail = 0;  // contraction steps of 32 and 16 (0, 0: not available)
+    // the same for the four folded analysis operators (anaCE, anaSE, anaCO, anaSO)
+    DeviceBuffer ana16[4];
+    float ana16Inv[4] = {};
+    int ana16N32 = 0, ana16Tail = 0;
     bool fold = false;
     size_t Zh = 0;  // folded z rows (multiple of 16)
     DeviceBuffer FV, H, G, P, Q, Tg, modeTab, jTab, stage, weighted, partial, scalar;
@@ -1981,6 +2385,113 @@ struct KernelImpl {
         return out;
     }
 
+    // The folded synthesis operators in split-FP16 form for k_synth16: per
+    // matrix (rows z <= half of tile 16 zt + l%16, folded columns jh), per
+    // tile: N32 steps of [hi | lo][64 lanes][8] (jh = 32 s + 8 (l/16) + t)
+    // then, with a tail, [hi | lo][64][4] (jh = 32 N32 + 4 (l/16) + t);
+    // columns past Jh are zero.  The even and odd matrices of a pair share
+    // one power-of-two scale, so that their sums share it.
+    void setupSplit16Synthesis(size_t half, size_t Jh) {
+        const size_t rem = Jh % 32;
+        const int n32 = static_cast<int>(Jh / 32 + (rem > 16 ? 1 : 0)), tail = rem > 0 && rem <= 16 ? 1 : 0;
+        if (n32 > 2) return;  // longer contractions stay on the FP32 operators
+        syn16N32 = n32;
+        syn16Tail = tail;
+        struct Src {
+            const std::vector<double>* m;
+            size_t par;
+            double scale;
+        };
+        const Src src[kSynthMatSlots] = {{&d.synC, 0, 0.0}, {&d.synS, 0, 0.0}, {&d.synC, 1, 0.0}, {&d.synS, 1, 0.0},
+                                         {&d.synC, 0, 0.5}, {&d.synS, 0, -0.5}, {&d.synC, 1, 0.5}, {&d.synS, 1, -0.5}};
+        auto entry = [&](int slot, size_t z, size_t jh) {
+            if (z > half || jh >= Jh) return 0.0;
+            const size_t j = 2 * jh + src[slot].par;
+            const double a = (*src[slot].m)[z * d.Js + j];
+            return src[slot].scale == 0.0 ? a : j < d.c.Nj ? a * (src[slot].scale * d.m[j]) : 0.0;
+        };
+        const size_t tiles = Zh / 16, tileHalves = static_cast<size_t>(n32) * 1024 + static_cast<size_t>(tail) * 512;
+        for (int pair = 0; pair < kSynthMatSlots; ++pair) {
+            if (pair % 4 >= 2) continue;  // slots m and m + 2 (even, odd) are prepared together
+            float mx = 0.0f;
+            for (int slot : {pair, pair + 2})
+                for (size_t z = 0; z <= half; ++z)
+                    for (size_t jh = 0; jh < Jh; ++jh) mx = std::max(mx, static_cast<float>(std::fabs(entry(slot, z, jh))));
+            const int e = split16Exponent(mx);
+            const float sc = std::ldexp(1.0f, e);
+            for (int slot : {pair, pair + 2}) {
+                std::vector<_Float16> out(tiles * tileHalves);
+                auto put = [&](size_t base, size_t z, size_t jh) {
+                    const float v = static_cast<float>(entry(slot, z, jh)) * sc;
+                    const _Float16 hi = static_cast<_Float16>(v);
+                    out[base] = hi;
+                    return static_cast<_Float16>(v - static_cast<float>(hi));
+                };
+                for (size_t zt = 0; zt < tiles; ++zt) {
+                    _Float16* tile = out.data() + zt * tileHalves;
+                    for (int st = 0; st < n32; ++st)
+                        for (size_t l = 0; l < 64; ++l)
+                            for (size_t t = 0; t < 8; ++t) {
+                                const size_t o = static_cast<size_t>(st) * 1024 + l * 8 + t;
+                                tile[o + 512] = put(zt * tileHalves + o, 16 * zt + l % 16, 32 * st + 8 * (l / 16) + t);
+                            }
+                    if (tail)
+                        for (size_t l = 0; l < 64; ++l)
+                            for (size_t t = 0; t < 4; ++t) {
+                                const size_t o = static_cast<size_t>(n32) * 1024 + l * 4 + t;
+                                tile[o + 256] = put(zt * tileHalves + o, 16 * zt + l % 16, 32 * n32 + 4 * (l / 16) + t);
+                            }
+                }
+                upload(syn16[slot], out);
+                syn16Inv[slot] = std::ldexp(1.0f, -e);
+            }
+        }
+    }
+
+    // The folded analysis operators in split-FP16 form for k_proj16: per row
+    // tile (rows jh = 16 tile + l%16 of the even- or odd-j part), N32 steps
+    // of [hi | lo][64][8] (z = 32 s + 8 (l/16) + t) then, with a tail,
+    // [hi | lo][64][4] (z = 32 N32 + 4 (l/16) + t); one scale per matrix.
+    void setupSplit16Analysis(size_t half, size_t Jah) {
+        const int n32 = static_cast<int>(Zh / 32), tail = Zh % 32 ? 1 : 0;
+        if (n32 < 1 || n32 > 2 || Jah % 16) return;  // other lengths stay on the FP32 operators
+        ana16N32 = n32;
+        ana16Tail = tail;
+        const std::vector<double>* src[4] = {&d.anaC, &d.anaS, &d.anaC, &d.anaS};
+        const size_t tiles = Jah / 16, tileHalves = static_cast<size_t>(n32) * 1024 + static_cast<size_t>(tail) * 512;
+        for (int slot = 0; slot < 4; ++slot) {
+            const size_t par = slot / 2;
+            auto entry = [&](size_t jh, size_t z) { return z > half ? 0.0 : (*src[slot])[(2 * jh + par) * d.Za + z]; };
+            float mx = 0.0f;
+            for (size_t jh = 0; jh < Jah; ++jh)
+                for (size_t z = 0; z <= half; ++z) mx = std::max(mx, static_cast<float>(std::fabs(entry(jh, z))));
+            const int e = split16Exponent(mx);
+            const float sc = std::ldexp(1.0f, e);
+            std::vector<_Float16> out(tiles * tileHalves);
+            auto put = [&](size_t o, size_t jh, size_t z, size_t loOff) {
+                const float v = static_cast<float>(entry(jh, z)) * sc;
+                const _Float16 hi = static_cast<_Float16>(v);
+                out[o] = hi;
+                out[o + loOff] = static_cast<_Float16>(v - static_cast<float>(hi));
+            };
+            for (size_t jt = 0; jt < tiles; ++jt) {
+                const size_t base = jt * tileHalves;
+                for (int st = 0; st < n32; ++st)
+                    for (size_t l = 0; l < 64; ++l)
+                        for (size_t t = 0; t < 8; ++t)
+                            put(base + static_cast<size_t>(st) * 1024 + l * 8 + t, 16 * jt + l % 16,
+                                32 * st + 8 * (l / 16) + t, 512);
+                if (tail)
+                    for (size_t l = 0; l < 64; ++l)
+                        for (size_t t = 0; t < 4; ++t)
+                            put(base + static_cast<size_t>(n32) * 1024 + l * 4 + t, 16 * jt + l % 16,
+                                32 * n32 + 4 * (l / 16) + t, 256);
+            }
+            upload(ana16[slot], out);
+            ana16Inv[slot] = std::ldexp(1.0f, -e);
+        }
+    }
+
     template <typename T>
     void setupTyped() {
         uploadAs<T>(synC, mfmaOperand(d.synC, d.Zs, d.Js));
@@ -2021,6 +2532,10 @@ struct KernelImpl {
             uploadAs<T>(synSmE, synPart(d.synS, 0, -0.5));
             uploadAs<T>(synCmO, synPart(d.synC, 1, 0.5));
             uploadAs<T>(synSmO, synPart(d.synS, 1, -0.5));
+            if (precision == WV_PRECISION_MIXED_SPLIT16) {
+                setupSplit16Synthesis(half, Jh);
+                setupSplit16Analysis(half, Jah);
+            }
             uploadAs<T>(anaCE, anaPart(d.anaC, 0));
             uploadAs<T>(anaCO, anaPart(d.anaC, 1));
             uploadAs<T>(anaSE, anaPart(d.anaS, 0));
@@ -2320,6 +2835,33 @@ struct KernelImpl {
         const SynthMats<T> m = folded ? SynthMats<T>{{synCE.as<T>(), synSE.as<T>(), synCO.as<T>(), synSO.as<T>(),
                                                       synCmE.as<T>(), synSmE.as<T>(), synCmO.as<T>(), synSmO.as<T>()}}
                                       : (mats ? *mats : synthMats<T>());
+        if constexpr (sizeof(T) == 4) {
+            if (folded && syn16N32 + syn16Tail > 0 && maxch <= 4) {
+                // split-FP16 products: every tile on MFMA (the midpoint row
+                // included), FQ channel slots per task, four waves
+                using Fn16 = void (*)(const cx<float>*, const ColumnDev*, int, int, SynthMats16, int, int, int, int,
+                                      int, int, SynthPlan, cx<float>*);
+                const int key = syn16N32 * 2 + syn16Tail;
+                const Fn16 fn16 = key == 1 ? k_synth16<FQ, 0, 1> : key == 2 ? k_synth16<FQ, 1, 0>
+                                : key == 3 ? k_synth16<FQ, 1, 1> : key == 4 ? k_synth16<FQ, 2, 0> : k_synth16<FQ, 2, 1>;
+                SynthMats16 m16{};
+                for (int i = 0; i < kSynthMatSlots; ++i) {
+                    m16.m[i] = syn16[i].as<_Float16>();
+                    m16.inv[i] = syn16Inv[i];
+                }
+                const int nc8 = 4 * syn16N32 + 2 * syn16Tail;
+                const size_t lds16 = static_cast<size_t>(4096) * nc8;
+                timed(name, bytes, s, [&] {
+                    hipLaunchKernelGGL(fn16, dim3(static_cast<unsigned>(ncg)), dim3(256), lds16, s,
+                                       in ? in->as<cx<float>>() : FV.as<cx<float>>(), cols.as<ColumnDev>(),
+                                       static_cast<int>(ncol), static_cast<int>(d.M), m16, static_cast<int>(d.Js),
+                                       static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj), static_cast<int>(d.c.Ny),
+                                       static_cast<int>(d.nkxP), static_cast<int>(Zh / 16), sp,
+                                       out ? out->as<cx<float>>() : H.as<cx<float>>());
+                });
+                return;
+            }
+        }
         timed(name, bytes, s, [&] {
             hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(ncg)), dim3(64 * waves), lds, s,
                                in ? in->as<cx<T>>() : FV.as<cx<T>>(), cols.as<ColumnDev>(), static_cast<int>(ncol),
@@ -2435,13 +2977,45 @@ struct KernelImpl {
     // The mixed-precision flux of the constant-N nonhydrostatic transform
     // forms d/dz w from continuity (XMap<5>); the FP64 flux synthesizes every
     // derivative, as the reference's dataflow does.
-    bool deriveDzW() const { return precision == WV_PRECISION_MIXED && NT == 4 && !d.stratified && lrowValid; }
+    bool deriveDzW() const { return precision != WV_PRECISION_FP64 && NT == 4 && !d.stratified && lrowValid; }
 
     // Vertical analysis of the transformed products into Tg.
     template <typename T>
     void launchProj(hipStream_t s) {
         const int ncg = static_cast<int>((ncol + 7) / 8);
         const double bytes = static_cast<double>(NT) * (static_cast<double>(d.c.Nz) * ncol + d.M) * sizeof(cx<T>);
+        if constexpr (sizeof(T) == 4) {
+            const int njt = static_cast<int>(d.Ja / 16);
+            const int nc8 = 4 * ana16N32 + 2 * ana16Tail;
+            if (fold && ana16N32 > 0 && 2 * 64 * njt >= NT * 16 * nc8) {
+                // split-FP16 analysis: the staged operands (NT targets x sums,
+                // differences x hi, lo x chunks x 16 columns of 16 B), later
+                // the transposed results
+                const size_t staged = static_cast<size_t>(NT) * 4 * nc8 * 16 * 16;
+                const size_t results = static_cast<size_t>(d.Ja) * 4 * 17 * sizeof(float);
+                const size_t lds = std::max(staged, results);
+                using Fn16 = void (*)(const cx<float>*, const ColumnDev*, int, SynthMats16, int, int, int, int, int,
+                                      int, cx<float>*);
+                const int key = (ana16N32 - 1) * 2 + ana16Tail;
+                const Fn16 fn16 = NT == 4 ? (key == 0 ? k_proj16<4, 1, 0> : key == 1 ? k_proj16<4, 1, 1>
+                                             : key == 2 ? k_proj16<4, 2, 0> : k_proj16<4, 2, 1>)
+                                          : (key == 0 ? k_proj16<3, 1, 0> : key == 1 ? k_proj16<3, 1, 1>
+                                             : key == 2 ? k_proj16<3, 2, 0> : k_proj16<3, 2, 1>);
+                SynthMats16 a16{};
+                for (int i = 0; i < 4; ++i) {
+                    a16.m[i] = ana16[i].as<_Float16>();
+                    a16.inv[i] = ana16Inv[i];
+                }
+                timed("proj", bytes, s, [&] {
+                    hipLaunchKernelGGL(fn16, dim3(static_cast<unsigned>(ncg)), dim3(static_cast<unsigned>(64 * njt)), lds,
+                                       s, Q.as<cx<float>>(), cols.as<ColumnDev>(), static_cast<int>(ncol), a16,
+                                       static_cast<int>(d.Ja), static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj),
+                                       static_cast<int>(d.c.Ny), static_cast<int>(d.nkxP), static_cast<int>(d.M),
+                                       Tg.as<cx<float>>());
+                });
+                return;
+            }
+        }
         timed("proj", bytes, s, [&] {
             const int njt = static_cast<int>(d.Ja / 16);
             const dim3 grid(static_cast<unsigned>(ncg)), block(static_cast<unsigned>(64 * njt));
@@ -3377,7 +3951,7 @@ uint32_t wv_contract_version(void) { return WV_CONTRACT_VERSION; }
 wv_status wv_create(const wv_config* config, wv_precision precision, int device, wv_kernel** kernel) {
     if (!config || !kernel) return fail(WV_INVALID_POINTER, "wv_create requires a configuration and an output handle.");
     *kernel = nullptr;
-    if (precision != WV_PRECISION_FP64 && precision != WV_PRECISION_MIXED)
+    if (precision != WV_PRECISION_FP64 && precision != WV_PRECISION_MIXED && precision != WV_PRECISION_MIXED_SPLIT16)
         return fail(WV_INVALID_CONFIGURATION, "Unknown precision.");
     return guarded([&] {
         int count = 0;
@@ -3397,7 +3971,7 @@ wv_status wv_create_hydrostatic(const wv_config* config, const wv_vertical_opera
     if (!config || !ops || !kernel)
         return fail(WV_INVALID_POINTER, "wv_create_hydrostatic requires a configuration, operators and an output handle.");
     *kernel = nullptr;
-    if (precision != WV_PRECISION_FP64 && precision != WV_PRECISION_MIXED)
+    if (precision != WV_PRECISION_FP64 && precision != WV_PRECISION_MIXED && precision != WV_PRECISION_MIXED_SPLIT16)
         return fail(WV_INVALID_CONFIGURATION, "Unknown precision.");
     for (const double* p : {ops->z, ops->N2, ops->dLnN2, ops->h, ops->P, ops->Q, ops->PFinv, ops->PF, ops->QGinv, ops->QG})
         if (!p) return fail(WV_INVALID_POINTER, "Vertical operator pointers must be non-null.");
@@ -3489,7 +4063,9 @@ const char* wv_inverse_normalization_placement_identifier(const wv_kernel* kerne
 }
 const char* wv_optimization_implementation_identifier(const wv_kernel* kernel) {
     if (!kernel) return nullptr;
-    return kernel->impl.precision == WV_PRECISION_FP64 ? "hip-gfx950-fp64" : "hip-gfx950-mixed-fp32";
+    return kernel->impl.precision == WV_PRECISION_FP64   ? "hip-gfx950-fp64"
+           : kernel->impl.precision == WV_PRECISION_MIXED ? "hip-gfx950-mixed-fp32"
+                                                          : "hip-gfx950-mixed-split-fp16";
 }
 size_t wv_coefficient_worker_count(const wv_kernel* kernel) { (void)kernel; return 0; }
 size_t wv_phase_reservation_bytes(const wv_kernel* kernel) {

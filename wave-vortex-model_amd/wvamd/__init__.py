@@ -32,7 +32,7 @@ STATUS_NAMES = [
     "sizeOverflow", "allocationFailure", "fftPlanFailure", "fftExecutionFailure", "numericalFailure",
     "unsupportedOperation", "reentrantExecution",
 ]
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed-split16": 2}
 CONTRACT_VERSION = 4
 
 
