@@ -35,20 +35,22 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_rk4_256x256x129_mi
 # per-dispatch SQ counters (MFMA instructions among them) of the same kernels
 COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "sq_counters_rk4_256x256x129_mixed.md")
 MFMA_PEAK_TFLOPS_F32 = 157.3
+MFMA_PEAK_TFLOPS_F16 = 2500.0  # dense FP16
+MIXED = "mixed-split16"        # the headline mixed-precision form: split-FP16 vertical GEMMs
 VALU_SIMDS = 256 * 4          # CUs x SIMDs
 PEAK_CLOCK_HZ = 2.4e9         # peak engine clock
-MFMA_FLOPS_F32_16X16X4 = 2 * 16 * 16 * 4  # the only MFMA shape of the mixed RHS
 # stage name -> kernel of the mixed-precision RHS at 256x256x129
-STAGE_KERNEL = {"fields": "k_fields<float", "synth": "k_synth<float", "yfft_inv": "k_yfft<float, 8, 4, true",
-                "xfused": "k_xfused<float", "yfft_fwd": "k_yfft<float, 8, 4, false", "proj": "k_proj<float",
+STAGE_KERNEL = {"fields": "k_fields<float", "synth": "k_synth16", "yfft_inv": "k_yfft<float, 8, 4, true",
+                "xfused": "k_xfused<float", "yfft_fwd": "k_yfft<float, 8, 4, false", "proj": "k_proj16",
                 "stage": "k_stage<float"}
 
 
 def per_kernel_rooflines(rec):
     """Every kernel of the mixed RHS against the chip's peaks: algorithmic
     bytes per launch over its HIP-event time against HBM, its measured HBM
-    bytes per dispatch, and for the MFMA kernels the FP32 MFMA rate (MFMA
-    instructions per dispatch from the counter summary)."""
+    bytes per dispatch, and for the MFMA kernels the executed MFMA rate
+    (FP16 and FP32 MFMA operations per dispatch from the counter summary,
+    SQ_INSTS_VALU_MFMA_MOPS_* x 512) against the dense peak of its type."""
     from wvamd.rocprof_summary import hbm_traffic, kernel_counters
     out = []
     for r in rec:
@@ -60,10 +62,13 @@ def per_kernel_rooflines(rec):
         t = hbm_traffic(TRAFFIC_SUMMARY, prefix)
         e["traffic_MB"] = round(t["traffic_bytes"] / 1e6, 1) if t else None
         c = kernel_counters(COUNTER_SUMMARY, prefix)
-        if c and c.get("SQ_INSTS_MFMA"):
-            tflops = c["SQ_INSTS_MFMA"] * MFMA_FLOPS_F32_16X16X4 / (avg_ms * 1e-3) / 1e12
-            e["mfma_TFLOPs"] = round(tflops, 1)
-            e["mfma_frac"] = round(tflops / MFMA_PEAK_TFLOPS_F32, 4)
+        for key, peak in (("SQ_INSTS_VALU_MFMA_MOPS_F16", MFMA_PEAK_TFLOPS_F16),
+                          ("SQ_INSTS_VALU_MFMA_MOPS_F32", MFMA_PEAK_TFLOPS_F32)):
+            if c and c.get(key):
+                tflops = c[key] * 512 / (avg_ms * 1e-3) / 1e12
+                e["mfma_TFLOPs"] = round(tflops, 1)
+                e["mfma_type"] = key.rsplit("_", 1)[1].lower()
+                e["mfma_frac"] = round(tflops / peak, 4)
         out.append(e)
     return out
 
@@ -93,6 +98,28 @@ def initial_state(k, seed=4001, jmax=64, urms=0.05):
     f = k.transform_wave_vortex_to_uvw(*[k.to_device(a) for a in (Ap, Am, A0)]).cpu().numpy()
     dt = flows.cfl_time_step(f, md["k"], md["l"], k.coefficient_table("omega"), k.config["Lz"], 0.5)
     return (Ap, Am, A0), dt
+
+
+def alternating_steps(kernels, state, dt, steps, warmup, runs):
+    """timed_steps for several kernels with their repetitions interleaved
+    (kernel a run 1, kernel b run 1, ..., kernel a run 2, ...), so that a
+    drift of the clocks over the benchmark affects each alike; returns the
+    seconds of each run per kernel."""
+    import torch
+    devs, ts = [], []
+    for k in kernels:
+        dev = [k.to_device(a) for a in state]
+        ts.append(k.rk4_advance(*dev, t=0.0, h=dt, nsteps=warmup) if warmup else 0.0)
+        devs.append(dev)
+    torch.cuda.synchronize()
+    secs = [[] for _ in kernels]
+    for _ in range(runs):
+        for i, k in enumerate(kernels):
+            t0 = time.perf_counter()
+            ts[i] = k.rk4_advance(*devs[i], t=ts[i], h=dt, nsteps=steps)
+            torch.cuda.synchronize()
+            secs[i].append(time.perf_counter() - t0)
+    return secs, devs
 
 
 def timed_steps(k, state, dt, steps, warmup, runs=1):
@@ -261,7 +288,7 @@ def main():
     ap.add_argument("--energy-steps", type=int, default=200, help="steps of the untimed lockstep error run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--profile", choices=["mixed", "fp64", "both"], default=None,
+    ap.add_argument("--profile", choices=["mixed", "mixed-fp32", "fp64", "both"], default=None,
                     help="run only `steps` RK4 steps of the workload plus a 512 MiB copy kernel "
                          "(the byte-counter calibration), for rocprofv3")
     ap.add_argument("--no-c3", action="store_true", help="skip the arbitrary-stratification (512x512x65) leg")
@@ -294,8 +321,9 @@ def main():
     if args.profile:
         kk = wvamd.WVTransformConstantStratificationKernel(cfg, "fp64")
         state, dt = initial_state(kk)
+        names = {"mixed": MIXED, "mixed-fp32": "mixed", "fp64": "fp64"}
         for prec in (("mixed", "fp64") if args.profile == "both" else (args.profile,)):
-            k = kk if prec == "fp64" else wvamd.WVTransformConstantStratificationKernel(cfg, prec)
+            k = kk if prec == "fp64" else wvamd.WVTransformConstantStratificationKernel(cfg, names[prec])
             secs, _ = timed_steps(k, state, dt, args.steps, args.warmup)
             print(f"{prec}: {args.steps / secs[0]:.2f} steps/s", flush=True)
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
@@ -305,12 +333,13 @@ def main():
         print("copy calibration: 536870912 bytes read, 536870912 bytes written", flush=True)
         return
     k64 = wvamd.WVTransformConstantStratificationKernel(cfg, "fp64")
-    k32 = wvamd.WVTransformConstantStratificationKernel(cfg, "mixed")
+    k32 = wvamd.WVTransformConstantStratificationKernel(cfg, MIXED)
+    kf32 = wvamd.WVTransformConstantStratificationKernel(cfg, "mixed")  # FP32-MFMA vertical GEMMs, for comparison
     state, dt = initial_state(k64)
 
     runs = max(1, args.runs)
-    secs32, d32 = timed_steps(k32, state, dt, args.steps, args.warmup, runs)
-    secs64, d64 = timed_steps(k64, state, dt, args.steps, args.warmup, runs)
+    (secs32, secs64, secsf), (d32, d64, df) = alternating_steps((k32, k64, kf32), state, dt, args.steps,
+                                                                args.warmup, runs)
     e32 = k64.total_energy(*d32)
     e64 = k64.total_energy(*d64)
     energy_err = abs(e32 - e64) / abs(e64)
@@ -318,7 +347,9 @@ def main():
     sec64 = float(np.median(secs64))
     sps32 = args.steps / sec32
     sps64 = args.steps / sec64
-    del d32, d64
+    spsf = args.steps / float(np.median(secsf))
+    ef = abs(k64.total_energy(*df) - e64) / abs(e64)
+    del d32, d64, df, kf32
     worst_energy, coef_err = lockstep_errors(k32, k64, state, dt, args.energy_steps) if args.energy_steps > 0 \
         else (None, None)
 
@@ -369,7 +400,8 @@ def main():
     out = {
         "metric": METRIC, "value": round(sps32, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * sec32 / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32 fields/FFTs/MFMA, f64 coefficients+RK4",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 fields/FFTs, split-f16 MFMA vertical GEMMs (f32 accumulate), f64 coefficients+RK4",
         "data": "synthetic (seeded |k|^-2 flow, rms u 5 cm/s, dt = 0.5 CFL)",
         "config": {"workload": "wave-vortex nonlinear flux + fixed RK4, 256x256x129 nonhydrostatic constant N",
                    "Lx_m": 200e3, "Lz_m": 1300.0, "Nj": 85, "Nkl": k64.Nkl, "dt_s": dt},
@@ -377,6 +409,9 @@ def main():
         "mixed_over_fp64": round(sps32 / sps64, 3), "energy_rel_err_mixed_vs_fp64": energy_err,
         "runs": runs, "mixed_steps_per_s_runs": [round(args.steps / s, 2) for s in secs32],
         "fp64_steps_per_s_runs": [round(args.steps / s, 2) for s in secs64],
+        "mixed_fp32_mfma": {"steps_per_s": round(spsf, 3), "over_fp64": round(spsf / sps64, 3),
+                            "energy_rel_err_vs_fp64": ef,
+                            "note": "the same mixed path with the vertical GEMMs on FP32 MFMA"},
         "lockstep": {"steps": args.energy_steps, "energy_rel_err_max_over_steps": worst_energy,
                      "coef_rel_err_max_abs": coef_err,
                      "note": "coef_rel_err_max_abs.APV: the APV field (eta_true = eta, constant-N background) "
