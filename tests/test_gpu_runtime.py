@@ -3,14 +3,18 @@
 tests/runtime/runtime_driver.cpp is built twice from the same source: over
 the reference kernel (oracle/_ref/runtime_driver_ref, CPU; its output is
 tests/golden/runtime_driver.npz) and over the drop-in adapter
-(wave-vortex-model_amd/adapter/_build/runtime_driver_hip: the contract's
+(oracle/_ref/hip_runtime/runtime_driver_hip: the contract's
 class name and header over libwvamd).  The runtime sources -- the forcing
 engine, the constant-stratification integration system, WVFixedStepRK4
-(advanceToTime with a clipped last step), WVAdaptiveRK23 and the observers
+(advanceToTime with a clipped last step), WVAdaptiveRK23, the built-in
+forcings of the frozen schedule besides nonlinear advection (adaptive
+damping, beta-plane PV advection, linear and quadratic bottom friction,
+pseudo-topographic wave generation, fixed amplitudes) and the observers
 coupled to the right-hand side (Lagrangian floats with spline and drifters
 with linear velocity sampling, an antialiased F-grid dye, all fed by the
-kernel's advection fields) -- are compiled unchanged in both.  Results must agree: states and right-hand sides at 1e-12
-relative (max|diff| / max|ref|), times and step counts exactly.
+kernel's advection fields) -- are compiled unchanged in both.  Results must
+agree: states and right-hand sides at 1e-12 relative (max|diff| /
+max|ref|), times and step counts exactly.
 """
 import os
 import subprocess
@@ -25,7 +29,7 @@ from driver_io import read_records  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-DRIVER = os.path.join(ROOT, "wave-vortex-model_amd", "adapter", "_build", "runtime_driver_hip")
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "hip_runtime", "runtime_driver_hip")
 # scalars compared exactly: end times, step counts and sizes, RHS evaluations
 EXACT = {"rk4_nh_t", "rk4_nh_steps", "rk4_nh_last_step", "rk4_h_t", "rk4_h_steps", "rk4_h_last_step", "rk23_t",
          "rk23_accepted", "rk23_rejected", "rk23_rhs", "obs_rhs_count"}
