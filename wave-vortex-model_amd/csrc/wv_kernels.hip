@@ -1614,28 +1614,36 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
 // differences, column) from their largest entry.  The contraction covers
 // every folded row z <= L/2 (the midpoint row included).  Same waves (one
 // per 16-row tile of j, even j first) and the same transposed stores of Tg.
-template <int NT, int N32, int TAIL>
-__global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q, const ColumnDev* __restrict__ cols,
+template <int NT, int NTB, int N32, int TAIL>
+__global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Qall, const ColumnDev* __restrict__ cols,
                                                  int ncol, SynthMats16 ana, int Ja, int Nz, int Nj, int Ny, int nkxP,
-                                                 int M, cx<float>* __restrict__ Tg) {
+                                                 int M, cx<float>* __restrict__ TgAll) {
     using MF = Mfma<float>;
     constexpr int NC8 = 4 * N32 + 2 * TAIL;  // 8-row chunks of the contraction
     constexpr int kTileHalves = N32 * 1024 + TAIL * 512;
-    constexpr int kItems = NT * 16 * NC8;    // staging items (target, chunk, column, part)
+    constexpr int kItems = NTB * 16 * NC8;   // staging items (target, chunk, column, part)
+    // workgroup -> (column group cg, targets t0 ... t0 + NTB - 1 of the
+    // NT): one XCD runs consecutive column groups of one target set
+    const int ncg = (ncol + 7) / 8;
+    const int logical = xcdBlock(blockIdx.x, gridDim.x);
+    const int cg = logical % ncg;
+    const int t0 = (logical / ncg) * NTB;
+    const size_t tStrideAll = static_cast<size_t>(Nz) * Ny * nkxP;
+    const cx<float>* __restrict__ Q = Qall + t0 * tStrideAll;
+    cx<float>* __restrict__ Tg = TgAll + t0 * static_cast<size_t>(M);
     extern __shared__ __align__(16) unsigned char smem[];
     half8* sb = reinterpret_cast<half8*>(smem);  // [target][sums, diffs][hl][chunk][column c x part]
     __shared__ ColumnDev scol[8];
-    __shared__ unsigned colMax[NT * 16];  // per (target, sums or diffs, column): largest |entry| as float bits
-    __shared__ float colInv[NT * 16];
+    __shared__ unsigned colMax[NTB * 16];  // per (target, sums or diffs, column): largest |entry| as float bits
+    __shared__ float colInv[NTB * 16];
     const int lane = threadIdx.x & 63;
     const int jt = waveIndex();
-    const int cg = xcdBlock(blockIdx.x, gridDim.x);
     if (threadIdx.x < 8) {
         const int col = cg * 8 + threadIdx.x;
         if (col < ncol) scol[threadIdx.x] = cols[col];
         else scol[threadIdx.x] = ColumnDev{0, 0, -1, kColCompletion, 0.0, 0.0, 0.0, 0.0, 0.0};
     }
-    if (threadIdx.x < NT * 16) colMax[threadIdx.x] = 0u;
+    if (threadIdx.x < NTB * 16) colMax[threadIdx.x] = 0u;
     __syncthreads();
     const size_t tStride = static_cast<size_t>(Nz) * Ny * nkxP, plane = static_cast<size_t>(Ny) * nkxP;
     const int L = Nz - 1;
@@ -1650,11 +1658,11 @@ __global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q
         if (static_cast<int>(threadIdx.x - lane) + it * static_cast<int>(blockDim.x) >= kItems) break;
         const int item = threadIdx.x + it * blockDim.x;
         const int part = item & 1, c = (item >> 1) & 7, g = (item >> 4) % NC8, t = (item >> 4) / NC8;
-        const bool live = item < kItems;
+        const bool live = item < kItems && t0 + t < NT;
         const ColumnDev& cc = scol[c];
         const size_t colBase = static_cast<size_t>(cc.iy) * nkxP + cc.kx;
         const bool colLive = live && cc.wv >= 0;
-        const int tt = t < NT ? t : NT - 1;
+        const int tt = t < NTB && t0 + t < NT ? t : 0;
         float mxs = 0.0f, mxd = 0.0f;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -1705,24 +1713,35 @@ __global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q
     const int nEven = Ja / 32;
     const int P = jt >= nEven ? 1 : 0;  // parity of this wave's j
     const int jh = jt - P * nEven;      // row tile within the parity
-    typename MF::acc acc[NT];
+    // family of each of this workgroup's targets (u, v: cosine; w, eta:
+    // sine); -1 past the last target
+    int tfam[NTB];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = MF::zero();
-    float scale[NT];
+    for (int t = 0; t < NTB; ++t) tfam[t] = t0 + t < NT ? (projTarget<NT>(t0 + t) >= 2 ? 1 : 0) : -1;
+    typename MF::acc acc[NTB];
+#pragma unroll
+    for (int t = 0; t < NTB; ++t) acc[t] = MF::zero();
+    float scale[NTB];
+#pragma unroll
+    for (int t = 0; t < NTB; ++t) scale[t] = 0.0f;
 #pragma unroll
     for (int fam = 0; fam < 2; ++fam) {
         const _Float16* A = ana.m[fam + 2 * P] + static_cast<size_t>(jh) * kTileHalves;
         const int sd = fam ^ P;  // even j of the cosine family: sums, ...
+        bool any = false;
 #pragma unroll
-        for (int t = 0; t < NT; ++t)
-            if ((projTarget<NT>(t) >= 2) == (fam == 1)) scale[t] = ana.inv[fam + 2 * P] * colInv[(t * 2 + sd) * 8 + (n >> 1)];
+        for (int t = 0; t < NTB; ++t) any = any || tfam[t] == fam;
+        if (!any) continue;  // uniform: no target of this family here
+#pragma unroll
+        for (int t = 0; t < NTB; ++t)
+            if (tfam[t] == fam) scale[t] = ana.inv[fam + 2 * P] * colInv[(t * 2 + sd) * 8 + (n >> 1)];
 #pragma unroll
         for (int s = 0; s < N32; ++s) {
             const half8 aH = reinterpret_cast<const half8*>(A + (2 * s) * 512)[lane];
             const half8 aL = reinterpret_cast<const half8*>(A + (2 * s + 1) * 512)[lane];
 #pragma unroll
-            for (int t = 0; t < NT; ++t)
-                if ((projTarget<NT>(t) >= 2) == (fam == 1)) {
+            for (int t = 0; t < NTB; ++t)
+                if (tfam[t] == fam) {
                     const int b = ((t * 2 + sd) * 2) * NC8 + 4 * s + kq;
                     const half8 bH = sb[b * 16 + n], bL = sb[(b + NC8) * 16 + n];
                     acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(aL, bH, acc[t], 0, 0, 0);
@@ -1736,8 +1755,8 @@ __global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q
             const half4 aL = reinterpret_cast<const half4*>(tA + 256)[lane];
             const int ch8 = 4 * N32 + (kq >> 1), sub = 4 * (kq & 1);
 #pragma unroll
-            for (int t = 0; t < NT; ++t)
-                if ((projTarget<NT>(t) >= 2) == (fam == 1)) {
+            for (int t = 0; t < NTB; ++t)
+                if (tfam[t] == fam) {
                     const int b = ((t * 2 + sd) * 2) * NC8 + ch8;
                     const half4 bH = *reinterpret_cast<const half4*>(reinterpret_cast<const _Float16*>(sb + b * 16 + n) + sub);
                     const half4 bL =
@@ -1753,7 +1772,7 @@ __global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q
     __syncthreads();  // every wave has finished reading the staged operands
     float* res = reinterpret_cast<float*>(smem);
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+    for (int t = 0; t < NTB; ++t)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int j = 2 * (16 * jh + MF::row(kq, i)) + P;
@@ -1770,7 +1789,8 @@ __global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q
         if (j >= Nj) continue;
         const size_t idx = static_cast<size_t>(j) + static_cast<size_t>(Nj) * static_cast<size_t>(cd.wv);
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
+        for (int t = 0; t < NTB; ++t) {
+            if (t0 + t >= NT) continue;
             const float* rt = res + (t * Ja + j) * kResStride + 2 * ec;
             Tg[t * static_cast<size_t>(M) + idx] = cx<float>{rt[0], conj ? -rt[1] : rt[1]};
         }
@@ -1860,6 +1880,102 @@ __device__ __forceinline__ CoefRow coefficients(const CoefConsts& c, int j, cons
     return r;
 }
 
+// The mixed-precision form of the same tables: every quotient and product in
+// FP32 (fast quotients), except the frequency, which the phase needs to FP64
+// accuracy (the angle omega t reaches hundreds of radians over a run).  The
+// spectra and the projected fluxes they feed carry FP32 accuracy anyway.
+struct CoefRowF {
+    double om;
+    float NAp, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled;
+    cx<float> UAp, VAp, WAp, UA0, VA0, ApmD;
+};
+__device__ __forceinline__ CoefRowF coefficientsF(const CoefConsts& c, int j, const ModeDev& md) {
+    // a / b as a times the hardware reciprocal of b (about 2 ulp)
+    auto dv = [](float a, float b) { return a * __builtin_amdgcn_rcpf(b); };
+    CoefRowF r{};
+    const double Khd = md.Kh, Kh2d = Khd * Khd, Mjd = c.m[j];
+    const bool wave = Khd > 0.0 && j > 0, inertial = Khd == 0.0, geostrophic = Khd > 0.0, mda = Khd == 0.0 && j > 0;
+    double hpmd = 1.0;
+    if (j > 0)
+        hpmd = c.stratified ? c.h0[j]
+               : c.hydro    ? c.N02 / (c.g * Mjd * Mjd)
+                            : (c.N02 - c.f2) / (c.g * (Mjd * Mjd + Kh2d));
+    const double omd = sqrt(c.g * hpmd * Kh2d + c.f2);
+    r.om = omd;
+    const float Kh = static_cast<float>(Khd), hpm = static_cast<float>(hpmd), om = static_cast<float>(omd);
+    const float Mj = static_cast<float>(Mjd), h0 = static_cast<float>(c.h0[j]);
+    const float Fg = static_cast<float>(c.Fg[j]), Gg = static_cast<float>(c.Gg[j]);
+    const float f = static_cast<float>(c.f), f2 = static_cast<float>(c.f2), g = static_cast<float>(c.g);
+    const float sign = j % 2 == 0 ? 1.0f : -1.0f;
+    float Fw = Fg, Gw = Gg;
+    if (!c.hydro) {
+        Fw = j == 0 ? 2.0f : sign * hpm * Mj * static_cast<float>(c.S);
+        Gw = j == 0 ? 1.0f : sign * static_cast<float>(c.S);
+    }
+    const float Fwg = dv(Fg, Fw), Gwg = dv(Gg, Gw);
+    const float fs = dv(Fg, Fwg);
+    const float prefactor = c.stratified ? 0.0f : sign * static_cast<float>(c.P0);
+    r.ApmDScaled = 0.5f * Mj * prefactor;
+    const float cosA = static_cast<float>(md.cosA), sinA = static_cast<float>(md.sinA);
+    if (wave) {
+        const float gs = static_cast<float>(c.gws[j]);
+        const float fo = dv(f, om);
+        r.UAp = {cosA * fs, -fo * sinA * fs};
+        r.VAp = {sinA * fs, fo * cosA * fs};
+        r.WAp = {0.0f, -Kh * hpm * gs};
+        r.NAp = dv(-Kh * hpm, om) * gs;
+        const float deltaScale = dv(h0 * Gwg, Fg);
+        const float twoKh = 2.0f * Kh * hpm;
+        r.ApmD = {0.0f, dv(-1.0f, twoKh) * deltaScale};
+        r.ApmN = dv(dv(-om, twoKh) * Gwg, Gg);
+    } else if (inertial) {
+        r.UAp = {fs, 0.0f};
+        r.VAp = {0.0f, fs};
+    }
+    if (geostrophic) {
+        const float lr2inv = j == 0 ? 0.0f : dv(f2, g * h0);
+        const float den = Kh * Kh + lr2inv;
+        r.UA0 = {0.0f, dv(static_cast<float>(md.l), den) * Fg};
+        r.VA0 = {0.0f, dv(-static_cast<float>(md.k), den) * Fg};
+        r.NA0 = j == 0 ? 0.0f : dv(-dv(f, g), den) * Gg;
+        r.A0Vort = dv(1.0f, Fg);
+        r.A0Buoy = j == 0 ? 0.0f : dv(dv(-f, h0), Gg);
+    } else if (mda) {
+        r.NA0 = Gg;
+        r.A0Buoy = dv(1.0f, Gg);
+        r.A0Vort = dv(f2, 2.0f * h0 * Fg);
+    }
+    return r;
+}
+
+// exp(i omega elapsed) in FP32: the angle formed and reduced modulo 2 pi in
+// FP64 (two-part 2 pi, Cody-Waite), then the FP32 sine and cosine of the
+// reduced angle.
+__device__ __forceinline__ cx<float> phaseF(double omega, double elapsed) {
+    constexpr double kTwoPiHi = 6.283185307179586232, kTwoPiLo = 2.4492935982947064e-16;
+    const double a = omega * elapsed;
+    const double q = rint(a * (1.0 / kTwoPiHi));
+    const double r = fma(-q, kTwoPiLo, fma(-q, kTwoPiHi, a));
+    float sn, cs;
+    sincosf(static_cast<float>(r), &sn, &cs);
+    return cx<float>{cs, sn};
+}
+
+// storeFieldSpectra with the FP32 tables.
+__device__ __forceinline__ void storeFieldSpectraF(const CoefRowF& cr, int idx, int M, cx<double> Apd, cx<double> Amd,
+                                                   cx<double> A0d, cx<float> p, cx<float>* __restrict__ FV) {
+    auto f = [](cx<double> a) { return cx<float>{static_cast<float>(a.re), static_cast<float>(a.im)}; };
+    const cx<float> ap = cmul(f(Apd), p), am = cmul(f(Amd), cconj(p)), a0 = f(A0d);
+    const cx<float> u = cadd(cadd(cmul(cr.UAp, ap), cmul(cconj(cr.UAp), am)), cmul(cr.UA0, a0));
+    const cx<float> v = cadd(cadd(cmul(cr.VAp, ap), cmul(cconj(cr.VAp), am)), cmul(cr.VA0, a0));
+    const cx<float> w = cadd(cmul(cr.WAp, ap), cmul(cr.WAp, am));
+    const cx<float> e = cadd(cadd(cscale(ap, cr.NAp), cscale(am, -cr.NAp)), cscale(a0, cr.NA0));
+    FV[idx] = u;
+    FV[M + idx] = v;
+    FV[2 * M + idx] = w;
+    FV[3 * M + idx] = e;
+}
+
 // Evolved field spectra of (Ap, Am, A0) at the given elapsed time, with the
 // coefficient tables formed by coefficients() -- the same values k_stage
 // forms when it emits the next evaluation's spectra, so that an RK4 run gives
@@ -1872,87 +1988,38 @@ __global__ __launch_bounds__(256) void k_fields(const cx<double>* __restrict__ A
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= M) return;
     const int wv = idx / Nj, j = idx - wv * Nj;
-    const CoefRow cr = coefficients(cc, j, modes[wv]);
-    storeFieldSpectra(cr.UAp, cr.VAp, cr.WAp, cr.NAp, cr.UA0, cr.VA0, cr.NA0, idx, M, Ap[idx], Am[idx], A0[idx],
-                      phaseAt(cr.om, elapsed), FV);
+    if constexpr (sizeof(T) == 4) {
+        const CoefRowF cr = coefficientsF(cc, j, modes[wv]);
+        storeFieldSpectraF(cr, idx, M, Ap[idx], Am[idx], A0[idx], phaseF(cr.om, elapsed), FV);
+    } else {
+        const CoefRow cr = coefficients(cc, j, modes[wv]);
+        storeFieldSpectra(cr.UAp, cr.VAp, cr.WAp, cr.NAp, cr.UA0, cr.VA0, cr.NA0, idx, M, Ap[idx], Am[idx], A0[idx],
+                          phaseAt(cr.om, elapsed), FV);
+    }
 }
 
-// Wave-vortex projection of the analysed products (projectFluxTargetMode,
-// ...Kernel.cpp) and the flux store or RK4 stage of WVFixedStepRK4::step,
-// one coefficient (j, mode) per thread; in RK modes with emitFields set it
-// also forms the field spectra of the state it produces (the input of the
-// next flux evaluation).
-template <typename T, int NT>
-__global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, const ModeDev* __restrict__ modes,
-                                               int Nj, int M, CoefConsts cc, double elapsed, double hscale,
-                                               Epilogue ep, cx<T>* __restrict__ FV) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= M) return;
-    const int wv = idx / Nj, j = idx - wv * Nj;
-    constexpr bool hydroT = NT == 3;  // hydrostatic: ApmD, else ApmDScaled
-    const ModeDev md = modes[wv];
-    const bool inertial = wv == 0;
-    // the coefficient tables, formed here instead of read from memory
-    const CoefRow cr = coefficients(cc, j, md);
-    const double om = cr.om, a0v = cr.A0Vort, a0b = cr.A0Buoy, na0 = cr.NA0, apmn = cr.ApmN;
-    const cx<double> apmd = cr.ApmD;
-    const double dsc = cr.ApmDScaled;
-    cx<T> tv[NT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) tv[t] = Tg[t * static_cast<size_t>(M) + idx];
-    const cx<double> p = phaseAt(om, elapsed);
-    cx<double> Fp{0.0, 0.0}, Fm{0.0, 0.0}, F0{0.0, 0.0};
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int target = projTarget<NT>(t);
-        const cx<double> value = cscale(cx<double>{static_cast<double>(tv[t].re), static_cast<double>(tv[t].im)}, hscale);
-        cx<double> a0{0.0, 0.0};
-        if (target == 0) a0 = cscale(cmuli(value, -md.l), a0v);
-        if (target == 1) a0 = cscale(cmuli(value, md.k), a0v);
-        if (target == 3) a0 = cscale(value, a0b);
-        const cx<double> buoyValue = csub(target == 3 ? value : cx<double>{0.0, 0.0}, cscale(a0, na0));
-        const cx<double> buoy = cscale(buoyValue, apmn);
-        cx<double> wave{0.0, 0.0};
-        if constexpr (hydroT) {
-            if (target == 0) wave = cmul(cmuli(value, md.k), apmd);
-            if (target == 1) wave = cmul(cmuli(value, md.l), apmd);
-        } else {
-            if (target == 0) wave = cscale(value, md.cosA * dsc);
-            if (target == 1) wave = cscale(value, md.sinA * dsc);
-            if (target == 2) wave = cmuli(value, (md.Kh / 2.0) * cc.apmW[j]);
-        }
-        cx<double> cAp = cadd(wave, buoy), cAm = csub(wave, buoy);
-        if (inertial) {
-            cx<double> a{0.0, 0.0};
-            if (target == 0) a = cscale(value, cc.inertialScale[j]);
-            if (target == 1) a = cscale(cmuli(value, -1.0), cc.inertialScale[j]);
-            cAp = a;
-            cAm = cconj(a);
-        }
-        Fp = cadd(Fp, cmul(cAp, cconj(p)));
-        Fm = cadd(Fm, cmul(cAm, p));
-        F0 = cadd(F0, a0);
-    }
-    const cx<double> k3[3] = {Fp, Fm, F0};
+// The flux store or RK4 stage of WVFixedStepRK4::step for one coefficient,
+// given its flux k3 = (Fp, Fm, F0); next = the state it produces (RK modes).
+// The weighted flux sum k1 + 2 k2 + 2 k3 + k4 is stored in the flux's own
+// precision T (the fluxes of the mixed path carry FP32 accuracy; the state
+// update y + h/6 w is FP64 in both).  Stage arithmetic as scaledSum
+// (value + scale*increment, WVRungeKutta.cpp:15-19), rounded per operation as
+// on the CPU.
+template <typename T>
+__device__ __forceinline__ void rkStageUpdate(const Epilogue& ep, int idx, const cx<double> (&k3)[3],
+                                              cx<double> (&next)[3]) {
     cx<double>* outs[3] = {ep.out0, ep.out1, ep.out2};
     cx<double>* ys[3] = {ep.y0, ep.y1, ep.y2};
     cx<double>* sts[3] = {ep.st0, ep.st1, ep.st2};
-    // The weighted flux sum k1 + 2 k2 + 2 k3 + k4 is stored in the flux's own
-    // precision T (the fluxes of the mixed path carry FP32 accuracy; the
-    // state update y + h/6 w below is FP64 in both).
     cx<T>* ws[3] = {reinterpret_cast<cx<T>*>(ep.w0), reinterpret_cast<cx<T>*>(ep.w1), reinterpret_cast<cx<T>*>(ep.w2)};
     auto wload = [&](int a) {
         const cx<T> v = ws[a][idx];
         return cx<double>{static_cast<double>(v.re), static_cast<double>(v.im)};
     };
     auto wstore = [&](int a, cx<double> v) { ws[a][idx] = {static_cast<T>(v.re), static_cast<T>(v.im)}; };
-    // RK4 stage arithmetic of WVFixedStepRK4::step with scaledSum
-    // (value + scale*increment, WVRungeKutta.cpp:15-19), rounded per
-    // operation as on the CPU.
     auto axpy = [](cx<double> y, cx<double> x, double s) {
         return cx<double>{addRn(y.re, mulRn(s, x.re)), addRn(y.im, mulRn(s, x.im))};
     };
-    cx<double> next[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const cx<double> kv = k3[a];
@@ -1977,13 +2044,109 @@ __global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, con
             }
         }
     }
+}
+
+// Wave-vortex projection of one target's analysed value (projectFluxTargetMode,
+// ...Kernel.cpp) added to (Fp, Fm, F0) with phase p, in arithmetic R; the
+// tables in cr (CoefRow or CoefRowF).
+template <typename R, int NT, typename Row>
+__device__ __forceinline__ void projectTarget(int target, cx<R> value, const Row& cr, const ModeDev& md, R apmW,
+                                              R inertialScale, bool inertial, cx<R> p, cx<R>& Fp, cx<R>& Fm,
+                                              cx<R>& F0) {
+    constexpr bool hydroT = NT == 3;  // hydrostatic: ApmD, else ApmDScaled
+    const R k = static_cast<R>(md.k), l = static_cast<R>(md.l);
+    cx<R> a0{R(0), R(0)};
+    if (target == 0) a0 = cscale(cmuli(value, -l), static_cast<R>(cr.A0Vort));
+    if (target == 1) a0 = cscale(cmuli(value, k), static_cast<R>(cr.A0Vort));
+    if (target == 3) a0 = cscale(value, static_cast<R>(cr.A0Buoy));
+    const cx<R> buoyValue = csub(target == 3 ? value : cx<R>{R(0), R(0)}, cscale(a0, static_cast<R>(cr.NA0)));
+    const cx<R> buoy = cscale(buoyValue, static_cast<R>(cr.ApmN));
+    cx<R> wave{R(0), R(0)};
+    if constexpr (hydroT) {
+        const cx<R> apmd{static_cast<R>(cr.ApmD.re), static_cast<R>(cr.ApmD.im)};
+        if (target == 0) wave = cmul(cmuli(value, k), apmd);
+        if (target == 1) wave = cmul(cmuli(value, l), apmd);
+    } else {
+        const R dsc = static_cast<R>(cr.ApmDScaled);
+        if (target == 0) wave = cscale(value, static_cast<R>(md.cosA) * dsc);
+        if (target == 1) wave = cscale(value, static_cast<R>(md.sinA) * dsc);
+        if (target == 2) wave = cmuli(value, (static_cast<R>(md.Kh) / R(2)) * apmW);
+    }
+    cx<R> cAp = cadd(wave, buoy), cAm = csub(wave, buoy);
+    if (inertial) {
+        cx<R> a{R(0), R(0)};
+        if (target == 0) a = cscale(value, inertialScale);
+        if (target == 1) a = cscale(cmuli(value, R(-1)), inertialScale);
+        cAp = a;
+        cAm = cconj(a);
+    }
+    Fp = cadd(Fp, cmul(cAp, cconj(p)));
+    Fm = cadd(Fm, cmul(cAm, p));
+    F0 = cadd(F0, a0);
+}
+
+// Wave-vortex projection of the analysed products tv (projectFluxTargetMode,
+// ...Kernel.cpp) of coefficient idx = j + Nj wv and the flux store or RK4
+// stage of WVFixedStepRK4::step; in RK modes with emitFields set it also
+// forms the field spectra of the state it produces (the input of the next
+// flux evaluation).  FP64 (T = double): tables, phase and projection in
+// FP64, the tables bit for bit the host's; mixed (T = float): the FP32
+// tables of coefficientsF, the FP32 phase of phaseF and the projection in
+// FP32.  The stage arithmetic on the state is FP64 in both.
+template <typename T, int NT>
+__device__ __forceinline__ void stageCoefficient(const cx<T> (&tv)[NT], int idx, int j, int wv,
+                                                 const ModeDev* __restrict__ modes, int M, const CoefConsts& cc,
+                                                 double elapsed, double hscale, const Epilogue& ep,
+                                                 cx<T>* __restrict__ FV) {
+    const ModeDev md = modes[wv];
+    const bool inertial = wv == 0;
+    // the coefficient tables, formed here instead of read from memory
+    using Row = typename std::conditional<sizeof(T) == 4, CoefRowF, CoefRow>::type;
+    Row cr;
+    cx<T> p;
+    if constexpr (sizeof(T) == 4) {
+        cr = coefficientsF(cc, j, md);
+        p = phaseF(cr.om, elapsed);
+    } else {
+        cr = coefficients(cc, j, md);
+        p = phaseAt(cr.om, elapsed);
+    }
+    cx<T> Fp{T(0), T(0)}, Fm{T(0), T(0)}, F0{T(0), T(0)};
+    const T apmW = static_cast<T>(cc.apmW[j]), iscale = static_cast<T>(cc.inertialScale[j]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+        projectTarget<T, NT>(projTarget<NT>(t), cscale(tv[t], static_cast<T>(hscale)), cr, md, apmW, iscale, inertial,
+                             p, Fp, Fm, F0);
+    auto d = [](cx<T> a) { return cx<double>{static_cast<double>(a.re), static_cast<double>(a.im)}; };
+    const cx<double> k3[3] = {d(Fp), d(Fm), d(F0)};
+    cx<double> next[3];
+    rkStageUpdate<T>(ep, idx, k3, next);
     if (ep.mode != kFluxOut && ep.emitFields) {
         // stages 2 and 4 produce the state of their own time: the same phase
-        cx<double> pn = p;
-        if (ep.nextElapsed != elapsed) pn = phaseAt(om, ep.nextElapsed);
-        storeFieldSpectra(cr.UAp, cr.VAp, cr.WAp, cr.NAp, cr.UA0, cr.VA0, cr.NA0, idx, M, next[0], next[1], next[2], pn,
-                          FV);
+        if constexpr (sizeof(T) == 4) {
+            const cx<float> pn = ep.nextElapsed != elapsed ? phaseF(cr.om, ep.nextElapsed) : p;
+            storeFieldSpectraF(cr, idx, M, next[0], next[1], next[2], pn, FV);
+        } else {
+            const cx<double> pn = ep.nextElapsed != elapsed ? phaseAt(cr.om, ep.nextElapsed) : p;
+            storeFieldSpectra(cr.UAp, cr.VAp, cr.WAp, cr.NAp, cr.UA0, cr.VA0, cr.NA0, idx, M, next[0], next[1],
+                              next[2], pn, FV);
+        }
     }
+}
+
+// stageCoefficient over the analysed products Tg[t][idx], one coefficient
+// (j, mode) per thread.
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_stage(const cx<T>* __restrict__ Tg, const ModeDev* __restrict__ modes,
+                                               int Nj, int M, CoefConsts cc, double elapsed, double hscale,
+                                               Epilogue ep, cx<T>* __restrict__ FV) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M) return;
+    const int wv = idx / Nj, j = idx - wv * Nj;
+    cx<T> tv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) tv[t] = Tg[t * static_cast<size_t>(M) + idx];
+    stageCoefficient<T, NT>(tv, idx, j, wv, modes, M, cc, elapsed, hscale, ep, FV);
 }
 
 // r2c along x of nf real FP64 fields [Nx,Ny,Nz,nf] into P[c][z][y][kx < nkx]
@@ -2987,27 +3150,32 @@ struct KernelImpl {
         if constexpr (sizeof(T) == 4) {
             const int njt = static_cast<int>(d.Ja / 16);
             const int nc8 = 4 * ana16N32 + 2 * ana16Tail;
-            if (fold && ana16N32 > 0 && 2 * 64 * njt >= NT * 16 * nc8) {
-                // split-FP16 analysis: the staged operands (NT targets x sums,
-                // differences x hi, lo x chunks x 16 columns of 16 B), later
-                // the transposed results
-                const size_t staged = static_cast<size_t>(NT) * 4 * nc8 * 16 * 16;
-                const size_t results = static_cast<size_t>(d.Ja) * 4 * 17 * sizeof(float);
+            // two targets per workgroup (u, v of the cosine family; w, eta
+            // or eta alone of the sine family): half the staged operands,
+            // twice the workgroups
+            constexpr int kNTB = 2;
+            if (fold && ana16N32 > 0 && 2 * 64 * njt >= kNTB * 16 * nc8) {
+                // split-FP16 analysis: the staged operands (kNTB targets x
+                // sums, differences x hi, lo x chunks x 16 columns of 16 B),
+                // later the transposed results
+                const size_t staged = static_cast<size_t>(kNTB) * 4 * nc8 * 16 * 16;
+                const size_t results = static_cast<size_t>(d.Ja) * kNTB * 17 * sizeof(float);
                 const size_t lds = std::max(staged, results);
                 using Fn16 = void (*)(const cx<float>*, const ColumnDev*, int, SynthMats16, int, int, int, int, int,
                                       int, cx<float>*);
                 const int key = (ana16N32 - 1) * 2 + ana16Tail;
-                const Fn16 fn16 = NT == 4 ? (key == 0 ? k_proj16<4, 1, 0> : key == 1 ? k_proj16<4, 1, 1>
-                                             : key == 2 ? k_proj16<4, 2, 0> : k_proj16<4, 2, 1>)
-                                          : (key == 0 ? k_proj16<3, 1, 0> : key == 1 ? k_proj16<3, 1, 1>
-                                             : key == 2 ? k_proj16<3, 2, 0> : k_proj16<3, 2, 1>);
+                const Fn16 fn16 = NT == 4 ? (key == 0 ? k_proj16<4, kNTB, 1, 0> : key == 1 ? k_proj16<4, kNTB, 1, 1>
+                                             : key == 2 ? k_proj16<4, kNTB, 2, 0> : k_proj16<4, kNTB, 2, 1>)
+                                          : (key == 0 ? k_proj16<3, kNTB, 1, 0> : key == 1 ? k_proj16<3, kNTB, 1, 1>
+                                             : key == 2 ? k_proj16<3, kNTB, 2, 0> : k_proj16<3, kNTB, 2, 1>);
                 SynthMats16 a16{};
                 for (int i = 0; i < 4; ++i) {
                     a16.m[i] = ana16[i].as<_Float16>();
                     a16.inv[i] = ana16Inv[i];
                 }
                 timed("proj", bytes, s, [&] {
-                    hipLaunchKernelGGL(fn16, dim3(static_cast<unsigned>(ncg)), dim3(static_cast<unsigned>(64 * njt)), lds,
+                    hipLaunchKernelGGL(fn16, dim3(static_cast<unsigned>(ncg * ((NT + kNTB - 1) / kNTB))),
+                                       dim3(static_cast<unsigned>(64 * njt)), lds,
                                        s, Q.as<cx<float>>(), cols.as<ColumnDev>(), static_cast<int>(ncol), a16,
                                        static_cast<int>(d.Ja), static_cast<int>(d.c.Nz), static_cast<int>(d.c.Nj),
                                        static_cast<int>(d.c.Ny), static_cast<int>(d.nkxP), static_cast<int>(d.M),
@@ -3049,26 +3217,31 @@ struct KernelImpl {
         });
     }
 
-    // Projection + flux store or RK stage (+ the next flux input's spectra).
+    // HBM bytes of the stage epilogue's coefficient arrays: flux out 48 B per
+    // coefficient; RK stage 1 reads y, writes w (and the stage unless the
+    // next input's spectra are formed: 96 or 144 B); stages 2, 3 read w, y
+    // and write w (and the stage: 144 or 192 B); stage 4 reads w, y and
+    // writes y (144 B); with the next input's spectra four spectra (y and
+    // the stage state are FP64 triples, 48 B; the weighted sum is a triple
+    // in T, 3 sizeof(cx<T>))
     template <typename T>
-    void launchStage(double elapsed, const Epilogue& ep, hipStream_t s) {
-        const int M = static_cast<int>(d.M);
-        const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
-        // per coefficient: the NT analysed values and the epilogue's
-        // coefficient arrays: flux out 48 B; RK stage 1 reads y, writes w
-        // (and the stage unless the next input's spectra are formed: 96 or
-        // 144 B); stages 2, 3 read w, y and write w (and the stage: 144 or
-        // 192 B); stage 4 reads w, y and writes y (144 B); with the next
-        // input's spectra four spectra
-        // (y and the stage state are FP64 triples, 48 B; the weighted sum is
-        // a triple in T, 3 sizeof(cx<T>))
+    double stageEpilogueBytes(const Epilogue& ep) const {
         const double stBytes = ep.emitFields ? 0.0 : 48.0, wB = 3.0 * sizeof(cx<T>);
         const double epBytes = ep.mode == kFluxOut ? 48.0
                                : ep.mode == kRk1   ? 48.0 + wB + stBytes
                                : ep.mode == kRk4   ? 96.0 + wB
                                                    : 48.0 + 2.0 * wB + stBytes;
         const double fieldBytes = (ep.mode != kFluxOut && ep.emitFields) ? 4.0 * sizeof(cx<T>) : 0.0;
-        const double bytes = static_cast<double>(M) * (NT * sizeof(cx<T>) + epBytes + fieldBytes);
+        return static_cast<double>(d.M) * (epBytes + fieldBytes);
+    }
+
+    // Projection + flux store or RK stage (+ the next flux input's spectra).
+    template <typename T>
+    void launchStage(double elapsed, const Epilogue& ep, hipStream_t s) {
+        const int M = static_cast<int>(d.M);
+        const double hscale = 1.0 / static_cast<double>(d.c.Nx * d.c.Ny);
+        // per coefficient: the NT analysed values and the epilogue's arrays
+        const double bytes = static_cast<double>(M) * NT * sizeof(cx<T>) + stageEpilogueBytes<T>(ep);
         // the kernel reads y and w in every mode
         Epilogue e = ep;
         if (e.mode == kFluxOut) {
