@@ -1614,28 +1614,22 @@ __global__ __launch_bounds__(1024) void k_proj(const cx<T>* __restrict__ Q, cons
 // differences, column) from their largest entry.  The contraction covers
 // every folded row z <= L/2 (the midpoint row included).  Same waves (one
 // per 16-row tile of j, even j first) and the same transposed stores of Tg.
-template <int NT, int NTB, int N32, int TAIL>
-__global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Qall, const ColumnDev* __restrict__ cols,
-                                                 int ncol, SynthMats16 ana, int Ja, int Nz, int Nj, int Ny, int nkxP,
-                                                 int M, cx<float>* __restrict__ TgAll) {
+// Targets T0 ... T0 + NTB - 1 of the NT (fixed at compile time, so that
+// every MFMA of the body is unconditional) for column group cg.
+template <int NT, int NTB, int T0, int N32, int TAIL>
+__device__ __forceinline__ void proj16Body(const cx<float>* __restrict__ Qall, const ColumnDev* __restrict__ cols,
+                                           int ncol, const SynthMats16& ana, int Ja, int Nz, int Nj, int Ny, int nkxP,
+                                           int M, cx<float>* __restrict__ TgAll, int cg, unsigned char* smem,
+                                           ColumnDev* scol, unsigned* colMax, float* colInv) {
     using MF = Mfma<float>;
     constexpr int NC8 = 4 * N32 + 2 * TAIL;  // 8-row chunks of the contraction
     constexpr int kTileHalves = N32 * 1024 + TAIL * 512;
     constexpr int kItems = NTB * 16 * NC8;   // staging items (target, chunk, column, part)
-    // workgroup -> (column group cg, targets t0 ... t0 + NTB - 1 of the
-    // NT): one XCD runs consecutive column groups of one target set
-    const int ncg = (ncol + 7) / 8;
-    const int logical = xcdBlock(blockIdx.x, gridDim.x);
-    const int cg = logical % ncg;
-    const int t0 = (logical / ncg) * NTB;
+    constexpr int t0 = T0;
     const size_t tStrideAll = static_cast<size_t>(Nz) * Ny * nkxP;
     const cx<float>* __restrict__ Q = Qall + t0 * tStrideAll;
     cx<float>* __restrict__ Tg = TgAll + t0 * static_cast<size_t>(M);
-    extern __shared__ __align__(16) unsigned char smem[];
     half8* sb = reinterpret_cast<half8*>(smem);  // [target][sums, diffs][hl][chunk][column c x part]
-    __shared__ ColumnDev scol[8];
-    __shared__ unsigned colMax[NTB * 16];  // per (target, sums or diffs, column): largest |entry| as float bits
-    __shared__ float colInv[NTB * 16];
     const int lane = threadIdx.x & 63;
     const int jt = waveIndex();
     if (threadIdx.x < 8) {
@@ -1715,9 +1709,10 @@ __global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q
     const int jh = jt - P * nEven;      // row tile within the parity
     // family of each of this workgroup's targets (u, v: cosine; w, eta:
     // sine); -1 past the last target
+    auto tfamOf = [](int t) { return T0 + t < NT ? (projTarget<NT>(T0 + t) >= 2 ? 1 : 0) : -1; };
     int tfam[NTB];
 #pragma unroll
-    for (int t = 0; t < NTB; ++t) tfam[t] = t0 + t < NT ? (projTarget<NT>(t0 + t) >= 2 ? 1 : 0) : -1;
+    for (int t = 0; t < NTB; ++t) tfam[t] = tfamOf(t);
     typename MF::acc acc[NTB];
 #pragma unroll
     for (int t = 0; t < NTB; ++t) acc[t] = MF::zero();
@@ -1797,6 +1792,29 @@ __global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Q
     }
 }
 
+// Split-FP16 analysis: workgroup -> (column group cg, targets t0 ... t0 +
+// NTB - 1 of the NT); one XCD runs consecutive column groups of one target
+// set.
+template <int NT, int NTB, int N32, int TAIL>
+__global__ __launch_bounds__(1024) void k_proj16(const cx<float>* __restrict__ Qall, const ColumnDev* __restrict__ cols,
+                                                 int ncol, SynthMats16 ana, int Ja, int Nz, int Nj, int Ny, int nkxP,
+                                                 int M, cx<float>* __restrict__ TgAll) {
+    static_assert(NT <= 2 * NTB, "at most two target sets");
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ ColumnDev scol[8];
+    __shared__ unsigned colMax[NTB * 16];  // per (target, sums or diffs, column): largest |entry| as float bits
+    __shared__ float colInv[NTB * 16];
+    const int ncg = (ncol + 7) / 8;
+    const int logical = xcdBlock(blockIdx.x, gridDim.x);
+    const int cg = logical % ncg;
+    if (logical < ncg)
+        proj16Body<NT, NTB, 0, N32, TAIL>(Qall, cols, ncol, ana, Ja, Nz, Nj, Ny, nkxP, M, TgAll, cg, smem, scol,
+                                          colMax, colInv);
+    else if constexpr (NT > NTB)
+        proj16Body<NT, NTB, NTB, N32, TAIL>(Qall, cols, ncol, ana, Ja, Nz, Nj, Ny, nkxP, M, TgAll, cg, smem, scol,
+                                            colMax, colInv);
+}
+
 // Per-mode wavenumbers of the projection.
 struct ModeDev {
     double k, l, Kh, cosA, sinA;
@@ -1815,6 +1833,23 @@ struct CoefConsts {
     const double* gws;  // G wave scale
     const double* apmW;
     const double* inertialScale;
+    // per-j FP64 factors of the mixed-precision projection tables
+    // (coefficientsF): [kMixedJ][NjMixed]
+    const double* mixedJ;
+    int NjMixed;
+};
+// rows of CoefConsts::mixedJ
+enum MixedJ : int {
+    kMjGwgOverGg = 0,  // (Gg / Gw) / Gg
+    kMjDeltaScale,     // (h0 Gg / Gw) / Fg
+    kMjLr2inv,         // f^2 / (g h0), 0 at j = 0
+    kMjA0VortGeo,      // 1 / Fg
+    kMjA0BuoyGeo,      // (-f / h0) / Gg, 0 at j = 0
+    kMjApmDScaled,     // (m_j / 2) prefactor
+    kMjA0VortMda,      // (f^2 / (2 h0)) / Fg
+    kMjA0BuoyMda,      // 1 / Gg
+    kMjNA0Num,         // -(f / g) Gg, 0 at j = 0
+    kMixedJ
 };
 
 // The per-coefficient tables of coefficient (j, mode) -- the descriptor's
@@ -1880,71 +1915,71 @@ __device__ __forceinline__ CoefRow coefficients(const CoefConsts& c, int j, cons
     return r;
 }
 
-// The mixed-precision form of the same tables: every quotient and product in
-// FP32 (fast quotients), except the frequency, which the phase needs to FP64
-// accuracy (the angle omega t reaches hundreds of radians over a run).  The
-// spectra and the projected fluxes they feed carry FP32 accuracy anyway.
+// The mixed-precision form of the same tables.  The projection tables
+// (A0Vort, A0Buoy, NA0, ApmN, ApmD, ApmDScaled) stay FP64: the wave-vortex
+// projection of a nearly balanced flow sums terms that cancel to a small
+// remainder, and FP32 tables would bias that remainder the same way on every
+// step.  Their per-j factors come from host tables (CoefConsts::mixedJ), so
+// a coefficient needs four FP64 quotients and one square root.  The
+// synthesis tables of the field spectra (UAp, VAp, WAp, NAp, UA0, VA0, NA0)
+// are FP32 (fast quotients): they only perturb the flux's input at FP32
+// rounding, below the FP32 transforms' own error.  The frequency is FP64:
+// the phase needs it to FP64 accuracy (the angle omega t reaches hundreds of
+// radians over a run).
 struct CoefRowF {
-    double om;
-    float NAp, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled;
-    cx<float> UAp, VAp, WAp, UA0, VA0, ApmD;
+    double om, NA0, A0Vort, A0Buoy, ApmN, ApmDScaled;
+    cx<double> ApmD;
+    float NApF, NA0F;
+    cx<float> UAp, VAp, WAp, UA0, VA0;
 };
 __device__ __forceinline__ CoefRowF coefficientsF(const CoefConsts& c, int j, const ModeDev& md) {
     // a / b as a times the hardware reciprocal of b (about 2 ulp)
     auto dv = [](float a, float b) { return a * __builtin_amdgcn_rcpf(b); };
+    const int Nj = c.NjMixed;
+    auto pj = [&](int row) { return c.mixedJ[row * Nj + j]; };
     CoefRowF r{};
-    const double Khd = md.Kh, Kh2d = Khd * Khd, Mjd = c.m[j];
-    const bool wave = Khd > 0.0 && j > 0, inertial = Khd == 0.0, geostrophic = Khd > 0.0, mda = Khd == 0.0 && j > 0;
-    double hpmd = 1.0;
+    const double Kh = md.Kh, Kh2 = Kh * Kh, Mj = c.m[j];
+    const bool wave = Kh > 0.0 && j > 0, inertial = Kh == 0.0, geostrophic = Kh > 0.0, mda = Kh == 0.0 && j > 0;
+    double hpm = 1.0;
     if (j > 0)
-        hpmd = c.stratified ? c.h0[j]
-               : c.hydro    ? c.N02 / (c.g * Mjd * Mjd)
-                            : (c.N02 - c.f2) / (c.g * (Mjd * Mjd + Kh2d));
-    const double omd = sqrt(c.g * hpmd * Kh2d + c.f2);
-    r.om = omd;
-    const float Kh = static_cast<float>(Khd), hpm = static_cast<float>(hpmd), om = static_cast<float>(omd);
-    const float Mj = static_cast<float>(Mjd), h0 = static_cast<float>(c.h0[j]);
-    const float Fg = static_cast<float>(c.Fg[j]), Gg = static_cast<float>(c.Gg[j]);
-    const float f = static_cast<float>(c.f), f2 = static_cast<float>(c.f2), g = static_cast<float>(c.g);
+        hpm = c.stratified ? c.h0[j] : c.hydro ? c.N02 / (c.g * Mj * Mj) : (c.N02 - c.f2) / (c.g * (Mj * Mj + Kh2));
+    const double om = sqrt(c.g * hpm * Kh2 + c.f2);
+    r.om = om;
+    r.ApmDScaled = pj(kMjApmDScaled);
+    // FP32 synthesis tables: fs = Fg / (Fg / Fw) = Fw
+    const float Khf = static_cast<float>(Kh), hpmf = static_cast<float>(hpm), omf = static_cast<float>(om);
+    const float Fg = static_cast<float>(c.Fg[j]), f = static_cast<float>(c.f);
     const float sign = j % 2 == 0 ? 1.0f : -1.0f;
-    float Fw = Fg, Gw = Gg;
-    if (!c.hydro) {
-        Fw = j == 0 ? 2.0f : sign * hpm * Mj * static_cast<float>(c.S);
-        Gw = j == 0 ? 1.0f : sign * static_cast<float>(c.S);
-    }
-    const float Fwg = dv(Fg, Fw), Gwg = dv(Gg, Gw);
-    const float fs = dv(Fg, Fwg);
-    const float prefactor = c.stratified ? 0.0f : sign * static_cast<float>(c.P0);
-    r.ApmDScaled = 0.5f * Mj * prefactor;
+    const float fs = c.hydro ? Fg : j == 0 ? 2.0f : sign * hpmf * static_cast<float>(Mj) * static_cast<float>(c.S);
     const float cosA = static_cast<float>(md.cosA), sinA = static_cast<float>(md.sinA);
     if (wave) {
         const float gs = static_cast<float>(c.gws[j]);
-        const float fo = dv(f, om);
+        const float fo = dv(f, omf);
         r.UAp = {cosA * fs, -fo * sinA * fs};
         r.VAp = {sinA * fs, fo * cosA * fs};
-        r.WAp = {0.0f, -Kh * hpm * gs};
-        r.NAp = dv(-Kh * hpm, om) * gs;
-        const float deltaScale = dv(h0 * Gwg, Fg);
-        const float twoKh = 2.0f * Kh * hpm;
-        r.ApmD = {0.0f, dv(-1.0f, twoKh) * deltaScale};
-        r.ApmN = dv(dv(-om, twoKh) * Gwg, Gg);
+        r.WAp = {0.0f, -Khf * hpmf * gs};
+        r.NApF = dv(-Khf * hpmf, omf) * gs;
+        const double twoKh = 2.0 * Kh * hpm;
+        r.ApmD = {0.0, -pj(kMjDeltaScale) / twoKh};
+        r.ApmN = (-om / twoKh) * pj(kMjGwgOverGg);
     } else if (inertial) {
         r.UAp = {fs, 0.0f};
         r.VAp = {0.0f, fs};
     }
     if (geostrophic) {
-        const float lr2inv = j == 0 ? 0.0f : dv(f2, g * h0);
-        const float den = Kh * Kh + lr2inv;
-        r.UA0 = {0.0f, dv(static_cast<float>(md.l), den) * Fg};
-        r.VA0 = {0.0f, dv(-static_cast<float>(md.k), den) * Fg};
-        r.NA0 = j == 0 ? 0.0f : dv(-dv(f, g), den) * Gg;
-        r.A0Vort = dv(1.0f, Fg);
-        r.A0Buoy = j == 0 ? 0.0f : dv(dv(-f, h0), Gg);
+        const double den = Kh2 + pj(kMjLr2inv);
+        const float rden = __builtin_amdgcn_rcpf(static_cast<float>(den));
+        r.UA0 = {0.0f, static_cast<float>(md.l) * rden * Fg};
+        r.VA0 = {0.0f, -static_cast<float>(md.k) * rden * Fg};
+        r.NA0 = pj(kMjNA0Num) / den;
+        r.A0Vort = pj(kMjA0VortGeo);
+        r.A0Buoy = pj(kMjA0BuoyGeo);
     } else if (mda) {
-        r.NA0 = Gg;
-        r.A0Buoy = dv(1.0f, Gg);
-        r.A0Vort = dv(f2, 2.0f * h0 * Fg);
+        r.NA0 = c.Gg[j];
+        r.A0Buoy = pj(kMjA0BuoyMda);
+        r.A0Vort = pj(kMjA0VortMda);
     }
+    r.NA0F = static_cast<float>(r.NA0);
     return r;
 }
 
@@ -1969,7 +2004,7 @@ __device__ __forceinline__ void storeFieldSpectraF(const CoefRowF& cr, int idx, 
     const cx<float> u = cadd(cadd(cmul(cr.UAp, ap), cmul(cconj(cr.UAp), am)), cmul(cr.UA0, a0));
     const cx<float> v = cadd(cadd(cmul(cr.VAp, ap), cmul(cconj(cr.VAp), am)), cmul(cr.VA0, a0));
     const cx<float> w = cadd(cmul(cr.WAp, ap), cmul(cr.WAp, am));
-    const cx<float> e = cadd(cadd(cscale(ap, cr.NAp), cscale(am, -cr.NAp)), cscale(a0, cr.NA0));
+    const cx<float> e = cadd(cadd(cscale(ap, cr.NApF), cscale(am, -cr.NApF)), cscale(a0, cr.NA0F));
     FV[idx] = u;
     FV[M + idx] = v;
     FV[2 * M + idx] = w;
@@ -2089,10 +2124,10 @@ __device__ __forceinline__ void projectTarget(int target, cx<R> value, const Row
 // ...Kernel.cpp) of coefficient idx = j + Nj wv and the flux store or RK4
 // stage of WVFixedStepRK4::step; in RK modes with emitFields set it also
 // forms the field spectra of the state it produces (the input of the next
-// flux evaluation).  FP64 (T = double): tables, phase and projection in
-// FP64, the tables bit for bit the host's; mixed (T = float): the FP32
-// tables of coefficientsF, the FP32 phase of phaseF and the projection in
-// FP32.  The stage arithmetic on the state is FP64 in both.
+// flux evaluation).  FP64 (T = double): tables and phase in FP64, the
+// tables bit for bit the host's; mixed (T = float): the tables of
+// coefficientsF and the FP32 phase of phaseF.  The projection and the stage
+// arithmetic on the state are FP64 in both.
 template <typename T, int NT>
 __device__ __forceinline__ void stageCoefficient(const cx<T> (&tv)[NT], int idx, int j, int wv,
                                                  const ModeDev* __restrict__ modes, int M, const CoefConsts& cc,
@@ -2111,14 +2146,17 @@ __device__ __forceinline__ void stageCoefficient(const cx<T> (&tv)[NT], int idx,
         cr = coefficients(cc, j, md);
         p = phaseAt(cr.om, elapsed);
     }
-    cx<T> Fp{T(0), T(0)}, Fm{T(0), T(0)}, F0{T(0), T(0)};
-    const T apmW = static_cast<T>(cc.apmW[j]), iscale = static_cast<T>(cc.inertialScale[j]);
+    // the projection in FP64 in both precisions (its terms cancel to a
+    // small remainder for a nearly balanced flow)
+    auto d = [](cx<T> a) { return cx<double>{static_cast<double>(a.re), static_cast<double>(a.im)}; };
+    const cx<double> pd = d(p);
+    cx<double> Fp{0.0, 0.0}, Fm{0.0, 0.0}, F0{0.0, 0.0};
+    const double apmW = cc.apmW[j], iscale = cc.inertialScale[j];
 #pragma unroll
     for (int t = 0; t < NT; ++t)
-        projectTarget<T, NT>(projTarget<NT>(t), cscale(tv[t], static_cast<T>(hscale)), cr, md, apmW, iscale, inertial,
-                             p, Fp, Fm, F0);
-    auto d = [](cx<T> a) { return cx<double>{static_cast<double>(a.re), static_cast<double>(a.im)}; };
-    const cx<double> k3[3] = {d(Fp), d(Fm), d(F0)};
+        projectTarget<double, NT>(projTarget<NT>(t), cscale(d(tv[t]), hscale), cr, md, apmW, iscale, inertial, pd, Fp,
+                                  Fm, F0);
+    const cx<double> k3[3] = {Fp, Fm, F0};
     cx<double> next[3];
     rkStageUpdate<T>(ep, idx, k3, next);
     if (ep.mode != kFluxOut && ep.emitFields) {
@@ -2413,7 +2451,7 @@ struct KernelImpl {
     int ana16N32 = 0, ana16Tail = 0;
     bool fold = false;
     size_t Zh = 0;  // folded z rows (multiple of 16)
-    DeviceBuffer FV, H, G, P, Q, Tg, modeTab, jTab, stage, weighted, partial, scalar;
+    DeviceBuffer FV, H, G, P, Q, Tg, modeTab, jTab, mixedJTab, stage, weighted, partial, scalar;
     CoefConsts coefConsts{};
     DeviceBuffer rowLimit;  // per y-FFT kx tile: largest retained |ky|
     DeviceBuffer lrow;      // per storage row iy: its y wavenumber (rowSign * l of its columns)
@@ -2468,7 +2506,7 @@ struct KernelImpl {
     size_t descriptorBytes() const {
         return sumBytes({&rowLimit, &lrow, &cols, &omega, &UAp, &VAp, &WAp, &NAp, &UA0, &VA0, &NA0, &A0Vort, &A0Buoy,
                          &ApmN, &ApmDScaled, &ApmD, &ApmTE, &A0TE, &apmW, &inertialScale, &fWaveScale, &gWaveScale,
-                         &Fg, &Gg, &modeTab, &jTab, &mainColumn, &compColumn, &scKeep});
+                         &Fg, &Gg, &modeTab, &jTab, &mixedJTab, &mainColumn, &compColumn, &scKeep});
     }
     // transform operators and twiddle tables (the "plans")
     size_t planBytes() const {
@@ -2851,9 +2889,31 @@ struct KernelImpl {
             }
             upload(jTab, jt);
             const double* jp = jTab.as<double>();
+            // per-j factors of the mixed-precision projection tables
+            // (the descriptor's formulas, WVKernelTypes.cpp:214-284, with
+            // everything that depends on j alone formed here)
+            std::vector<double> mj(static_cast<size_t>(kMixedJ) * Nj, 0.0);
+            for (size_t j = 0; j < Nj; ++j) {
+                const double h0 = d.h0[j], Fg = d.Fg[j], Gg = d.Gg[j], sign = j % 2 == 0 ? 1.0 : -1.0;
+                const double Gw = cfg.isHydrostatic ? Gg : j == 0 ? 1.0 : sign * d.waveNorm;
+                const double Gwg = Gg / Gw;
+                const double prefactor = d.stratified ? 0.0 : sign * d.projPrefactor;
+                auto at = [&](int row) -> double& { return mj[static_cast<size_t>(row) * Nj + j]; };
+                at(kMjGwgOverGg) = Gwg / Gg;
+                at(kMjDeltaScale) = h0 * Gwg / Fg;
+                at(kMjLr2inv) = j == 0 ? 0.0 : d.f2 / (cfg.g * h0);
+                at(kMjA0VortGeo) = 1.0 / Fg;
+                at(kMjA0BuoyGeo) = j == 0 ? 0.0 : (-d.f / h0) / Gg;
+                at(kMjApmDScaled) = d.m[j] / 2.0 * prefactor;
+                at(kMjA0VortMda) = j == 0 ? 0.0 : d.f2 / (2.0 * h0) / Fg;
+                at(kMjA0BuoyMda) = 1.0 / Gg;
+                at(kMjNA0Num) = j == 0 ? 0.0 : -(d.f / cfg.g) * Gg;
+            }
+            upload(mixedJTab, mj);
             coefConsts = CoefConsts{d.f, d.f2, d.N02, cfg.g, cfg.Lz, d.waveNorm, d.projPrefactor,
                                     cfg.isHydrostatic ? 1 : 0, d.stratified ? 1 : 0,
-                                    jp, jp + Nj, jp + 2 * Nj, jp + 3 * Nj, jp + 4 * Nj, jp + 5 * Nj, jp + 6 * Nj};
+                                    jp, jp + Nj, jp + 2 * Nj, jp + 3 * Nj, jp + 4 * Nj, jp + 5 * Nj, jp + 6 * Nj,
+                                    mixedJTab.as<double>(), static_cast<int>(Nj)};
         }
         hipCheck(hipMemset(H.p, 0, H.bytes), "hipMemset");
         hipCheck(hipMemset(P.p, 0, P.bytes), "hipMemset");
