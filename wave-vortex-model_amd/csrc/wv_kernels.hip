@@ -892,8 +892,10 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
     const size_t plane = (static_cast<size_t>(ch) * Nz + blockIdx.y) * planeStride;
     const int nel = Ny * KT;
     loadTwiddles(tw, twY, Ny);
-    // all of a thread's loads are issued before its LDS stores
-    constexpr int kMaxPerThread = 16;  // nel <= 16 * blockDim (checked at setup)
+    // all of a thread's loads are issued before its LDS stores; entries per
+    // thread: nel / blockDim for a compiled length, else at most 16 (nel <=
+    // 16 blockDim, checked at setup)
+    constexpr int kMaxPerThread = L2N > 0 ? ((1 << (L2N + L2KT)) + kFftThreads - 1) / kFftThreads : 16;
     cx<T> v[kMaxPerThread];
 #pragma unroll
     for (int i = 0; i < kMaxPerThread; ++i) {
@@ -2771,7 +2773,14 @@ struct KernelImpl {
         precision = prec;
         device = dev;
         hipCheck(hipSetDevice(dev), "hipSetDevice");
+        // y-transform tile: KT columns of Ny, at most 4096 points; for FP64
+        // at most 2048 while that keeps 8 columns (128-byte rows): Ny = 256
+        // then runs 8 columns in a 32 KB tile, 8 workgroups per CU instead
+        // of 4 (FP32 keeps 16 columns, which measured faster than 8; FP64 at
+        // Ny = 512 keeps 8, 4 measured slower)
         KT = static_cast<int>(std::max<size_t>(1, std::min<size_t>(16, 4096 / std::max<size_t>(cfg.Ny, 1))));
+        if (prec == WV_PRECISION_FP64 && KT > 8 && static_cast<size_t>(KT) * cfg.Ny > 2048)
+            KT = static_cast<int>(std::max<size_t>(8, 2048 / cfg.Ny));
         while (KT & (KT - 1)) --KT;  // power of two
         std::string err;
         try {
@@ -3117,6 +3126,7 @@ struct KernelImpl {
         if (l2y == 6 && l2t == 4) fn = dv ? k_yfft<T, 6, 4, true> : k_yfft<T, 6, 4, false>;
         if (l2y == 7 && l2t == 4) fn = dv ? k_yfft<T, 7, 4, true> : k_yfft<T, 7, 4, false>;
         if (l2y == 8 && l2t == 4) fn = dv ? k_yfft<T, 8, 4, true> : k_yfft<T, 8, 4, false>;
+        if (l2y == 8 && l2t == 3) fn = dv ? k_yfft<T, 8, 3, true> : k_yfft<T, 8, 3, false>;
         if (l2y == 9 && l2t == 3) fn = dv ? k_yfft<T, 9, 3, true> : k_yfft<T, 9, 3, false>;
         if (l2y == 10 && l2t == 2) fn = dv ? k_yfft<T, 10, 2, true> : k_yfft<T, 10, 2, false>;
         const YDerivPlan plan = yp ? *yp : YDerivPlan{};
@@ -3477,6 +3487,7 @@ struct KernelImpl {
         if (l2y == 6 && l2t == 4) yfn = k_yfft<double, 6, 4, false>;
         if (l2y == 7 && l2t == 4) yfn = k_yfft<double, 7, 4, false>;
         if (l2y == 8 && l2t == 4) yfn = k_yfft<double, 8, 4, false>;
+        if (l2y == 8 && l2t == 3) yfn = k_yfft<double, 8, 3, false>;
         if (l2y == 9 && l2t == 3) yfn = k_yfft<double, 9, 3, false>;
         if (l2y == 10 && l2t == 2) yfn = k_yfft<double, 10, 2, false>;
         auto yfft = [&](int nch, int sign) {  // in place: a workgroup loads its whole tile before it stores
