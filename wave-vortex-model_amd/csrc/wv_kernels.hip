@@ -1096,19 +1096,22 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
             const size_t rowOff = static_cast<size_t>(row) * nkxP;
             const cx<T>* A = live ? G + sa * chStride + rowOff : zeros;
             const cx<T>* B = live && sb >= 0 ? G + sb * chStride + rowOff : zeros;
+            // (32-bit byte offsets from the uniform bases: no 64-bit
+            // address arithmetic per lane)
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = kc + lane + 64 * h;
-                const int kcl = k < nkx ? k : nkx - 1;
-                va[i][h] = A[kcl];
-                vb[i][h] = B[kcl];
+                const unsigned off = static_cast<unsigned>(k < nkx ? k : nkx - 1) * sizeof(cx<T>);
+                va[i][h] = *reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(A) + off);
+                vb[i][h] = *reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(B) + off);
             }
         }
         T kk[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int k = kc + lane + 64 * h;
-            kk[h] = kxv[k < nkx ? k : nkx - 1];
+            const unsigned off = static_cast<unsigned>(k < nkx ? k : nkx - 1) * sizeof(T);
+            kk[h] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(kxv) + off);
         }
         __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
 #pragma unroll
@@ -1193,18 +1196,24 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
         fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
     // wave w unpacks (row, pair) rq = w, w + nwaves, ...; its lanes walk k
     constexpr int nwaves = kFftThreads / 64;
-    for (int rq = waveIndex(); rq < RB * 2; rq += nwaves)
-    for (int k = threadIdx.x & 63; k < nkx; k += 64) {
+    for (int rq = waveIndex(); rq < RB * 2; rq += nwaves) {
         const int q = rq & 1, rr = rq >> 1;
         const int row = row0 + rr;
         if (row >= nrows) continue;
+        // uniform bases of the row's two target rows
+        cx<T>* const Pa = P + (2 * q) * chStride + static_cast<size_t>(row) * nkxP;
+        cx<T>* const Pb = P + (2 * q + 1) * chStride + static_cast<size_t>(row) * nkxP;
         const cx<T>* rb = buf + rr * RS + q * NxP;
-        const cx<T> Za = rb[ldsIndex(k, 1)];
-        const cx<T> Zb = rb[ldsIndex(k == 0 ? 0 : Nx - k, 1)];
         const T h = T(0.5), hb = q == 1 ? h / etaScale : h;
-        const size_t o = static_cast<size_t>(row) * nkxP + k;
-        if (2 * q < XM::targets) P[(2 * q) * chStride + o] = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
-        if (2 * q + 1 < XM::targets) P[(2 * q + 1) * chStride + o] = {(Za.im + Zb.im) * hb, (Zb.re - Za.re) * hb};
+        for (int k = threadIdx.x & 63; k < nkx; k += 64) {
+            const cx<T> Za = rb[ldsIndex(k, 1)];
+            const cx<T> Zb = rb[ldsIndex(k == 0 ? 0 : Nx - k, 1)];
+            const unsigned off = static_cast<unsigned>(k) * sizeof(cx<T>);
+            if (2 * q < XM::targets)
+                *reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Pa) + off) = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
+            if (2 * q + 1 < XM::targets)
+                *reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Pb) + off) = {(Za.im + Zb.im) * hb, (Zb.re - Za.re) * hb};
+        }
     }
 }
 
