@@ -186,3 +186,41 @@ def test_fg_all_derivatives_match_reference_golden(torch_gpu, path):
     Apm, A0 = _dev(k, (g["Ap"], g["A0"]))
     assert rel_err(k.transform_to_spatial_domain_with_all_derivatives(True, Apm, A0).cpu().numpy(), g["fAll"]) <= 1e-13
     assert rel_err(k.transform_to_spatial_domain_with_all_derivatives(False, Apm, A0).cpu().numpy(), g["gAll"]) <= 1e-13
+
+
+# Horizontal lengths that are not powers of two: the in-LDS FFTs run
+# mixed-radix passes (16, 8, 4, 2, then 3, 5, 7, 11, 13) for lengths whose
+# prime factors are at most 13, and a direct DFT for the rest (17 x 19).
+ODD_SHAPES = [(96, 80, 17, False), (45, 36, 9, False), (26, 22, 9, True), (48, 39, 9, False), (17, 19, 9, False)]
+
+
+@pytest.mark.parametrize("Nx,Ny,Nz,hydro", ODD_SHAPES, ids=[f"{a}x{b}x{c}{'h' if h else ''}" for a, b, c, h in ODD_SHAPES])
+def test_flux_and_transforms_fp64_non_power_of_two_match_oracle(torch_gpu, Nx, Ny, Nz, hydro):
+    import wvamd
+    from oracle import Oracle
+    cfg = wvamd.make_config(Nx, Ny, Nz, None, Lx=60e3, Ly=50e3, isHydrostatic=hydro)
+    k = _kernel(cfg)
+    o = Oracle(cfg)
+    rng = np.random.default_rng(Nx * 1000 + Ny)
+    A = [1e-3 * (rng.standard_normal(k.spectral_shape) + 1j * rng.standard_normal(k.spectral_shape)) for _ in range(3)]
+    F = _host(k.nonlinear_flux(*_dev(k, A), t=450.0, t0=0.0))
+    err = rel_err_joint(F, o.nonlinear_flux(*A, t=450.0, t0=0.0))
+    fields = k.transform_wave_vortex_to_uvweta(*_dev(k, A), t=450.0).cpu().numpy()
+    ref = o.inverse_uvweta(*A, t=450.0)
+    ferr = float(np.abs(fields - ref).max() / np.abs(ref).max())
+    print(f"[gpu] {Nx}x{Ny}x{Nz} hydro={hydro} fp64 flux rel err {err:.2e}, inverse transform {ferr:.2e}", flush=True)
+    assert err <= FP64_TOL and ferr <= 1e-13
+
+
+def test_flux_mixed_non_power_of_two_matches_fp64(torch_gpu):
+    import wvamd
+    cfg = wvamd.make_config(96, 80, 17, None, Lx=60e3, Ly=50e3)
+    k64, k32 = _kernel(cfg), _kernel(cfg, "mixed-split16")
+    rng = np.random.default_rng(96)
+    A = [1e-3 * (rng.standard_normal(k64.spectral_shape) + 1j * rng.standard_normal(k64.spectral_shape))
+         for _ in range(3)]
+    F64 = _host(k64.nonlinear_flux(*_dev(k64, A), t=450.0, t0=0.0))
+    F32 = _host(k32.nonlinear_flux(*_dev(k32, A), t=450.0, t0=0.0))
+    err = rel_err_joint(F32, F64)
+    print(f"[gpu] 96x80x17 mixed-split16 flux rel err {err:.2e}", flush=True)
+    assert err <= MIXED_TOL

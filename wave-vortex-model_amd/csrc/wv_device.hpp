@@ -92,7 +92,9 @@ struct Mfma<double> {
 // tw[q] = exp(-2*pi*i*q/N).
 // Power-of-two N runs Stockham autosort passes of radix 16 (then 8, 4 or 2
 // for the remaining factor), each butterfly an in-register DFT, with all index
-// arithmetic in shifts and masks; other N use a direct DFT.  A pass holds at
+// arithmetic in shifts and masks; other N with prime factors up to 13 run
+// mixed-radix passes (powers of two, then 3, 5, 7, 11, 13) with runtime
+// index arithmetic; the rest a direct DFT.  A pass holds at
 // most 16 complex values per thread, so the caller must launch at least
 // nb*N/16 threads.  Every workgroup that runs these transforms has
 // kFftThreads threads.  The caller synchronizes before the call; the function
@@ -289,6 +291,167 @@ __device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restri
     __syncthreads();
 }
 
+// cos(2 pi m / P) and sin(2 pi m / P), m = 0 ... (P - 1)/2, for the odd
+// primes of the mixed-radix passes (longer tables follow by symmetry).
+template <int P>
+struct PrimeRoots;
+template <>
+struct PrimeRoots<3> {
+    static constexpr double c[2] = {1.0, -0.5};
+    static constexpr double s[2] = {0.0, 0.8660254037844386};
+};
+template <>
+struct PrimeRoots<5> {
+    static constexpr double c[3] = {1.0, 0.30901699437494745, -0.8090169943749475};
+    static constexpr double s[3] = {0.0, 0.9510565162951535, 0.5877852522924731};
+};
+template <>
+struct PrimeRoots<7> {
+    static constexpr double c[4] = {1.0, 0.6234898018587335, -0.2225209339563144, -0.9009688679024191};
+    static constexpr double s[4] = {0.0, 0.7818314824680298, 0.9749279121818236, 0.4338837391175581};
+};
+template <>
+struct PrimeRoots<11> {
+    static constexpr double c[6] = {1.0, 0.8412535328311812, 0.41541501300188644, -0.14231483827328514,
+                                    -0.6548607339452851, -0.9594929736144974};
+    static constexpr double s[6] = {0.0, 0.5406408174555976, 0.9096319953545183, 0.9898214418809327,
+                                    0.7557495743542583, 0.28173255684142967};
+};
+template <>
+struct PrimeRoots<13> {
+    static constexpr double c[7] = {1.0, 0.8854560256532099, 0.5680647467311558, 0.12053668025532305,
+                                    -0.3546048870425356, -0.7485107481711011, -0.970941817426052};
+    static constexpr double s[7] = {0.0, 0.46472317204376856, 0.8229838658936564, 0.992708874098054,
+                                    0.9350162426854148, 0.6631226582407952, 0.23931566428755777};
+};
+
+// In-register DFT of an odd prime P, natural order in and out: with
+// a_n = x_n + x_{P-n} and b_n = x_n - x_{P-n} (n = 1 ... (P-1)/2),
+//   X_0 = x_0 + sum a_n,
+//   X_k, X_{P-k} = x_0 + sum a_n cos(2 pi n k / P) +/- i SIGN sum b_n sin(2 pi n k / P).
+template <typename T, int P, int SIGN>
+__device__ __forceinline__ void dftOddPrime(cx<T>* v) {
+    constexpr int H = (P - 1) / 2;
+    using PR = PrimeRoots<P>;
+    cx<T> a[H + 1], b[H + 1];
+#pragma unroll
+    for (int n = 1; n <= H; ++n) {
+        a[n] = cadd(v[n], v[P - n]);
+        b[n] = csub(v[n], v[P - n]);
+    }
+    cx<T> out[P];
+    cx<T> x0 = v[0];
+    cx<T> sum = x0;
+#pragma unroll
+    for (int n = 1; n <= H; ++n) sum = cadd(sum, a[n]);
+    out[0] = sum;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        cx<T> re = x0, im{T(0), T(0)};
+#pragma unroll
+        for (int n = 1; n <= H; ++n) {
+            const int m = (n * k) % P;
+            const T cm = T(m <= H ? PR::c[m] : PR::c[P - m]);
+            const T sm = T(m <= H ? PR::s[m] : -PR::s[P - m]);
+            re = {re.re + a[n].re * cm, re.im + a[n].im * cm};
+            im = {im.re + b[n].re * sm, im.im + b[n].im * sm};
+        }
+        // i SIGN im
+        const cx<T> rot = SIGN < 0 ? cx<T>{im.im, -im.re} : cx<T>{-im.im, im.re};
+        out[k] = cadd(re, rot);
+        out[P - k] = csub(re, rot);
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) v[k] = out[k];
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dftSmall(cx<T>* v) {
+    if constexpr ((R & (R - 1)) == 0)
+        dftReg<T, R, SIGN>(v);
+    else
+        dftOddPrime<T, R, SIGN>(v);
+}
+
+// One Stockham pass of radix R (2, 4, 8, 16 or an odd prime up to 13) over
+// transforms of a runtime length N whose sub-transforms so far have length
+// Ns, with the full twiddle table tw[q] = exp(-2 pi i q / N): input r of the
+// butterfly of item (transform, j), k = j mod Ns, is multiplied by
+// exp(-2 pi i r k / (Ns R)) = tw[r k N / (Ns R)] (index < N), and its
+// outputs go to (j / Ns) Ns R + k + r Ns.  A pass holds at most about
+// kFftRegs complex values per thread: ceil(16 / R) butterflies.
+template <typename T, int R, int SIGN, typename Bt>
+__device__ __forceinline__ void stockhamPassAny(cx<T>* buf, int N, int Ns, const Bt& bt,
+                                                const cx<T>* __restrict__ tw) {
+    constexpr int IPT = (kFftRegs + R - 1) / R;
+    const int NR = N / R;
+    const int items = batchCount(bt) * NR;
+    const int step = N / (Ns * R);
+    cx<T> v[IPT][R];
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) {
+        const int item = fftThread() + it * kFftThreads;
+        if (item < items) {
+            int b, jj;
+            bt.splitN(item, NR, b, jj);
+            const int k = jj % Ns;
+            const int boff = bt.base(b);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cx<T> x = buf[bt.at(jj + r * NR) + boff];
+                if (r > 0 && Ns > 1) {
+                    cx<T> w = tw[r * k * step];
+                    if (SIGN > 0) w.im = -w.im;
+                    x = cmul(x, w);
+                }
+                v[it][r] = x;
+            }
+            dftSmall<T, R, SIGN>(v[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IPT; ++it) {
+        const int item = fftThread() + it * kFftThreads;
+        if (item < items) {
+            int b, jj;
+            bt.splitN(item, NR, b, jj);
+            const int k = jj % Ns;
+            const int d = (jj / Ns) * Ns * R + k;
+            const int boff = bt.base(b);
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[bt.at(d + r * Ns) + boff] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Mixed-radix FFT of a runtime length N (radix-16 passes for the power of
+// two, then 8/4/2 for its remainder, then 3, 5, 7, 11, 13) with the full
+// twiddle table; false (nothing done) when N has a prime factor above 13.
+template <typename T, int SIGN, typename Bt>
+__device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restrict__ tw) {
+    int rest = N, p2 = 0, e3 = 0, e5 = 0, e7 = 0, e11 = 0, e13 = 0;
+    while ((rest & 1) == 0) { rest >>= 1; ++p2; }
+    while (rest % 3 == 0) { rest /= 3; ++e3; }
+    while (rest % 5 == 0) { rest /= 5; ++e5; }
+    while (rest % 7 == 0) { rest /= 7; ++e7; }
+    while (rest % 11 == 0) { rest /= 11; ++e11; }
+    while (rest % 13 == 0) { rest /= 13; ++e13; }
+    if (rest != 1) return false;
+    int Ns = 1;
+    for (; p2 >= 4; p2 -= 4, Ns *= 16) stockhamPassAny<T, 16, SIGN>(buf, N, Ns, bt, tw);
+    if (p2 == 3) { stockhamPassAny<T, 8, SIGN>(buf, N, Ns, bt, tw); Ns *= 8; }
+    if (p2 == 2) { stockhamPassAny<T, 4, SIGN>(buf, N, Ns, bt, tw); Ns *= 4; }
+    if (p2 == 1) { stockhamPassAny<T, 2, SIGN>(buf, N, Ns, bt, tw); Ns *= 2; }
+    for (; e3 > 0; --e3, Ns *= 3) stockhamPassAny<T, 3, SIGN>(buf, N, Ns, bt, tw);
+    for (; e5 > 0; --e5, Ns *= 5) stockhamPassAny<T, 5, SIGN>(buf, N, Ns, bt, tw);
+    for (; e7 > 0; --e7, Ns *= 7) stockhamPassAny<T, 7, SIGN>(buf, N, Ns, bt, tw);
+    for (; e11 > 0; --e11, Ns *= 11) stockhamPassAny<T, 11, SIGN>(buf, N, Ns, bt, tw);
+    for (; e13 > 0; --e13, Ns *= 13) stockhamPassAny<T, 13, SIGN>(buf, N, Ns, bt, tw);
+    return true;
+}
+
 // Radix-2^LR passes from sub-transform length 2^L2S up to 2^L2N.
 template <typename T, int SIGN, int L2N, int L2S, int LR, typename Bt>
 __device__ __forceinline__ void radixPasses(cx<T>* buf, const Bt& bt, const cx<T>* __restrict__ tw) {
@@ -312,7 +475,9 @@ __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& b
         return;
     }
     if (log2N < 0) {
-        directDft<T, SIGN>(buf, N, bt, tw);
+        // lengths with prime factors up to 13: mixed-radix passes; others the
+        // direct DFT
+        if (!mixedRadixFft<T, SIGN>(buf, N, bt, tw)) directDft<T, SIGN>(buf, N, bt, tw);
         return;
     }
     // leading pass takes the factor 2^(log2N mod 4); the rest are radix 16
