@@ -114,6 +114,24 @@ __device__ __forceinline__ int ldsIndex(int n, int sn) { return sn == 1 ? n + (n
 // padded length of a unit-stride transform
 __host__ __device__ constexpr int ldsLength(int N) { return N + (N >> 4); }
 
+// Quotient and remainder by a runtime divisor 1 <= d <= 4096 for 0 <= a with
+// a + d < 2^15 (FFT item and position indices), through an FP32 reciprocal
+// raised by 2^-16: that covers the reciprocal's 1-ulp error and the
+// product's rounding, and floor(a (1/d)(1 + 2^-16 +- 2^-22)) = floor(a / d)
+// while (a / d + 1) d (2^-16 + 2^-21) < 1.  Checked exhaustively over that
+// range with the reciprocal rounded down, to nearest and up.
+struct FastDiv {
+    int d;
+    float inv;
+    __device__ __forceinline__ explicit FastDiv(int dv)
+        : d(dv), inv(__builtin_amdgcn_rcpf(static_cast<float>(dv)) * (1.0f + 0x1p-16f)) {}
+    __device__ __forceinline__ int quot(int a) const { return static_cast<int>(static_cast<float>(a) * inv); }
+    __device__ __forceinline__ void divmod(int a, int& q, int& r) const {
+        q = quot(a);
+        r = a - __mul24(q, d);
+    }
+};
+
 template <int NB1>
 struct BatchRows {
     int nb, sb, sb2;
@@ -127,10 +145,7 @@ struct BatchRows {
         b = item >> log2n;
         j = item & ((1 << log2n) - 1);
     }
-    __device__ __forceinline__ void splitN(int item, int n, int& b, int& j) const {
-        b = item / n;
-        j = item - b * n;
-    }
+    __device__ __forceinline__ void splitN(int item, const FastDiv& n, int& b, int& j) const { n.divmod(item, b, j); }
 };
 
 struct BatchCols {
@@ -142,7 +157,7 @@ struct BatchCols {
         b = item & ((1 << log2nb) - 1);
         j = item >> log2nb;
     }
-    __device__ __forceinline__ void splitN(int item, int, int& b, int& j) const { split(item, 0, b, j); }
+    __device__ __forceinline__ void splitN(int item, const FastDiv&, int& b, int& j) const { split(item, 0, b, j); }
 };
 
 template <int NB1>
@@ -263,17 +278,21 @@ template <typename T, int SIGN, typename Bt>
 __device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restrict__ tw) {
     cx<T> out[kFftRegs];
     const int items = batchCount(bt) * N;
+    const FastDiv fN(N);
 #pragma unroll
     for (int it = 0; it < kFftRegs; ++it) {
         const int item = fftThread() + it * kFftThreads;
         if (item < items) {
             int b, k;
-            bt.splitN(item, N, b, k);
+            bt.splitN(item, fN, b, k);
             cx<T> acc{T(0), T(0)};
-            for (int n = 0; n < N; ++n) {
-                cx<T> w = tw[(n * k) % N];
+            // twiddle index n k mod N, advanced by k per term
+            for (int n = 0, q = 0; n < N; ++n) {
+                cx<T> w = tw[q];
                 if (SIGN > 0) w.im = -w.im;
                 acc = cadd(acc, cmul(buf[bt.at(n) + bt.base(b)], w));
+                q += k;
+                if (q >= N) q -= N;
             }
             out[it] = acc;
         }
@@ -284,7 +303,7 @@ __device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restri
         const int item = fftThread() + it * kFftThreads;
         if (item < items) {
             int b, k;
-            bt.splitN(item, N, b, k);
+            bt.splitN(item, fN, b, k);
             buf[bt.at(k) + bt.base(b)] = out[it];
         }
     }
@@ -387,20 +406,24 @@ __device__ __forceinline__ void stockhamPassAny(cx<T>* buf, int N, int Ns, const
     const int NR = N / R;
     const int items = batchCount(bt) * NR;
     const int step = N / (Ns * R);
+    const FastDiv fNR(NR), fNs(Ns);
     cx<T> v[IPT][R];
+    int dst[IPT];  // output position (j / Ns) Ns R + k of each item
 #pragma unroll
     for (int it = 0; it < IPT; ++it) {
         const int item = fftThread() + it * kFftThreads;
         if (item < items) {
-            int b, jj;
-            bt.splitN(item, NR, b, jj);
-            const int k = jj % Ns;
+            int b, jj, q, k;
+            bt.splitN(item, fNR, b, jj);
+            fNs.divmod(jj, q, k);
             const int boff = bt.base(b);
+            dst[it] = __mul24(q, Ns * R) + k;
+            const int ks = __mul24(k, step);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 cx<T> x = buf[bt.at(jj + r * NR) + boff];
                 if (r > 0 && Ns > 1) {
-                    cx<T> w = tw[r * k * step];
+                    cx<T> w = tw[r * ks];
                     if (SIGN > 0) w.im = -w.im;
                     x = cmul(x, w);
                 }
@@ -415,12 +438,10 @@ __device__ __forceinline__ void stockhamPassAny(cx<T>* buf, int N, int Ns, const
         const int item = fftThread() + it * kFftThreads;
         if (item < items) {
             int b, jj;
-            bt.splitN(item, NR, b, jj);
-            const int k = jj % Ns;
-            const int d = (jj / Ns) * Ns * R + k;
+            bt.splitN(item, fNR, b, jj);
             const int boff = bt.base(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[bt.at(d + r * Ns) + boff] = v[it][r];
+            for (int r = 0; r < R; ++r) buf[bt.at(dst[it] + r * Ns) + boff] = v[it][r];
         }
     }
     __syncthreads();

@@ -907,7 +907,8 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
         v[i] = in[plane + static_cast<size_t>(zero ? 0 : iy) * nkxP + kx0 + kk];
     }
     __builtin_amdgcn_sched_barrier(0);  // all loads issued before the first store
-    auto fill = [&](bool deriv) {
+    // (both inlined: a call would move the staged values v to scratch)
+    auto fill = [&](bool deriv) __attribute__((always_inline)) {
 #pragma unroll
         for (int i = 0; i < kMaxPerThread; ++i) {
             const int e = fftThread() + i * kFftThreads;
@@ -921,7 +922,7 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
             if (e < nel) tile[e] = zero ? cx<T>{T(0), T(0)} : x;
         }
     };
-    auto transform = [&](size_t outPlane) {
+    auto transform = [&](size_t outPlane) __attribute__((always_inline)) {
         __syncthreads();
         if (sign > 0)
             fftLds<T, +1, L2N>(tile, Ny, log2Ny, BatchCols{log2KT}, tw);
@@ -1157,7 +1158,12 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     }
     for (int e = fftThread(); e < RB * Nx; e += kFftThreads) {
         int rr, x;
-        divmodN(e, Nx, log2Nx, rr, x);
+        if constexpr (RB == 1) {
+            rr = 0;
+            x = e;
+        } else {
+            divmodN(e, Nx, log2Nx, rr, x);
+        }
         const int xl = ldsIndex(x, 1);
         cx<T>* rb = buf + rr * RS;
         T ch[2 * PAIRS];
