@@ -132,14 +132,14 @@ struct FastDiv {
     }
 };
 
-template <int NB1>
+template <int NB1, bool PAD = true>
 struct BatchRows {
     int nb, sb, sb2;
     __device__ __forceinline__ int base(int b) const {
         const unsigned u = static_cast<unsigned>(b);
         return static_cast<int>(u % NB1) * sb + static_cast<int>(u / NB1) * sb2;
     }
-    __device__ __forceinline__ static int at(int n) { return n + (n >> 4); }
+    __device__ __forceinline__ static int at(int n) { return PAD ? n + (n >> 4) : n; }
     // item -> (transform, position) when each transform holds 2^log2n items
     __device__ __forceinline__ void split(int item, int log2n, int& b, int& j) const {
         b = item >> log2n;
@@ -160,8 +160,8 @@ struct BatchCols {
     __device__ __forceinline__ void splitN(int item, const FastDiv&, int& b, int& j) const { split(item, 0, b, j); }
 };
 
-template <int NB1>
-__device__ __forceinline__ int batchCount(const BatchRows<NB1>& bt) { return bt.nb; }
+template <int NB1, bool PAD>
+__device__ __forceinline__ int batchCount(const BatchRows<NB1, PAD>& bt) { return bt.nb; }
 __device__ __forceinline__ int batchCount(const BatchCols& bt) { return bt.nbt(); }
 
 // exp(-2 pi i q / 16), q = 0..15
@@ -447,10 +447,10 @@ __device__ __forceinline__ void stockhamPassAny(cx<T>* buf, int N, int Ns, const
     __syncthreads();
 }
 
-// Mixed-radix FFT of a runtime length N (radix-16 passes for the power of
-// two, then 8/4/2 for its remainder, then 3, 5, 7, 11, 13) with the full
+// Mixed-radix FFT of a runtime length N (passes of 3, 5, 7, 11, 13, then
+// radix-16 passes for the power of two and 8/4/2 for its remainder) with the full
 // twiddle table; false (nothing done) when N has a prime factor above 13.
-template <typename T, int SIGN, typename Bt>
+template <typename T, int SIGN, int MAXP = 13, typename Bt>
 __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restrict__ tw) {
     int rest = N, p2 = 0, e3 = 0, e5 = 0, e7 = 0, e11 = 0, e13 = 0;
     while ((rest & 1) == 0) { rest >>= 1; ++p2; }
@@ -459,17 +459,21 @@ __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, c
     while (rest % 7 == 0) { rest /= 7; ++e7; }
     while (rest % 11 == 0) { rest /= 11; ++e11; }
     while (rest % 13 == 0) { rest /= 13; ++e13; }
-    if (rest != 1) return false;
+    if (rest != 1 || (MAXP < 11 && e11 + e13 > 0)) return false;
+    // odd radices first: the power-of-two passes then scatter with an odd
+    // stride Ns (no LDS bank conflicts in an unpadded layout)
     int Ns = 1;
-    for (; p2 >= 4; p2 -= 4, Ns *= 16) stockhamPassAny<T, 16, SIGN>(buf, N, Ns, bt, tw);
-    if (p2 == 3) { stockhamPassAny<T, 8, SIGN>(buf, N, Ns, bt, tw); Ns *= 8; }
-    if (p2 == 2) { stockhamPassAny<T, 4, SIGN>(buf, N, Ns, bt, tw); Ns *= 4; }
-    if (p2 == 1) { stockhamPassAny<T, 2, SIGN>(buf, N, Ns, bt, tw); Ns *= 2; }
     for (; e3 > 0; --e3, Ns *= 3) stockhamPassAny<T, 3, SIGN>(buf, N, Ns, bt, tw);
     for (; e5 > 0; --e5, Ns *= 5) stockhamPassAny<T, 5, SIGN>(buf, N, Ns, bt, tw);
     for (; e7 > 0; --e7, Ns *= 7) stockhamPassAny<T, 7, SIGN>(buf, N, Ns, bt, tw);
-    for (; e11 > 0; --e11, Ns *= 11) stockhamPassAny<T, 11, SIGN>(buf, N, Ns, bt, tw);
-    for (; e13 > 0; --e13, Ns *= 13) stockhamPassAny<T, 13, SIGN>(buf, N, Ns, bt, tw);
+    if constexpr (MAXP >= 11) {
+        for (; e11 > 0; --e11, Ns *= 11) stockhamPassAny<T, 11, SIGN>(buf, N, Ns, bt, tw);
+        for (; e13 > 0; --e13, Ns *= 13) stockhamPassAny<T, 13, SIGN>(buf, N, Ns, bt, tw);
+    }
+    for (; p2 >= 4; p2 -= 4, Ns *= 16) stockhamPassAny<T, 16, SIGN>(buf, N, Ns, bt, tw);
+    if (p2 == 3) stockhamPassAny<T, 8, SIGN>(buf, N, Ns, bt, tw);
+    if (p2 == 2) stockhamPassAny<T, 4, SIGN>(buf, N, Ns, bt, tw);
+    if (p2 == 1) stockhamPassAny<T, 2, SIGN>(buf, N, Ns, bt, tw);
     return true;
 }
 
@@ -482,26 +486,10 @@ __device__ __forceinline__ void radixPasses(cx<T>* buf, const Bt& bt, const cx<T
     }
 }
 
-// log2N < 0 marks a length that is not a power of two.  L2N > 0 fixes the
-// length 2^L2N at compile time (N and log2N are then ignored): every pass
-// and every index shift is a constant, and the passes are of radix 2^LR
-// (LR = 4 or 2; the leading pass takes the remaining factor) with the
-// twiddle table of that pass sequence.
-template <typename T, int SIGN, int L2N = 0, int LR = 4, typename Bt>
-__device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
-    if constexpr (L2N > 0) {
-        constexpr int lead = L2N % LR;
-        if constexpr (lead != 0) stockhamPass<T, 1 << lead, SIGN>(buf, L2N, 0, bt, tw);
-        radixPasses<T, SIGN, L2N, lead, LR>(buf, bt, tw);
-        return;
-    }
-    if (log2N < 0) {
-        // lengths with prime factors up to 13: mixed-radix passes; others the
-        // direct DFT
-        if (!mixedRadixFft<T, SIGN>(buf, N, bt, tw)) directDft<T, SIGN>(buf, N, bt, tw);
-        return;
-    }
-    // leading pass takes the factor 2^(log2N mod 4); the rest are radix 16
+// FFT of a runtime power-of-two length 2^log2N: the leading pass takes the
+// factor 2^(log2N mod 4); the rest are radix 16.
+template <typename T, int SIGN, typename Bt>
+__device__ __forceinline__ void fftLdsPow2(cx<T>* buf, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
     int l2s = 0;
     switch (log2N & 3) {
         case 1: stockhamPass<T, 2, SIGN>(buf, log2N, l2s, bt, tw); l2s = 1; break;
@@ -510,6 +498,59 @@ __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& b
         default: break;
     }
     for (; l2s < log2N; l2s += 4) stockhamPass<T, 16, SIGN>(buf, log2N, l2s, bt, tw);
+}
+
+// FFT of a length that is not a power of two: mixed-radix passes when its
+// prime factors are at most 13, else the direct DFT.  SMOOTH7: the caller
+// has checked that they are at most 7 (only the passes of 2 ... 16, 3, 5
+// and 7 are compiled: fewer registers).
+template <typename T, int SIGN, bool SMOOTH7 = false, typename Bt>
+__device__ __forceinline__ void fftLdsAny(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restrict__ tw) {
+    if constexpr (SMOOTH7)
+        mixedRadixFft<T, SIGN, 7>(buf, N, bt, tw);
+    else if (!mixedRadixFft<T, SIGN>(buf, N, bt, tw))
+        directDft<T, SIGN>(buf, N, bt, tw);
+}
+
+// Kernels take the length class as a template argument L2N: L2N > 0 fixes
+// the length 2^L2N at compile time (N and log2N are then ignored): every
+// pass and every index shift is a constant, and the passes are of radix
+// 2^LR (LR = 4 or 2; the leading pass takes the remaining factor) with the
+// twiddle table of that pass sequence.  L2N = 0: any runtime length;
+// L2N = kSmooth7: a runtime length whose prime factors are at most 7.
+// log2N < 0 marks a length that is not a power of two.
+constexpr int kSmooth7 = -1;
+template <typename T, int SIGN, int L2N = 0, int LR = 4, typename Bt>
+__device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
+    if constexpr (L2N > 0) {
+        constexpr int lead = L2N % LR;
+        if constexpr (lead != 0) stockhamPass<T, 1 << lead, SIGN>(buf, L2N, 0, bt, tw);
+        radixPasses<T, SIGN, L2N, lead, LR>(buf, bt, tw);
+        return;
+    }
+    if (log2N < 0)
+        fftLdsAny<T, SIGN, L2N == kSmooth7>(buf, N, bt, tw);
+    else
+        fftLdsPow2<T, SIGN>(buf, log2N, bt, tw);
+}
+
+// Row transforms of the x kernels (transform b at row b / NB1, slot b %
+// NB1): power-of-two lengths in the padded layout (n + n/16, for their
+// radix-16 passes, which scatter with a stride of 16), other lengths in the
+// plain one (n; their odd-radix passes come first, and the addresses of the
+// generic passes stay linear).  rowsPadded(log2N) says which.
+__device__ __forceinline__ bool rowsPadded(int log2N) { return log2N >= 0; }
+template <typename T, int SIGN, int L2N, int NB1>
+__device__ __forceinline__ void fftRows(cx<T>* buf, int N, int log2N, int nb, int sb, int sb2,
+                                        const cx<T>* __restrict__ tw) {
+    if constexpr (L2N > 0) {
+        fftLds<T, SIGN, L2N>(buf, N, log2N, BatchRows<NB1, true>{nb, sb, sb2}, tw);
+    } else {
+        if (rowsPadded(log2N))
+            fftLdsPow2<T, SIGN>(buf, log2N, BatchRows<NB1, true>{nb, sb, sb2}, tw);
+        else
+            fftLdsAny<T, SIGN, L2N == kSmooth7>(buf, N, BatchRows<NB1, false>{nb, sb, sb2}, tw);
+    }
 }
 
 // Forward (SIGN = -1) FFT of NB 256-point transforms (unit stride, padded,

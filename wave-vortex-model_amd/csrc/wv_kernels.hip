@@ -862,7 +862,9 @@ __global__ __launch_bounds__(256) void k_synth16(const cx<float>* __restrict__ F
 // rowLimit[tile] (if given) is the largest |ky| retained in the kx tile: input
 // rows r < iy < Ny - r are known zeros (skipIn) or output rows there are not
 // needed (skipOut).  L2N > 0 fixes Ny = 2^L2N and the tile width 2^L2KT at
-// compile time (the runtime Ny, log2Ny and log2KT are then ignored).
+// compile time (the runtime Ny, log2Ny and log2KT are then ignored);
+// L2N = kSmooth7 takes runtime lengths whose prime factors are at most 7,
+// L2N = 0 any length.
 // Channel plan of the inverse y transform with y derivatives: block z takes
 // input channel src[z] to output channel src[z] and, when der[z] >= 0, also
 // i l times it to output channel der[z] (l = lrow[iy], the y wavenumber of
@@ -878,7 +880,7 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
                                                       int Nz, const cx<T>* __restrict__ twY, int sign,
                                                       const int* __restrict__ rowLimit, int skipIn, int skipOut,
                                                       YDerivPlan yp, const T* __restrict__ lrow) {
-    const int Ny = L2N > 0 ? 1 << L2N : NyArg;
+    const int Ny = L2N > 0 ? 1 << (L2N > 0 ? L2N : 0) : NyArg;
     const int log2Ny = L2N > 0 ? L2N : log2NyArg;
     const int log2KT = L2N > 0 ? L2KT : log2KTArg;
     const int KT = 1 << log2KT;
@@ -895,7 +897,7 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
     // all of a thread's loads are issued before its LDS stores; entries per
     // thread: nel / blockDim for a compiled length, else at most 16 (nel <=
     // 16 blockDim, checked at setup)
-    constexpr int kMaxPerThread = L2N > 0 ? ((1 << (L2N + L2KT)) + kFftThreads - 1) / kFftThreads : 16;
+    constexpr int kMaxPerThread = L2N > 0 ? ((1 << (L2N > 0 ? L2N + L2KT : 0)) + kFftThreads - 1) / kFftThreads : 16;
     cx<T> v[kMaxPerThread];
 #pragma unroll
     for (int i = 0; i < kMaxPerThread; ++i) {
@@ -1044,7 +1046,9 @@ struct XMap<5> {
 // Per (y, z) row: c2r of the spatial channels along x (two real transforms
 // per complex FFT), the NT advection products and r2c of the products.
 // kxv[k] = the x wavenumber of half-spectrum column k.  L2N > 0 fixes
-// Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored).
+// Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored);
+// L2N = kSmooth7 takes runtime lengths whose prime factors are at most 7,
+// L2N = 0 any length.
 // The forward transforms run the same radix-16 passes as the inverse ones:
 // at Nx = 256 with two threads per butterfly (fftFwd256, one full wave for
 // the two transforms of a row), at other lengths on the first 16 threads per
@@ -1063,10 +1067,13 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
                                                         T etaScale, const double* __restrict__ adv) {
     using XM = XMap<NT>;
     constexpr int PAIRS = XM::pairs;
-    const int Nx = L2N > 0 ? 1 << L2N : NxArg;
+    const int Nx = L2N > 0 ? 1 << (L2N > 0 ? L2N : 0) : NxArg;
     const int log2Nx = L2N > 0 ? L2N : log2NxArg;
     extern __shared__ __align__(16) unsigned char smem[];
     const int NxP = xfusedStride<T>(Nx);
+    // LDS position of entry n of a transform (the layout of fftRows)
+    const bool pad = L2N > 0 || rowsPadded(log2Nx);
+    auto xi = [=](int n) { return pad ? ldsIndex(n, 1) : n; };
     const int RS = xfusedRowStride<T>(Nx, PAIRS);
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
     cx<T>* buf = tw + Nx;  // [RB][RS], row rr holds PAIRS transforms of stride NxP
@@ -1132,8 +1139,8 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
                 // k = 0: real parts only (the imaginary part of the kx = 0
                 // column is ignored by the c2r transform)
                 const T aim = (h == 0 && k == 0) ? T(0) : a.im, bim = (h == 0 && k == 0) ? T(0) : b.im;
-                z[ldsIndex(k, 1)] = {a.re - bim, aim + b.re};
-                if (k > 0) z[ldsIndex(Nx - k, 1)] = {a.re + bim, b.re - aim};
+                z[xi(k)] = {a.re - bim, aim + b.re};
+                if (k > 0) z[xi(Nx - k)] = {a.re + bim, b.re - aim};
             }
         }
     }
@@ -1145,7 +1152,7 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
             const int lo = 96 - nkx;
             for (int e = lane; e < lo + (Nx - nkx - 159); e += 64) z[ldsIndex(e < lo ? nkx + e : 160 + e - lo, 1)] = cx<T>{T(0), T(0)};
         } else {
-            for (int k = nkx + lane; k <= Nx - nkx; k += 64) z[ldsIndex(k, 1)] = cx<T>{T(0), T(0)};
+            for (int k = nkx + lane; k <= Nx - nkx; k += 64) z[xi(k)] = cx<T>{T(0), T(0)};
         }
     }
     __syncthreads();
@@ -1154,7 +1161,7 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
         stockhamPass<T, 16, +1, BatchRows<PAIRS>, 0x3C0u>(buf, 8, 0, bt, tw);
         stockhamPass<T, 16, +1>(buf, 8, 4, bt, tw);
     } else {
-        fftLds<T, +1, L2N>(buf, Nx, log2Nx, BatchRows<PAIRS>{RB * PAIRS, NxP, RS}, tw);
+        fftRows<T, +1, L2N, PAIRS>(buf, Nx, log2Nx, RB * PAIRS, NxP, RS, tw);
     }
     for (int e = fftThread(); e < RB * Nx; e += kFftThreads) {
         int rr, x;
@@ -1164,7 +1171,7 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
         } else {
             divmodN(e, Nx, log2Nx, rr, x);
         }
-        const int xl = ldsIndex(x, 1);
+        const int xl = xi(x);
         cx<T>* rb = buf + rr * RS;
         T ch[2 * PAIRS];
 #pragma unroll
@@ -1199,7 +1206,7 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     if constexpr (L2N == 8)
         fftFwd256<T, 2 * RB>(buf, BatchRows<2>{2 * RB, NxP, RS}, tw);
     else
-        fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<2>{2 * RB, NxP, RS}, tw);
+        fftRows<T, -1, L2N, 2>(buf, Nx, log2Nx, 2 * RB, NxP, RS, tw);
     // wave w unpacks (row, pair) rq = w, w + nwaves, ...; its lanes walk k
     constexpr int nwaves = kFftThreads / 64;
     for (int rq = waveIndex(); rq < RB * 2; rq += nwaves) {
@@ -1212,8 +1219,8 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
         const cx<T>* rb = buf + rr * RS + q * NxP;
         const T h = T(0.5), hb = q == 1 ? h / etaScale : h;
         for (int k = threadIdx.x & 63; k < nkx; k += 64) {
-            const cx<T> Za = rb[ldsIndex(k, 1)];
-            const cx<T> Zb = rb[ldsIndex(k == 0 ? 0 : Nx - k, 1)];
+            const cx<T> Za = rb[xi(k)];
+            const cx<T> Zb = rb[xi(k == 0 ? 0 : Nx - k)];
             const unsigned off = static_cast<unsigned>(k) * sizeof(cx<T>);
             if (2 * q < XM::targets)
                 *reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(Pa) + off) = {(Za.re + Zb.re) * h, (Za.im - Zb.im) * h};
@@ -1242,7 +1249,7 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
                                               int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
                                               const cx<T>* __restrict__ twX, const double* __restrict__ adv = nullptr,
                                               double coriolis = 0.0) {
-    const int Nx = L2N > 0 ? 1 << L2N : NxArg;  // L2N > 0: the length fixed at compile time
+    const int Nx = L2N > 0 ? 1 << (L2N > 0 ? L2N : 0) : NxArg;  // L2N > 0: the length fixed at compile time
     const int log2Nx = L2N > 0 ? L2N : log2NxArg;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
@@ -1380,7 +1387,7 @@ template <int L2N, int L2KT>
 __global__ __launch_bounds__(kFftThreads) void k_ymask(cx<double>* __restrict__ C, int NyArg, int log2NyArg, int nkx,
                                                        int nkxP, int log2KTArg, const cx<double>* __restrict__ twY,
                                                        const double* __restrict__ keep) {
-    const int Ny = L2N > 0 ? 1 << L2N : NyArg;
+    const int Ny = L2N > 0 ? 1 << (L2N > 0 ? L2N : 0) : NyArg;
     const int log2Ny = L2N > 0 ? L2N : log2NyArg;
     const int log2KT = L2N > 0 ? L2KT : log2KTArg;
     const int KT = 1 << log2KT;
@@ -2211,7 +2218,7 @@ template <typename T, int L2N = 0>
 __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<T>* __restrict__ P, int nf, int rpb,
                                               int NxArg, int log2NxArg, int Ny, int Nz, int nkx, int nkxP,
                                               const cx<T>* __restrict__ twX, int adjointWeights) {
-    const int Nx = L2N > 0 ? 1 << L2N : NxArg;  // L2N > 0: the length fixed at compile time
+    const int Nx = L2N > 0 ? 1 << (L2N > 0 ? L2N : 0) : NxArg;  // L2N > 0: the length fixed at compile time
     const int log2Nx = L2N > 0 ? L2N : log2NxArg;
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
@@ -2406,6 +2413,14 @@ int log2Exact(size_t n) {
     int b = 0;
     while ((size_t(1) << b) < n) ++b;
     return b;
+}
+
+// prime factors of n all at most 7 (the lengths of the kSmooth7 kernels)
+bool smooth7(size_t n) {
+    if (n == 0) return false;
+    for (size_t p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
 }
 
 template <typename T>
@@ -2793,8 +2808,11 @@ struct KernelImpl {
         // then runs 8 columns in a 32 KB tile, 8 workgroups per CU instead
         // of 4 (FP32 keeps 16 columns, which measured faster than 8; FP64 at
         // Ny = 512 keeps 8, 4 measured slower)
+        // (runtime-length kernels stage 16 entries per thread whatever the
+        // tile: they keep the widest tile)
         KT = static_cast<int>(std::max<size_t>(1, std::min<size_t>(16, 4096 / std::max<size_t>(cfg.Ny, 1))));
-        if (prec == WV_PRECISION_FP64 && KT > 8 && static_cast<size_t>(KT) * cfg.Ny > 2048)
+        const bool ny2 = cfg.Ny > 0 && (cfg.Ny & (cfg.Ny - 1)) == 0;
+        if (prec == WV_PRECISION_FP64 && ny2 && KT > 8 && static_cast<size_t>(KT) * cfg.Ny > 2048)
             KT = static_cast<int>(std::max<size_t>(8, 2048 / cfg.Ny));
         while (KT & (KT - 1)) --KT;  // power of two
         std::string err;
@@ -3138,6 +3156,7 @@ struct KernelImpl {
         const int l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
         const bool dv = yp != nullptr;
         Fn fn = dv ? k_yfft<T, 0, 0, true> : k_yfft<T, 0, 0, false>;
+        if (smooth7(d.c.Ny)) fn = dv ? k_yfft<T, kSmooth7, 0, true> : k_yfft<T, kSmooth7, 0, false>;
         if (l2y == 6 && l2t == 4) fn = dv ? k_yfft<T, 6, 4, true> : k_yfft<T, 6, 4, false>;
         if (l2y == 7 && l2t == 4) fn = dv ? k_yfft<T, 7, 4, true> : k_yfft<T, 7, 4, false>;
         if (l2y == 8 && l2t == 4) fn = dv ? k_yfft<T, 8, 4, true> : k_yfft<T, 8, 4, false>;
@@ -3184,8 +3203,9 @@ struct KernelImpl {
             case 9: return launchXfusedKernel<T, NTc, xfusedRows<T>(512, pairs), 9>(s);
             default: break;
         }
-        if (xfusedRows<T>(static_cast<int>(d.c.Nx), pairs) == 2)
-            launchXfusedKernel<T, NTc, 2, 0>(s);
+        static_assert(xfusedRows<T>(64, pairs) == 1, "one-row blocks at every length");
+        if (smooth7(d.c.Nx))
+            launchXfusedKernel<T, NTc, 1, kSmooth7>(s);
         else
             launchXfusedKernel<T, NTc, 1, 0>(s);
     }
