@@ -146,8 +146,8 @@ __global__ __launch_bounds__(256) void k_xadj(const cx<T>* __restrict__ G, const
         for (int k = nkx + lane; k <= Nx - nkx; k += 64) zb[ldsIndex(k, 1)] = cx<T>{T(0), T(0)};
     }
     __syncthreads();
-    fftLds<T, +1>(buf, Nx, log2Nx, BatchRows<1>{PAIRS, 0, NxP}, tw);
-    fftLds<T, +1>(mub, Nx, log2Nx, BatchRows<1>{TP, 0, NxP}, tw);
+    fftLdsBatched<T, +1>(buf, Nx, log2Nx, PAIRS, NxP, tw);
+    fftLdsBatched<T, +1>(mub, Nx, log2Nx, TP, NxP, tw);
     // products transposed, in place per x
     for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
         const int xl = ldsIndex(x, 1);
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_xadj(const cx<T>* __restrict__ G, const
         for (int p = 0; p < PAIRS; ++p) buf[p * NxP + xl] = {out[2 * p], out[2 * p + 1]};
     }
     __syncthreads();
-    fftLds<T, -1>(buf, Nx, log2Nx, BatchRows<1>{PAIRS, 0, NxP}, tw);
+    fftLdsBatched<T, -1>(buf, Nx, log2Nx, PAIRS, NxP, tw);
     // unpack the real transforms, weight, fold into the spectrum channels
     for (int c = wave; c < XA::nh; c += 4) {
         for (int k = lane; k < nkx; k += 64) {

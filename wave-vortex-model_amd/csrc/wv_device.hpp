@@ -540,17 +540,34 @@ __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& b
 // plain one (n; their odd-radix passes come first, and the addresses of the
 // generic passes stay linear).  rowsPadded(log2N) says which.
 __device__ __forceinline__ bool rowsPadded(int log2N) { return log2N >= 0; }
+// A runtime length above 4096 / nb points runs in calls of at most
+// kFftRegs * kFftThreads points (transforms b0 ... at buf + b0 sb; the
+// caller then holds a single row, nb <= NB1).
 template <typename T, int SIGN, int L2N, int NB1>
 __device__ __forceinline__ void fftRows(cx<T>* buf, int N, int log2N, int nb, int sb, int sb2,
                                         const cx<T>* __restrict__ tw) {
     if constexpr (L2N > 0) {
         fftLds<T, SIGN, L2N>(buf, N, log2N, BatchRows<NB1, true>{nb, sb, sb2}, tw);
     } else {
-        if (rowsPadded(log2N))
-            fftLdsPow2<T, SIGN>(buf, log2N, BatchRows<NB1, true>{nb, sb, sb2}, tw);
-        else
-            fftLdsAny<T, SIGN, L2N == kSmooth7>(buf, N, BatchRows<NB1, false>{nb, sb, sb2}, tw);
+        const int per = nb * N <= kFftRegs * kFftThreads ? nb : max(1, kFftRegs * kFftThreads / N);
+        for (int b0 = 0; b0 < nb; b0 += per) {
+            const int n = min(per, nb - b0);
+            if (rowsPadded(log2N))
+                fftLdsPow2<T, SIGN>(buf + b0 * sb, log2N, BatchRows<NB1, true>{n, sb, sb2}, tw);
+            else
+                fftLdsAny<T, SIGN, L2N == kSmooth7>(buf + b0 * sb, N, BatchRows<NB1, false>{n, sb, sb2}, tw);
+        }
     }
+}
+
+// nb unit-stride padded transforms at buf + b stride, in calls of at most
+// kFftRegs * kFftThreads points (lengths above 4096 / nb).
+template <typename T, int SIGN, int L2N = 0>
+__device__ __forceinline__ void fftLdsBatched(cx<T>* buf, int N, int log2N, int nb, int stride,
+                                              const cx<T>* __restrict__ tw) {
+    const int per = nb * N <= kFftRegs * kFftThreads ? nb : max(1, kFftRegs * kFftThreads / N);
+    for (int b0 = 0; b0 < nb; b0 += per)
+        fftLds<T, SIGN, L2N>(buf + b0 * stride, N, log2N, BatchRows<1>{min(per, nb - b0), 0, stride}, tw);
 }
 
 // Forward (SIGN = -1) FFT of NB 256-point transforms (unit stride, padded,

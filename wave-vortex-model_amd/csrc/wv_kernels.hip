@@ -1268,7 +1268,7 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
         buf[q * NxP + ldsIndex(k, 1)] = {A.re - B.im, A.im + B.re};
     }
     __syncthreads();
-    fftLds<T, +1, L2N>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
+    fftLdsBatched<T, +1, L2N>(buf, Nx, log2Nx, ntr, NxP, twX);
     const size_t R = static_cast<size_t>(Nx) * Ny * Nz;
     auto field = [&](int sg, int x) {
         const cx<T> v = buf[(sg / 2) * NxP + ldsIndex(x, 1)];
@@ -2236,7 +2236,7 @@ __global__ __launch_bounds__(256) void k_xfwd(const double* __restrict__ in, cx<
         buf[q * NxP + ldsIndex(x, 1)] = {value(2 * q, x), value(2 * q + 1, x)};
     }
     __syncthreads();
-    fftLds<T, -1, L2N>(buf, Nx, log2Nx, BatchRows<1>{ntr, 0, NxP}, twX);
+    fftLdsBatched<T, -1, L2N>(buf, Nx, log2Nx, ntr, NxP, twX);
     const size_t chStride = static_cast<size_t>(Nz) * Ny * nkxP;
     for (int e = threadIdx.x; e < ntr * nkx; e += blockDim.x) {
         const int q = e / nkx, k = e % nkx;
@@ -2831,18 +2831,17 @@ struct KernelImpl {
         targetCode = hydro ? (0 | (1 << 4) | (3 << 8)) : (0 | (1 << 4) | (2 << 8) | (3 << 12));
         const size_t pairs = static_cast<size_t>(NT == 4 ? XMap<4>::pairs : XMap<3>::pairs);
         // In-LDS FFTs hold at most kFftRegs * kFftThreads = 4096 points per
-        // workgroup: the fused x kernel `pairs` transforms of one row, the
-        // generic x transforms (forward, inverse, tracer) 8 of them, the y
-        // transforms KT columns of Ny.
-        constexpr size_t kXBatch = 8;
+        // call: one transform of Nx or KT columns of Ny (the x kernels take a
+        // row's transforms in several calls when they hold more); the fused x
+        // kernel keeps a row's `pairs` transforms and the twiddles in LDS.
         const size_t fftPoints = static_cast<size_t>(kFftRegs) * kFftThreads;
-        if (std::max(pairs, kXBatch) * cfg.Nx > fftPoints || static_cast<size_t>(KT) * cfg.Ny > fftPoints)
-            throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx <= 512, Ny <= 4096)."};
+        if (cfg.Nx > fftPoints || static_cast<size_t>(KT) * cfg.Ny > fftPoints)
+            throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx, Ny <= 4096)."};
         int maxLds = 0;
         hipCheck(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
         maxLdsPerBlock = static_cast<size_t>(maxLds);
         const size_t elem = prec == WV_PRECISION_FP64 ? 16 : 8;
-        const size_t xLds = pairs * static_cast<size_t>(ldsLength(static_cast<int>(cfg.Nx))) * elem;
+        const size_t xLds = (cfg.Nx + pairs * static_cast<size_t>(ldsLength(static_cast<int>(cfg.Nx)))) * elem;
         const size_t yLds = (static_cast<size_t>(KT) + 1) * cfg.Ny * elem;
         if (xLds > static_cast<size_t>(maxLds) || yLds > static_cast<size_t>(maxLds))
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid needs more LDS per workgroup than the device provides."};
@@ -3398,14 +3397,21 @@ struct KernelImpl {
     }
 
     // The generic x transforms take rows in groups of about 16 real signals
-    // (8 complex transforms per workgroup).
-    static int xRowsPerBlock(int signalsPerRow) { return std::max(1, 16 / std::max(1, signalsPerRow)); }
+    // (8 complex transforms per workgroup), fewer above Nx = 512 (the
+    // workgroup's transforms hold about 4096 points; a row with more signals
+    // than that takes its transforms in several calls).
+    int xRowsPerBlock(int signalsPerRow) const {
+        const int Nx = static_cast<int>(d.c.Nx);
+        const int cap = std::max(2, std::min(16, 2 * (kFftRegs * kFftThreads) / std::max(Nx, 1)));
+        return std::max(1, cap / std::max(1, signalsPerRow));
+    }
     template <typename T>
     void launchXfwdRows(const double* in, cx<T>* P, int nf, int nkx, int nkxP, const cx<T>* tw, int adjointWeights,
                         hipStream_t s) {
         const int rpb = xRowsPerBlock(nf), ntr = (rpb * nf + 1) / 2;
         const int nrows = static_cast<int>(d.c.Ny * d.c.Nz);
         const size_t lds = static_cast<size_t>(ntr) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
+        if (lds > maxLdsPerBlock) throw Failure{WV_UNSUPPORTED_OPERATION, "x transforms exceed the LDS of a workgroup."};
         auto fn = k_xfwd<T, 0>;
         switch (log2Exact(d.c.Nx)) {  // lengths 64 ... 512 compiled as such
             case 6: fn = k_xfwd<T, 6>; break;
@@ -3428,6 +3434,7 @@ struct KernelImpl {
         const int rpb = xRowsPerBlock(nch), ntr = (rpb * nch + 1) / 2;
         const int nrows = static_cast<int>(d.c.Ny * d.c.Nz);
         const size_t lds = static_cast<size_t>(ntr) * ldsLength(static_cast<int>(d.c.Nx)) * sizeof(cx<T>);
+        if (lds > maxLdsPerBlock) throw Failure{WV_UNSUPPORTED_OPERATION, "x transforms exceed the LDS of a workgroup."};
         auto fn = k_xinv<T, OP, 0>;
         switch (log2Exact(d.c.Nx)) {  // lengths 64 ... 512 compiled as such
             case 6: fn = k_xinv<T, OP, 6>; break;
@@ -3891,6 +3898,8 @@ struct KernelImpl {
         const int pairs = NT == 4 ? XMap<4>::pairs : XMap<3>::pairs;
         const size_t lds = (static_cast<size_t>(Nx) + static_cast<size_t>(pairs + 2) * xfusedStride<T>(Nx)) *
                            sizeof(cx<T>);
+        if (lds > maxLdsPerBlock)
+            throw Failure{WV_UNSUPPORTED_OPERATION, "Adjoint x transforms exceed the LDS of a workgroup."};
         timed("xadj", 0.0, s, [&] {
             const dim3 grid(static_cast<unsigned>(Ny * Nz));
             if (NT == 4)
