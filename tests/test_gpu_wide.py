@@ -1,6 +1,8 @@
 """GPU parity at horizontal lengths above 512 points, where the x kernels
 take a row's transforms in several in-LDS FFT calls of at most 4096 points
-and the generic x transforms take fewer rows per workgroup.
+and the generic x transforms take fewer rows per workgroup, and at Ny = 1024,
+where the y tiles are 4 columns wide (FP32 rows of 32 bytes: the tiles are
+renumbered onto XCDs).
 
 Bars: FP64 flux, inverse and forward transforms and APV against the oracle at
 1e-12 relative (max|diff| / max|ref|, the inverse at 1e-13); the mixed flux
@@ -120,3 +122,17 @@ def test_mixed_flux_vjp_at_nx_1024_dot_product(torch_gpu):
     with pytest.raises(wvamd.WVKernelError) as e:
         k64.flux_vjp(*y, *lam, t=700.0)
     assert e.value.status == "unsupportedOperation"
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("mixed-split16", 1e-4)])
+def test_flux_at_ny_1024_matches_oracle(torch_gpu, prec, tol):
+    import wvamd
+    cfg = wvamd.make_config(16, 1024, 9, None, Lx=8e3, Ly=400e3)
+    print(f"[gpu] tall {prec} 16x1024x9", flush=True)
+    k = wvamd.WVTransformConstantStratificationKernel(cfg, prec)
+    o = Oracle(cfg)
+    A = _state(o, 1024)
+    F = [t.cpu().numpy() for t in k.nonlinear_flux(*[k.to_device(a) for a in A], t=450.0, t0=0.0)]
+    err = rel_err_joint(F, o.nonlinear_flux(*A, t=450.0, t0=0.0))
+    print(f"[gpu]   flux rel err {err:.2e}", flush=True)
+    assert err <= tol

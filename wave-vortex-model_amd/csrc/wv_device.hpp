@@ -547,7 +547,9 @@ template <typename T, int SIGN, int L2N, int NB1>
 __device__ __forceinline__ void fftRows(cx<T>* buf, int N, int log2N, int nb, int sb, int sb2,
                                         const cx<T>* __restrict__ tw) {
     if constexpr (L2N > 0) {
-        fftLds<T, SIGN, L2N>(buf, N, log2N, BatchRows<NB1, true>{nb, sb, sb2}, tw);
+        constexpr int per = (kFftRegs * kFftThreads) >> L2N;  // transforms per call
+        for (int b0 = 0; b0 < nb; b0 += per)
+            fftLds<T, SIGN, L2N>(buf + b0 * sb, N, log2N, BatchRows<NB1, true>{min(per, nb - b0), sb, sb2}, tw);
     } else {
         const int per = nb * N <= kFftRegs * kFftThreads ? nb : max(1, kFftRegs * kFftThreads / N);
         for (int b0 = 0; b0 < nb; b0 += per) {

@@ -879,7 +879,7 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
                                                       int NyArg, int log2NyArg, int nkx, int nkxP, int log2KTArg,
                                                       int Nz, const cx<T>* __restrict__ twY, int sign,
                                                       const int* __restrict__ rowLimit, int skipIn, int skipOut,
-                                                      YDerivPlan yp, const T* __restrict__ lrow) {
+                                                      YDerivPlan yp, const T* __restrict__ lrow, int lineRemap) {
     const int Ny = L2N > 0 ? 1 << (L2N > 0 ? L2N : 0) : NyArg;
     const int log2Ny = L2N > 0 ? L2N : log2NyArg;
     const int log2KT = L2N > 0 ? L2KT : log2KTArg;
@@ -887,11 +887,23 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
     extern __shared__ __align__(16) unsigned char smem[];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
     cx<T>* tile = tw + Ny;
-    const int kx0 = blockIdx.x * KT;
-    const int lim = rowLimit ? rowLimit[blockIdx.x] : Ny;
-    const int ch = DERIV ? yp.src[blockIdx.z] : static_cast<int>(blockIdx.z);
+    // lineRemap (tile rows under 64 bytes, e.g. FP32 at Ny = 1024): neighbouring
+    // tiles, which share lines, are numbered onto one XCD and its L2
+    // (measured at 1024 x 1024: inverse 5.7 -> 2.2 ms; at 64-byte rows it
+    // cost 10 %)
+    int bx = static_cast<int>(blockIdx.x), by = static_cast<int>(blockIdx.y), bz = static_cast<int>(blockIdx.z);
+    if (lineRemap) {
+        const int X = static_cast<int>(gridDim.x), Y = static_cast<int>(gridDim.y);
+        const int l = xcdBlock(bx + X * (by + Y * bz), X * Y * static_cast<int>(gridDim.z));
+        bx = l % X;
+        by = (l / X) % Y;
+        bz = l / (X * Y);
+    }
+    const int kx0 = bx * KT;
+    const int lim = rowLimit ? rowLimit[bx] : Ny;
+    const int ch = DERIV ? yp.src[bz] : bz;
     const size_t planeStride = static_cast<size_t>(Ny) * nkxP;
-    const size_t plane = (static_cast<size_t>(ch) * Nz + blockIdx.y) * planeStride;
+    const size_t plane = (static_cast<size_t>(ch) * Nz + by) * planeStride;
     const int nel = Ny * KT;
     loadTwiddles(tw, twY, Ny);
     // all of a thread's loads are issued before its LDS stores; entries per
@@ -939,10 +951,10 @@ __global__ __launch_bounds__(kFftThreads) void k_yfft(const cx<T>* __restrict__ 
     };
     fill(false);
     transform(plane);
-    if (DERIV && yp.der[blockIdx.z] >= 0) {
+    if (DERIV && yp.der[bz] >= 0) {
         __syncthreads();  // the tile is refilled
         fill(true);
-        transform((static_cast<size_t>(yp.der[blockIdx.z]) * Nz + blockIdx.y) * planeStride);
+        transform((static_cast<size_t>(yp.der[bz]) * Nz + by) * planeStride);
     }
 }
 
@@ -3151,7 +3163,7 @@ struct KernelImpl {
         // Ny = 64 ... 1024 with their default tile widths run kernels
         // compiled for that length; others the runtime-length kernel.
         using Fn = void (*)(const cx<T>*, cx<T>*, int, int, int, int, int, int, const cx<T>*, int, const int*, int,
-                            int, YDerivPlan, const T*);
+                            int, YDerivPlan, const T*, int);
         const int l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
         const bool dv = yp != nullptr;
         Fn fn = dv ? k_yfft<T, 0, 0, true> : k_yfft<T, 0, 0, false>;
@@ -3168,7 +3180,7 @@ struct KernelImpl {
                                dim3(kFftThreads), lds, s, in.as<cx<T>>(), out.as<cx<T>>(), static_cast<int>(d.c.Ny),
                                l2y, static_cast<int>(d.nkx), static_cast<int>(d.nkxP), l2t, static_cast<int>(d.c.Nz),
                                twY.as<cx<T>>(), sign, rowLimit.as<int>(), skipIn ? 1 : 0, skipOut ? 1 : 0, plan,
-                               lrow.as<T>());
+                               lrow.as<T>(), KT * sizeof(cx<T>) < 64 ? 1 : 0);
         });
     }
 
@@ -3190,8 +3202,8 @@ struct KernelImpl {
                            zeroRow.as<cx<T>>(), es, static_cast<const double*>(nullptr));
     }
 
-    // Nx = 64 ... 512 run kernels compiled for that length; others the
-    // runtime-length kernel.
+    // Nx = 64 ... 1024 run kernels compiled for that length; others the
+    // runtime-length kernels.
     template <typename T, int NTc>
     void launchXfusedFor(hipStream_t s) {
         constexpr int pairs = XMap<NTc>::pairs;
@@ -3200,6 +3212,7 @@ struct KernelImpl {
             case 7: return launchXfusedKernel<T, NTc, xfusedRows<T>(128, pairs), 7>(s);
             case 8: return launchXfusedKernel<T, NTc, xfusedRows<T>(256, pairs), 8>(s);
             case 9: return launchXfusedKernel<T, NTc, xfusedRows<T>(512, pairs), 9>(s);
+            case 10: return launchXfusedKernel<T, NTc, 1, 10>(s);
             default: break;
         }
         static_assert(xfusedRows<T>(64, pairs) == 1, "one-row blocks at every length");
@@ -3524,8 +3537,9 @@ struct KernelImpl {
             launchXfwdRows<double>(in, C, 1, scNxh, scNxhP, twx, 0, s);
         };
         using YFn = void (*)(const cx<double>*, cx<double>*, int, int, int, int, int, int, const cx<double>*, int,
-                             const int*, int, int, YDerivPlan, const double*);
+                             const int*, int, int, YDerivPlan, const double*, int);
         YFn yfn = k_yfft<double, 0, 0, false>;
+        if (smooth7(d.c.Ny)) yfn = k_yfft<double, kSmooth7, 0, false>;
         if (l2y == 6 && l2t == 4) yfn = k_yfft<double, 6, 4, false>;
         if (l2y == 7 && l2t == 4) yfn = k_yfft<double, 7, 4, false>;
         if (l2y == 8 && l2t == 4) yfn = k_yfft<double, 8, 4, false>;
@@ -3536,7 +3550,7 @@ struct KernelImpl {
             hipLaunchKernelGGL(yfn, dim3(static_cast<unsigned>(scNxhP / KT), static_cast<unsigned>(Nz), nch),
                                dim3(kFftThreads), (static_cast<size_t>(KT) + 1) * Ny * sizeof(cx<double>), s, C, C, Ny,
                                l2y, scNxh, scNxhP, l2t, Nz, scTwY.as<cx<double>>(), sign, nullptr, 0, 0, YDerivPlan{},
-                               nullptr);
+                               nullptr, KT * sizeof(cx<double>) < 64 ? 1 : 0);
         };
         auto xinv = [&](double* out, int nch) {
             launchXinvRows<double>(C, out, nch, scNxh, scNxhP, twx, s);
