@@ -124,7 +124,10 @@ typedef struct wv_stage_timing {
 /* --- lifetime ----------------------------------------------------------- */
 /* WVTransformConstantStratificationKernel::create (...Kernel.cpp:442-491). The
  * FFT engine argument of the reference has no counterpart: the transforms are
- * this library's own HIP kernels.  `device` selects the HIP device. */
+ * this library's own HIP kernels.  `device` selects the HIP device.  Any Nx and
+ * Ny (the reference's engine takes any length) up to 4096, bounded by the LDS
+ * of one workgroup: Nx <= 1024 in FP64 and <= 2048 in the mixed forms;
+ * larger grids return WV_UNSUPPORTED_OPERATION. */
 wv_status wv_create(const wv_config* config, wv_precision precision, int device, wv_kernel** kernel);
 void wv_destroy(wv_kernel* kernel);
 /* Vertical operators of an arbitrary stratification N^2(z) with a rigid lid, as
