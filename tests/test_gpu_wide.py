@@ -136,3 +136,25 @@ def test_flux_at_ny_1024_matches_oracle(torch_gpu, prec, tol):
     err = rel_err_joint(F, o.nonlinear_flux(*A, t=450.0, t0=0.0))
     print(f"[gpu]   flux rel err {err:.2e}", flush=True)
     assert err <= tol
+
+
+@pytest.mark.parametrize("Nx,Ny", [(768, 12), (527, 12)])
+def test_fp64_rounds_of_whole_transforms_match_oracle(torch_gpu, Nx, Ny):
+    """Runtime lengths whose 8 transforms per row exceed one in-LDS FFT call:
+    768 = 3 x 256 (mixed-radix passes) and 527 = 17 x 31 (direct DFT), run in
+    rounds of whole transforms."""
+    import wvamd
+    cfg = wvamd.make_config(Nx, Ny, 9, None, Lx=300e3, Ly=10e3)
+    print(f"[gpu] rounds fp64 {Nx}x{Ny}x9", flush=True)
+    k = wvamd.WVTransformConstantStratificationKernel(cfg, "fp64")
+    o = Oracle(cfg)
+    A = _state(o, Nx + 1)
+    dev = [k.to_device(a) for a in A]
+    F = [t.cpu().numpy() for t in k.nonlinear_flux(*dev, t=450.0, t0=0.0)]
+    ferr = rel_err_joint(F, o.nonlinear_flux(*A, t=450.0, t0=0.0))
+    fields = k.transform_wave_vortex_to_uvweta(*dev, t=450.0).cpu().numpy()
+    ref = o.inverse_uvweta(*A, t=450.0)
+    ierr = float(np.abs(fields - ref).max() / np.abs(ref).max())
+    aerr = rel_err(k.apv(*dev, t=450.0).cpu().numpy(), o.apv(*A, t=450.0))
+    print(f"[gpu]   flux {ferr:.2e} inverse {ierr:.2e} apv {aerr:.2e}", flush=True)
+    assert ferr <= 1e-12 and ierr <= 1e-13 and aerr <= 1e-12

@@ -222,7 +222,7 @@ constexpr int log2Radix() { return R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4; }
 // One Stockham pass of radix R over transforms of length 2^log2N whose
 // sub-transforms so far have length 2^log2Ns.  Bit r of ZM marks input r of
 // every butterfly as known zero (not read).
-template <typename T, int R, int SIGN, typename Bt, unsigned ZM = 0>
+template <typename T, int R, int SIGN, typename Bt, unsigned ZM = 0, bool ROUNDS = true>
 __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, const Bt& bt,
                                              const cx<T>* __restrict__ tw) {
     constexpr int IPT = kFftRegs / R;
@@ -231,83 +231,106 @@ __device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, 
     const int NR = 1 << log2NR;
     const int items = batchCount(bt) << log2NR;
     const int kmask = (1 << log2Ns) - 1;
-    cx<T> v[IPT][R];
+    // items i0 + thread + it kFftThreads below itemEnd
+    auto round = [&](int i0, int itemEnd) __attribute__((always_inline)) {
+        cx<T> v[IPT][R];
 #pragma unroll
-    for (int it = 0; it < IPT; ++it) {
-        const int item = fftThread() + it * kFftThreads;
-        if (item < items) {
-            int b, jj;
-            bt.split(item, log2NR, b, jj);
-            const int k = jj & kmask;
-            const int boff = bt.base(b);
+        for (int it = 0; it < IPT; ++it) {
+            const int item = i0 + fftThread() + it * kFftThreads;
+            if (item < itemEnd) {
+                int b, jj;
+                bt.split(item, log2NR, b, jj);
+                const int k = jj & kmask;
+                const int boff = bt.base(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if ((ZM >> r) & 1u) {
-                    v[it][r] = cx<T>{T(0), T(0)};
-                    continue;
+                for (int r = 0; r < R; ++r) {
+                    if ((ZM >> r) & 1u) {
+                        v[it][r] = cx<T>{T(0), T(0)};
+                        continue;
+                    }
+                    cx<T> x = buf[bt.at(jj + r * NR) + boff];
+                    if (r > 0 && log2Ns > 0) {
+                        // the table holds the unit twiddle at k = 0 too
+                        cx<T> w = tw[(r << log2Ns) + k - 1];
+                        if (SIGN > 0) w.im = -w.im;
+                        x = cmul(x, w);
+                    }
+                    v[it][r] = x;
                 }
-                cx<T> x = buf[bt.at(jj + r * NR) + boff];
-                if (r > 0 && log2Ns > 0) {
-                    // the table holds the unit twiddle at k = 0 too
-                    cx<T> w = tw[(r << log2Ns) + k - 1];
-                    if (SIGN > 0) w.im = -w.im;
-                    x = cmul(x, w);
-                }
-                v[it][r] = x;
+                dftReg<T, R, SIGN, ZM>(v[it]);
             }
-            dftReg<T, R, SIGN, ZM>(v[it]);
         }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int it = 0; it < IPT; ++it) {
-        const int item = fftThread() + it * kFftThreads;
-        if (item < items) {
-            int b, jj;
-            bt.split(item, log2NR, b, jj);
-            const int d = ((jj >> log2Ns) << (log2Ns + LR)) + (jj & kmask);
-            const int boff = bt.base(b);
+        for (int it = 0; it < IPT; ++it) {
+            const int item = i0 + fftThread() + it * kFftThreads;
+            if (item < itemEnd) {
+                int b, jj;
+                bt.split(item, log2NR, b, jj);
+                const int d = ((jj >> log2Ns) << (log2Ns + LR)) + (jj & kmask);
+                const int boff = bt.base(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[bt.at(d + (r << log2Ns)) + boff] = v[it][r];
+                for (int r = 0; r < R; ++r) buf[bt.at(d + (r << log2Ns)) + boff] = v[it][r];
+            }
         }
+        __syncthreads();
+    };
+    // ROUNDS: transforms of more than kFftRegs * kFftThreads points together
+    // run in rounds of whole transforms (kCap is a multiple of NR; rows
+    // only).  Without, the caller holds at most that many points.
+    if constexpr (ROUNDS) {
+        constexpr int kCap = kFftThreads * IPT;
+        for (int i0 = 0; i0 < items; i0 += kCap) round(i0, items);
+    } else {
+        round(0, items);
     }
-    __syncthreads();
 }
 
-template <typename T, int SIGN, typename Bt>
+template <typename T, int SIGN, bool ROUNDS, typename Bt>
 __device__ void directDft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restrict__ tw) {
-    cx<T> out[kFftRegs];
-    const int items = batchCount(bt) * N;
+    const int nb = batchCount(bt);
     const FastDiv fN(N);
+    // transforms b0 ... below bEnd
+    auto round = [&](int b0, int bEnd) __attribute__((always_inline)) {
+        const int i0 = b0 * N, items = bEnd * N;
+        cx<T> out[kFftRegs];
 #pragma unroll
-    for (int it = 0; it < kFftRegs; ++it) {
-        const int item = fftThread() + it * kFftThreads;
-        if (item < items) {
-            int b, k;
-            bt.splitN(item, fN, b, k);
-            cx<T> acc{T(0), T(0)};
-            // twiddle index n k mod N, advanced by k per term
-            for (int n = 0, q = 0; n < N; ++n) {
-                cx<T> w = tw[q];
-                if (SIGN > 0) w.im = -w.im;
-                acc = cadd(acc, cmul(buf[bt.at(n) + bt.base(b)], w));
-                q += k;
-                if (q >= N) q -= N;
+        for (int it = 0; it < kFftRegs; ++it) {
+            const int item = i0 + fftThread() + it * kFftThreads;
+            if (item < items) {
+                int b, k;
+                bt.splitN(item, fN, b, k);
+                cx<T> acc{T(0), T(0)};
+                // twiddle index n k mod N, advanced by k per term
+                for (int n = 0, q = 0; n < N; ++n) {
+                    cx<T> w = tw[q];
+                    if (SIGN > 0) w.im = -w.im;
+                    acc = cadd(acc, cmul(buf[bt.at(n) + bt.base(b)], w));
+                    q += k;
+                    if (q >= N) q -= N;
+                }
+                out[it] = acc;
             }
-            out[it] = acc;
         }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int it = 0; it < kFftRegs; ++it) {
-        const int item = fftThread() + it * kFftThreads;
-        if (item < items) {
-            int b, k;
-            bt.splitN(item, fN, b, k);
-            buf[bt.at(k) + bt.base(b)] = out[it];
+        for (int it = 0; it < kFftRegs; ++it) {
+            const int item = i0 + fftThread() + it * kFftThreads;
+            if (item < items) {
+                int b, k;
+                bt.splitN(item, fN, b, k);
+                buf[bt.at(k) + bt.base(b)] = out[it];
+            }
         }
+        __syncthreads();
+    };
+    if constexpr (ROUNDS) {
+        // rounds of whole transforms of at most kFftRegs * kFftThreads points
+        const int per = max(1, (kFftRegs * kFftThreads) / N);
+        for (int b0 = 0; b0 < nb; b0 += per) round(b0, min(nb, b0 + per));
+    } else {
+        round(0, nb);
     }
-    __syncthreads();
 }
 
 // cos(2 pi m / P) and sin(2 pi m / P), m = 0 ... (P - 1)/2, for the odd
@@ -384,12 +407,62 @@ __device__ __forceinline__ void dftOddPrime(cx<T>* v) {
     for (int k = 0; k < P; ++k) v[k] = out[k];
 }
 
+// Composite radices of the mixed-radix passes: R = P1 P2 with coprime
+// factors (6 = 3 2, 10 = 5 2, 12 = 3 4, 14 = 7 2, 15 = 3 5), P1 the odd one
+// (3 for 15).
+template <int R>
+constexpr int pfaFirst() { return R == 15 ? 3 : (R & 1) ? R : pfaFirst<R / 2>(); }
+template <int R>
+constexpr bool isPrimeRadix() { return R == 3 || R == 5 || R == 7 || R == 11 || R == 13; }
+constexpr int inverseMod(int a, int m) {
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 0;
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dftSmall(cx<T>* v);
+
+// In-register DFT of R = P1 P2 points (gcd(P1, P2) = 1), natural order in
+// and out, by the prime-factor algorithm: with the input map n = (P2 n1 +
+// P1 n2) mod R and the output map k = (e1 k1 + e2 k2) mod R, e1 = 1 mod P1
+// and 0 mod P2, e2 = 0 mod P1 and 1 mod P2, the DFT is a DFT of P1 points
+// along n1 followed by one of P2 points along n2, with no twiddles between.
+template <typename T, int P1, int P2, int SIGN>
+__device__ __forceinline__ void dftPfa(cx<T>* v) {
+    constexpr int R = P1 * P2;
+    constexpr int e1 = P2 * inverseMod(P2 % P1, P1), e2 = P1 * inverseMod(P1 % P2, P2);
+    static_assert(e1 % P1 == 1 && e1 % P2 == 0 && e2 % P1 == 0 && e2 % P2 == 1, "coprime factors");
+    cx<T> a[P1][P2];
+#pragma unroll
+    for (int n1 = 0; n1 < P1; ++n1)
+#pragma unroll
+        for (int n2 = 0; n2 < P2; ++n2) a[n1][n2] = v[(P2 * n1 + P1 * n2) % R];
+#pragma unroll
+    for (int n2 = 0; n2 < P2; ++n2) {
+        cx<T> t[P1];
+#pragma unroll
+        for (int n1 = 0; n1 < P1; ++n1) t[n1] = a[n1][n2];
+        dftSmall<T, P1, SIGN>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < P1; ++k1) a[k1][n2] = t[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P1; ++k1) {
+        dftSmall<T, P2, SIGN>(a[k1]);
+#pragma unroll
+        for (int k2 = 0; k2 < P2; ++k2) v[(e1 * k1 + e2 * k2) % R] = a[k1][k2];
+    }
+}
+
 template <typename T, int R, int SIGN>
 __device__ __forceinline__ void dftSmall(cx<T>* v) {
     if constexpr ((R & (R - 1)) == 0)
         dftReg<T, R, SIGN>(v);
-    else
+    else if constexpr (isPrimeRadix<R>())
         dftOddPrime<T, R, SIGN>(v);
+    else
+        dftPfa<T, pfaFirst<R>(), R / pfaFirst<R>(), SIGN>(v);
 }
 
 // One Stockham pass of radix R (2, 4, 8, 16 or an odd prime up to 13) over
@@ -398,59 +471,77 @@ __device__ __forceinline__ void dftSmall(cx<T>* v) {
 // butterfly of item (transform, j), k = j mod Ns, is multiplied by
 // exp(-2 pi i r k / (Ns R)) = tw[r k N / (Ns R)] (index < N), and its
 // outputs go to (j / Ns) Ns R + k + r Ns.  A pass holds at most about
-// kFftRegs complex values per thread: ceil(16 / R) butterflies.
-template <typename T, int R, int SIGN, typename Bt>
+// kFftRegs complex values per thread: ceil(16 / R) butterflies (IPT; the
+// composite radices run one butterfly per thread).
+template <typename T, int R, int SIGN, typename Bt, int IPT = (kFftRegs + R - 1) / R, bool ROUNDS = true>
 __device__ __forceinline__ void stockhamPassAny(cx<T>* buf, int N, int Ns, const Bt& bt,
                                                 const cx<T>* __restrict__ tw) {
-    constexpr int IPT = (kFftRegs + R - 1) / R;
     const int NR = N / R;
-    const int items = batchCount(bt) * NR;
+    const int nb = batchCount(bt);
     const int step = N / (Ns * R);
     const FastDiv fNR(NR), fNs(Ns);
-    cx<T> v[IPT][R];
-    int dst[IPT];  // output position (j / Ns) Ns R + k of each item
+    // transforms b0 ... below bEnd
+    auto round = [&](int b0, int bEnd) __attribute__((always_inline)) {
+        const int i0 = b0 * NR, items = bEnd * NR;
+        cx<T> v[IPT][R];
+        int dst[IPT];  // output position (j / Ns) Ns R + k of each item
 #pragma unroll
-    for (int it = 0; it < IPT; ++it) {
-        const int item = fftThread() + it * kFftThreads;
-        if (item < items) {
-            int b, jj, q, k;
-            bt.splitN(item, fNR, b, jj);
-            fNs.divmod(jj, q, k);
-            const int boff = bt.base(b);
-            dst[it] = __mul24(q, Ns * R) + k;
-            const int ks = __mul24(k, step);
+        for (int it = 0; it < IPT; ++it) {
+            const int item = i0 + fftThread() + it * kFftThreads;
+            if (item < items) {
+                int b, jj, q, k;
+                bt.splitN(item, fNR, b, jj);
+                fNs.divmod(jj, q, k);
+                const int boff = bt.base(b);
+                dst[it] = __mul24(q, Ns * R) + k;
+                const int ks = __mul24(k, step);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                cx<T> x = buf[bt.at(jj + r * NR) + boff];
-                if (r > 0 && Ns > 1) {
-                    cx<T> w = tw[r * ks];
-                    if (SIGN > 0) w.im = -w.im;
-                    x = cmul(x, w);
+                for (int r = 0; r < R; ++r) {
+                    cx<T> x = buf[bt.at(jj + r * NR) + boff];
+                    if (r > 0 && Ns > 1) {
+                        cx<T> w = tw[r * ks];
+                        if (SIGN > 0) w.im = -w.im;
+                        x = cmul(x, w);
+                    }
+                    v[it][r] = x;
                 }
-                v[it][r] = x;
+                dftSmall<T, R, SIGN>(v[it]);
             }
-            dftSmall<T, R, SIGN>(v[it]);
         }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int it = 0; it < IPT; ++it) {
-        const int item = fftThread() + it * kFftThreads;
-        if (item < items) {
-            int b, jj;
-            bt.splitN(item, fNR, b, jj);
-            const int boff = bt.base(b);
+        for (int it = 0; it < IPT; ++it) {
+            const int item = i0 + fftThread() + it * kFftThreads;
+            if (item < items) {
+                int b, jj;
+                bt.splitN(item, fNR, b, jj);
+                const int boff = bt.base(b);
 #pragma unroll
-            for (int r = 0; r < R; ++r) buf[bt.at(dst[it] + r * Ns) + boff] = v[it][r];
+                for (int r = 0; r < R; ++r) buf[bt.at(dst[it] + r * Ns) + boff] = v[it][r];
+            }
         }
+        __syncthreads();
+    };
+    if constexpr (ROUNDS) {
+        // rounds of whole transforms, at most kFftThreads IPT items each
+        // (rows only)
+        const int per = max(1, (kFftThreads * IPT) / NR);
+        for (int b0 = 0; b0 < nb; b0 += per) round(b0, min(nb, b0 + per));
+    } else {
+        round(0, nb);
     }
-    __syncthreads();
 }
 
-// Mixed-radix FFT of a runtime length N (passes of 3, 5, 7, 11, 13, then
-// radix-16 passes for the power of two and 8/4/2 for its remainder) with the full
-// twiddle table; false (nothing done) when N has a prime factor above 13.
-template <typename T, int SIGN, int MAXP = 13, typename Bt>
+template <typename T, int R, int SIGN, bool ROUNDS, typename Bt>
+__device__ __forceinline__ void passAny(cx<T>* buf, int N, int Ns, const Bt& bt, const cx<T>* __restrict__ tw) {
+    stockhamPassAny<T, R, SIGN, Bt, (kFftRegs + R - 1) / R, ROUNDS>(buf, N, Ns, bt, tw);
+}
+
+// Mixed-radix FFT of a runtime length N (passes of 15, 6, 12, 10 or 14 where
+// they apply, of 3, 5, 7, 11, 13, then radix-16 passes for the power of two
+// and 8/4/2 for its remainder) with the full twiddle table; false (nothing
+// done) when N has a prime factor above 13.
+template <typename T, int SIGN, int MAXP = 13, bool ROUNDS = true, typename Bt>
 __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restrict__ tw) {
     int rest = N, p2 = 0, e3 = 0, e5 = 0, e7 = 0, e11 = 0, e13 = 0;
     while ((rest & 1) == 0) { rest >>= 1; ++p2; }
@@ -461,19 +552,40 @@ __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, c
     while (rest % 13 == 0) { rest /= 13; ++e13; }
     if (rest != 1 || (MAXP < 11 && e11 + e13 > 0)) return false;
     // odd radices first: the power-of-two passes then scatter with an odd
-    // stride Ns (no LDS bank conflicts in an unpadded layout)
+    // stride Ns (no LDS bank conflicts in an unpadded layout).  Where every
+    // butterfly of a pass has its own thread, pairs of factors run as one
+    // composite pass (prime-factor DFTs in registers, one pass fewer): 3 x 5,
+    // then the power of two's remainder beyond its radix-16 passes with an
+    // odd factor (6, 12, 10, 14).
     int Ns = 1;
-    for (; e3 > 0; --e3, Ns *= 3) stockhamPassAny<T, 3, SIGN>(buf, N, Ns, bt, tw);
-    for (; e5 > 0; --e5, Ns *= 5) stockhamPassAny<T, 5, SIGN>(buf, N, Ns, bt, tw);
-    for (; e7 > 0; --e7, Ns *= 7) stockhamPassAny<T, 7, SIGN>(buf, N, Ns, bt, tw);
-    if constexpr (MAXP >= 11) {
-        for (; e11 > 0; --e11, Ns *= 11) stockhamPassAny<T, 11, SIGN>(buf, N, Ns, bt, tw);
-        for (; e13 > 0; --e13, Ns *= 13) stockhamPassAny<T, 13, SIGN>(buf, N, Ns, bt, tw);
+    const int nb = batchCount(bt);
+    auto single = [&](int R) { return nb * (N / R) <= kFftThreads; };
+    for (; e3 > 0 && e5 > 0 && single(15); --e3, --e5, Ns *= 15)
+        stockhamPassAny<T, 15, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
+    if ((p2 & 3) == 1 && e3 > 0 && single(6)) {
+        stockhamPassAny<T, 6, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
+        --e3, --p2, Ns *= 6;
+    } else if ((p2 & 3) == 2 && e3 > 0 && single(12)) {
+        stockhamPassAny<T, 12, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
+        --e3, p2 -= 2, Ns *= 12;
+    } else if ((p2 & 3) == 1 && e5 > 0 && single(10)) {
+        stockhamPassAny<T, 10, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
+        --e5, --p2, Ns *= 10;
+    } else if ((p2 & 3) == 1 && e7 > 0 && single(14)) {
+        stockhamPassAny<T, 14, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
+        --e7, --p2, Ns *= 14;
     }
-    for (; p2 >= 4; p2 -= 4, Ns *= 16) stockhamPassAny<T, 16, SIGN>(buf, N, Ns, bt, tw);
-    if (p2 == 3) stockhamPassAny<T, 8, SIGN>(buf, N, Ns, bt, tw);
-    if (p2 == 2) stockhamPassAny<T, 4, SIGN>(buf, N, Ns, bt, tw);
-    if (p2 == 1) stockhamPassAny<T, 2, SIGN>(buf, N, Ns, bt, tw);
+    for (; e3 > 0; --e3, Ns *= 3) passAny<T, 3, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
+    for (; e5 > 0; --e5, Ns *= 5) passAny<T, 5, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
+    for (; e7 > 0; --e7, Ns *= 7) passAny<T, 7, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
+    if constexpr (MAXP >= 11) {
+        for (; e11 > 0; --e11, Ns *= 11) passAny<T, 11, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
+        for (; e13 > 0; --e13, Ns *= 13) passAny<T, 13, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
+    }
+    for (; p2 >= 4; p2 -= 4, Ns *= 16) passAny<T, 16, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
+    if (p2 == 3) passAny<T, 8, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
+    if (p2 == 2) passAny<T, 4, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
+    if (p2 == 1) passAny<T, 2, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
     return true;
 }
 
@@ -488,38 +600,48 @@ __device__ __forceinline__ void radixPasses(cx<T>* buf, const Bt& bt, const cx<T
 
 // FFT of a runtime power-of-two length 2^log2N: the leading pass takes the
 // factor 2^(log2N mod 4); the rest are radix 16.
-template <typename T, int SIGN, typename Bt>
+template <typename T, int SIGN, bool ROUNDS = true, typename Bt>
 __device__ __forceinline__ void fftLdsPow2(cx<T>* buf, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
     int l2s = 0;
     switch (log2N & 3) {
-        case 1: stockhamPass<T, 2, SIGN>(buf, log2N, l2s, bt, tw); l2s = 1; break;
-        case 2: stockhamPass<T, 4, SIGN>(buf, log2N, l2s, bt, tw); l2s = 2; break;
-        case 3: stockhamPass<T, 8, SIGN>(buf, log2N, l2s, bt, tw); l2s = 3; break;
+        case 1: stockhamPass<T, 2, SIGN, Bt, 0, ROUNDS>(buf, log2N, l2s, bt, tw); l2s = 1; break;
+        case 2: stockhamPass<T, 4, SIGN, Bt, 0, ROUNDS>(buf, log2N, l2s, bt, tw); l2s = 2; break;
+        case 3: stockhamPass<T, 8, SIGN, Bt, 0, ROUNDS>(buf, log2N, l2s, bt, tw); l2s = 3; break;
         default: break;
     }
-    for (; l2s < log2N; l2s += 4) stockhamPass<T, 16, SIGN>(buf, log2N, l2s, bt, tw);
+    for (; l2s < log2N; l2s += 4) stockhamPass<T, 16, SIGN, Bt, 0, ROUNDS>(buf, log2N, l2s, bt, tw);
 }
 
 // FFT of a length that is not a power of two: mixed-radix passes when its
 // prime factors are at most 13, else the direct DFT.  SMOOTH7: the caller
 // has checked that they are at most 7 (only the passes of 2 ... 16, 3, 5
 // and 7 are compiled: fewer registers).
-template <typename T, int SIGN, bool SMOOTH7 = false, typename Bt>
+template <typename T, int SIGN, bool SMOOTH7 = false, bool ROUNDS = true, typename Bt>
 __device__ __forceinline__ void fftLdsAny(cx<T>* buf, int N, const Bt& bt, const cx<T>* __restrict__ tw) {
     if constexpr (SMOOTH7)
-        mixedRadixFft<T, SIGN, 7>(buf, N, bt, tw);
-    else if (!mixedRadixFft<T, SIGN>(buf, N, bt, tw))
-        directDft<T, SIGN>(buf, N, bt, tw);
+        mixedRadixFft<T, SIGN, 7, ROUNDS>(buf, N, bt, tw);
+    else if (!mixedRadixFft<T, SIGN, 13, ROUNDS>(buf, N, bt, tw))
+        directDft<T, SIGN, ROUNDS>(buf, N, bt, tw);
 }
 
 // Kernels take the length class as a template argument L2N: L2N > 0 fixes
 // the length 2^L2N at compile time (N and log2N are then ignored): every
 // pass and every index shift is a constant, and the passes are of radix
 // 2^LR (LR = 4 or 2; the leading pass takes the remaining factor) with the
-// twiddle table of that pass sequence.  L2N = 0: any runtime length;
-// L2N = kSmooth7: a runtime length whose prime factors are at most 7.
-// log2N < 0 marks a length that is not a power of two.
+// twiddle table of that pass sequence.  Runtime lengths: L2N = 0 any
+// length, kSmooth7Rounds a length whose prime factors are at most 7, both
+// in passes written as rounds of whole transforms (any number of points;
+// measured to take fewer registers in the y kernels); kAnyLength and
+// kSmooth7 the same in passes of one round (at most kFftRegs * kFftThreads
+// points; fewer registers in the x kernels).  log2N < 0 marks a length
+// that is not a power of two.
 constexpr int kSmooth7 = -1;
+constexpr int kAnyLength = -2;
+constexpr int kSmooth7Rounds = -3;
+template <int L2N>
+constexpr bool fftRounds() { return L2N >= 0 || L2N == kSmooth7Rounds; }
+template <int L2N>
+constexpr bool fftSmooth7() { return L2N == kSmooth7 || L2N == kSmooth7Rounds; }
 template <typename T, int SIGN, int L2N = 0, int LR = 4, typename Bt>
 __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
     if constexpr (L2N > 0) {
@@ -529,9 +651,9 @@ __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& b
         return;
     }
     if (log2N < 0)
-        fftLdsAny<T, SIGN, L2N == kSmooth7>(buf, N, bt, tw);
+        fftLdsAny<T, SIGN, fftSmooth7<L2N>(), fftRounds<L2N>()>(buf, N, bt, tw);
     else
-        fftLdsPow2<T, SIGN>(buf, log2N, bt, tw);
+        fftLdsPow2<T, SIGN, fftRounds<L2N>()>(buf, log2N, bt, tw);
 }
 
 // Row transforms of the x kernels (transform b at row b / NB1, slot b %
@@ -540,36 +662,28 @@ __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& b
 // plain one (n; their odd-radix passes come first, and the addresses of the
 // generic passes stay linear).  rowsPadded(log2N) says which.
 __device__ __forceinline__ bool rowsPadded(int log2N) { return log2N >= 0; }
-// A runtime length above 4096 / nb points runs in calls of at most
-// kFftRegs * kFftThreads points (transforms b0 ... at buf + b0 sb; the
-// caller then holds a single row, nb <= NB1).
+// (More than kFftRegs * kFftThreads points in all: every pass runs in rounds
+// of whole transforms.)
 template <typename T, int SIGN, int L2N, int NB1>
 __device__ __forceinline__ void fftRows(cx<T>* buf, int N, int log2N, int nb, int sb, int sb2,
                                         const cx<T>* __restrict__ tw) {
     if constexpr (L2N > 0) {
-        constexpr int per = (kFftRegs * kFftThreads) >> L2N;  // transforms per call
-        for (int b0 = 0; b0 < nb; b0 += per)
-            fftLds<T, SIGN, L2N>(buf + b0 * sb, N, log2N, BatchRows<NB1, true>{min(per, nb - b0), sb, sb2}, tw);
+        fftLds<T, SIGN, L2N>(buf, N, log2N, BatchRows<NB1, true>{nb, sb, sb2}, tw);
     } else {
-        const int per = nb * N <= kFftRegs * kFftThreads ? nb : max(1, kFftRegs * kFftThreads / N);
-        for (int b0 = 0; b0 < nb; b0 += per) {
-            const int n = min(per, nb - b0);
-            if (rowsPadded(log2N))
-                fftLdsPow2<T, SIGN>(buf + b0 * sb, log2N, BatchRows<NB1, true>{n, sb, sb2}, tw);
-            else
-                fftLdsAny<T, SIGN, L2N == kSmooth7>(buf + b0 * sb, N, BatchRows<NB1, false>{n, sb, sb2}, tw);
-        }
+        if (rowsPadded(log2N))
+            fftLdsPow2<T, SIGN, fftRounds<L2N>()>(buf, log2N, BatchRows<NB1, true>{nb, sb, sb2}, tw);
+        else
+            fftLdsAny<T, SIGN, fftSmooth7<L2N>(), fftRounds<L2N>()>(buf, N, BatchRows<NB1, false>{nb, sb, sb2},
+                                                                 tw);
     }
 }
 
-// nb unit-stride padded transforms at buf + b stride, in calls of at most
-// kFftRegs * kFftThreads points (lengths above 4096 / nb).
+// nb unit-stride padded transforms at buf + b stride (any number of points:
+// the passes run in rounds of whole transforms).
 template <typename T, int SIGN, int L2N = 0>
 __device__ __forceinline__ void fftLdsBatched(cx<T>* buf, int N, int log2N, int nb, int stride,
                                               const cx<T>* __restrict__ tw) {
-    const int per = nb * N <= kFftRegs * kFftThreads ? nb : max(1, kFftRegs * kFftThreads / N);
-    for (int b0 = 0; b0 < nb; b0 += per)
-        fftLds<T, SIGN, L2N>(buf + b0 * stride, N, log2N, BatchRows<1>{min(per, nb - b0), 0, stride}, tw);
+    fftLds<T, SIGN, L2N>(buf, N, log2N, BatchRows<1>{nb, 0, stride}, tw);
 }
 
 // Forward (SIGN = -1) FFT of NB 256-point transforms (unit stride, padded,

@@ -863,8 +863,8 @@ __global__ __launch_bounds__(256) void k_synth16(const cx<float>* __restrict__ F
 // rows r < iy < Ny - r are known zeros (skipIn) or output rows there are not
 // needed (skipOut).  L2N > 0 fixes Ny = 2^L2N and the tile width 2^L2KT at
 // compile time (the runtime Ny, log2Ny and log2KT are then ignored);
-// L2N = kSmooth7 takes runtime lengths whose prime factors are at most 7,
-// L2N = 0 any length.
+// L2N <= 0 takes runtime lengths (kSmooth7, kSmooth7Rounds: prime factors
+// at most 7).
 // Channel plan of the inverse y transform with y derivatives: block z takes
 // input channel src[z] to output channel src[z] and, when der[z] >= 0, also
 // i l times it to output channel der[z] (l = lrow[iy], the y wavenumber of
@@ -1059,8 +1059,8 @@ struct XMap<5> {
 // per complex FFT), the NT advection products and r2c of the products.
 // kxv[k] = the x wavenumber of half-spectrum column k.  L2N > 0 fixes
 // Nx = 2^L2N at compile time (the runtime Nx and log2Nx are then ignored);
-// L2N = kSmooth7 takes runtime lengths whose prime factors are at most 7,
-// L2N = 0 any length.
+// L2N <= 0 takes runtime lengths (kSmooth7, kSmooth7Rounds: prime factors
+// at most 7).
 // The forward transforms run the same radix-16 passes as the inverse ones:
 // at Nx = 256 with two threads per butterfly (fftFwd256, one full wave for
 // the two transforms of a row), at other lengths on the first 16 threads per
@@ -3167,7 +3167,7 @@ struct KernelImpl {
         const int l2y = log2Exact(d.c.Ny), l2t = log2Exact(KT);
         const bool dv = yp != nullptr;
         Fn fn = dv ? k_yfft<T, 0, 0, true> : k_yfft<T, 0, 0, false>;
-        if (smooth7(d.c.Ny)) fn = dv ? k_yfft<T, kSmooth7, 0, true> : k_yfft<T, kSmooth7, 0, false>;
+        if (smooth7(d.c.Ny)) fn = dv ? k_yfft<T, kSmooth7Rounds, 0, true> : k_yfft<T, kSmooth7Rounds, 0, false>;
         if (l2y == 6 && l2t == 4) fn = dv ? k_yfft<T, 6, 4, true> : k_yfft<T, 6, 4, false>;
         if (l2y == 7 && l2t == 4) fn = dv ? k_yfft<T, 7, 4, true> : k_yfft<T, 7, 4, false>;
         if (l2y == 8 && l2t == 4) fn = dv ? k_yfft<T, 8, 4, true> : k_yfft<T, 8, 4, false>;
@@ -3215,11 +3215,15 @@ struct KernelImpl {
             case 10: return launchXfusedKernel<T, NTc, 1, 10>(s);
             default: break;
         }
+        // (a row's transforms within one round of the in-LDS FFTs: the
+        // single-round kernels; else passes in rounds)
         static_assert(xfusedRows<T>(64, pairs) == 1, "one-row blocks at every length");
-        if (smooth7(d.c.Nx))
+        if (static_cast<size_t>(pairs) * d.c.Nx > static_cast<size_t>(kFftRegs) * kFftThreads)
+            launchXfusedKernel<T, NTc, 1, 0>(s);
+        else if (smooth7(d.c.Nx))
             launchXfusedKernel<T, NTc, 1, kSmooth7>(s);
         else
-            launchXfusedKernel<T, NTc, 1, 0>(s);
+            launchXfusedKernel<T, NTc, 1, kAnyLength>(s);
     }
 
     // The products with the caller's advecting fields (runtime-length
@@ -3539,7 +3543,7 @@ struct KernelImpl {
         using YFn = void (*)(const cx<double>*, cx<double>*, int, int, int, int, int, int, const cx<double>*, int,
                              const int*, int, int, YDerivPlan, const double*, int);
         YFn yfn = k_yfft<double, 0, 0, false>;
-        if (smooth7(d.c.Ny)) yfn = k_yfft<double, kSmooth7, 0, false>;
+        if (smooth7(d.c.Ny)) yfn = k_yfft<double, kSmooth7Rounds, 0, false>;
         if (l2y == 6 && l2t == 4) yfn = k_yfft<double, 6, 4, false>;
         if (l2y == 7 && l2t == 4) yfn = k_yfft<double, 7, 4, false>;
         if (l2y == 8 && l2t == 4) yfn = k_yfft<double, 8, 4, false>;
@@ -3583,10 +3587,11 @@ struct KernelImpl {
         if (antialias) {
             xfwd(rhs);
             using MFn = void (*)(cx<double>*, int, int, int, int, int, const cx<double>*, const double*);
-            MFn mfn = k_ymask<0, 0>;
+            MFn mfn = smooth7(d.c.Ny) ? k_ymask<kSmooth7Rounds, 0> : k_ymask<0, 0>;
             if (l2y == 6 && l2t == 4) mfn = k_ymask<6, 4>;
             if (l2y == 7 && l2t == 4) mfn = k_ymask<7, 4>;
             if (l2y == 8 && l2t == 4) mfn = k_ymask<8, 4>;
+            if (l2y == 8 && l2t == 3) mfn = k_ymask<8, 3>;
             if (l2y == 9 && l2t == 3) mfn = k_ymask<9, 3>;
             if (l2y == 10 && l2t == 2) mfn = k_ymask<10, 2>;
             hipLaunchKernelGGL(mfn, dim3(static_cast<unsigned>(scNxhP / KT), static_cast<unsigned>(Nz)),
