@@ -631,12 +631,11 @@ __device__ __forceinline__ void fftLdsAny(cx<T>* buf, int N, const Bt& bt, const
 // twiddle table of that pass sequence.  Runtime lengths: L2N = 0 any
 // length, kSmooth7Rounds a length whose prime factors are at most 7, both
 // in passes written as rounds of whole transforms (any number of points;
-// measured to take fewer registers in the y kernels); kAnyLength and
-// kSmooth7 the same in passes of one round (at most kFftRegs * kFftThreads
-// points; fewer registers in the x kernels).  log2N < 0 marks a length
-// that is not a power of two.
+// measured to take fewer registers in the y kernels); kSmooth7 the same in
+// passes of one round (at most kFftRegs * kFftThreads points; fewer
+// registers in the x kernels).  log2N < 0 marks a length that is not a
+// power of two.
 constexpr int kSmooth7 = -1;
-constexpr int kAnyLength = -2;
 constexpr int kSmooth7Rounds = -3;
 template <int L2N>
 constexpr bool fftRounds() { return L2N >= 0 || L2N == kSmooth7Rounds; }

@@ -3215,15 +3215,13 @@ struct KernelImpl {
             case 10: return launchXfusedKernel<T, NTc, 1, 10>(s);
             default: break;
         }
-        // (a row's transforms within one round of the in-LDS FFTs: the
-        // single-round kernels; else passes in rounds)
+        // a 7-smooth length whose row transforms fit one round of the in-LDS
+        // FFTs: the single-round kernel; else passes in rounds
         static_assert(xfusedRows<T>(64, pairs) == 1, "one-row blocks at every length");
-        if (static_cast<size_t>(pairs) * d.c.Nx > static_cast<size_t>(kFftRegs) * kFftThreads)
-            launchXfusedKernel<T, NTc, 1, 0>(s);
-        else if (smooth7(d.c.Nx))
+        if (static_cast<size_t>(pairs) * d.c.Nx <= static_cast<size_t>(kFftRegs) * kFftThreads && smooth7(d.c.Nx))
             launchXfusedKernel<T, NTc, 1, kSmooth7>(s);
         else
-            launchXfusedKernel<T, NTc, 1, kAnyLength>(s);
+            launchXfusedKernel<T, NTc, 1, 0>(s);
     }
 
     // The products with the caller's advecting fields (runtime-length
