@@ -1,5 +1,6 @@
 // Adjoint (vector-Jacobian) kernels of the nonlinear flux.  Included by
-// wv_kernels.hip after the forward kernels, whose data structures they use.
+// wv_kernels_device.hpp after the forward kernels, whose data structures they
+// use.
 //
 // The flux is F(y) = P(products(S y, S y)) with S the synthesis (coefficients
 // -> spatial u, v, w, eta and derivatives, linear) and P the projection
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(256) void k_fieldsadj(const cx<T>* __restrict__ col
 
 // out_a = alpha x_a + beta y_a for the three coefficient arrays
 // (x, y, out may alias elementwise).
-__global__ __launch_bounds__(256) void k_axpby3(const cx<double>* x0, const cx<double>* x1, const cx<double>* x2,
+static __global__ __launch_bounds__(256) void k_axpby3(const cx<double>* x0, const cx<double>* x1, const cx<double>* x2,
                                                 const cx<double>* y0, const cx<double>* y1, const cx<double>* y2,
                                                 double alpha, double beta, int M, cx<double>* o0, cx<double>* o1,
                                                 cx<double>* o2) {
