@@ -46,6 +46,10 @@ def _kernel(cfg, precision="fp64"):
 
 @pytest.mark.parametrize("N,Nz,Lx,hydro,nsteps", [(16, 9, 50e3, False, 6), (16, 9, 50e3, True, 6),
                                                   (32, 17, 50e3, False, 4),
+                                                  # runtime lengths: composite passes (60 = 15 x 4,
+                                                  # 96 = 6 x 16) and the direct DFT (19)
+                                                  (60, 9, 50e3, False, 3), (96, 17, 60e3, True, 2),
+                                                  (19, 9, 50e3, False, 3),
                                                   # Nx = 256: the x kernel's 256-point forward transforms
                                                   (256, 9, 200e3, False, 2), (256, 9, 200e3, True, 2)])
 def test_rk4_fp64_per_step_matches_oracle_small(torch_gpu, N, Nz, Lx, hydro, nsteps):
