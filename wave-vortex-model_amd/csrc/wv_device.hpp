@@ -119,7 +119,9 @@ __host__ __device__ constexpr int ldsLength(int N) { return N + (N >> 4); }
 // raised by 2^-16: that covers the reciprocal's 1-ulp error and the
 // product's rounding, and floor(a (1/d)(1 + 2^-16 +- 2^-22)) = floor(a / d)
 // while (a / d + 1) d (2^-16 + 2^-21) < 1.  Checked exhaustively over that
-// range with the reciprocal rounded down, to nearest and up.
+// range with the reciprocal rounded down, to nearest and up.  The FFT
+// passes divide item indices below (transforms per workgroup) x N <= 8 N and
+// positions below N by N or a divisor of it; setup keeps 9 Nx < 2^15.
 struct FastDiv {
     int d;
     float inv;
@@ -219,70 +221,75 @@ __device__ __forceinline__ void dftReg(cx<T>* v) {
 template <int R>
 constexpr int log2Radix() { return R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4; }
 
-// One Stockham pass of radix R over transforms of length 2^log2N whose
-// sub-transforms so far have length 2^log2Ns.  Bit r of ZM marks input r of
-// every butterfly as known zero (not read).
-template <typename T, int R, int SIGN, typename Bt, unsigned ZM = 0, bool ROUNDS = true>
-__device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, const Bt& bt,
-                                             const cx<T>* __restrict__ tw) {
+// Items i0 + thread + it kFftThreads below itemEnd of one Stockham pass of
+// radix R (stockhamPass).
+template <typename T, int R, int SIGN, typename Bt, unsigned ZM>
+__device__ __forceinline__ void stockhamRound(cx<T>* buf, int log2N, int log2Ns, const Bt& bt,
+                                              const cx<T>* __restrict__ tw, int i0, int itemEnd) {
     constexpr int IPT = kFftRegs / R;
     constexpr int LR = log2Radix<R>();
     const int log2NR = log2N - LR;
     const int NR = 1 << log2NR;
-    const int items = batchCount(bt) << log2NR;
     const int kmask = (1 << log2Ns) - 1;
-    // items i0 + thread + it kFftThreads below itemEnd
-    auto round = [&](int i0, int itemEnd) __attribute__((always_inline)) {
-        cx<T> v[IPT][R];
+    cx<T> v[IPT][R];
 #pragma unroll
-        for (int it = 0; it < IPT; ++it) {
-            const int item = i0 + fftThread() + it * kFftThreads;
-            if (item < itemEnd) {
-                int b, jj;
-                bt.split(item, log2NR, b, jj);
-                const int k = jj & kmask;
-                const int boff = bt.base(b);
+    for (int it = 0; it < IPT; ++it) {
+        const int item = i0 + fftThread() + it * kFftThreads;
+        if (item < itemEnd) {
+            int b, jj;
+            bt.split(item, log2NR, b, jj);
+            const int k = jj & kmask;
+            const int boff = bt.base(b);
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if ((ZM >> r) & 1u) {
-                        v[it][r] = cx<T>{T(0), T(0)};
-                        continue;
-                    }
-                    cx<T> x = buf[bt.at(jj + r * NR) + boff];
-                    if (r > 0 && log2Ns > 0) {
-                        // the table holds the unit twiddle at k = 0 too
-                        cx<T> w = tw[(r << log2Ns) + k - 1];
-                        if (SIGN > 0) w.im = -w.im;
-                        x = cmul(x, w);
-                    }
-                    v[it][r] = x;
+            for (int r = 0; r < R; ++r) {
+                if ((ZM >> r) & 1u) {
+                    v[it][r] = cx<T>{T(0), T(0)};
+                    continue;
                 }
-                dftReg<T, R, SIGN, ZM>(v[it]);
+                cx<T> x = buf[bt.at(jj + r * NR) + boff];
+                if (r > 0 && log2Ns > 0) {
+                    // the table holds the unit twiddle at k = 0 too
+                    cx<T> w = tw[(r << log2Ns) + k - 1];
+                    if (SIGN > 0) w.im = -w.im;
+                    x = cmul(x, w);
+                }
+                v[it][r] = x;
             }
+            dftReg<T, R, SIGN, ZM>(v[it]);
         }
-        __syncthreads();
+    }
+    __syncthreads();
 #pragma unroll
-        for (int it = 0; it < IPT; ++it) {
-            const int item = i0 + fftThread() + it * kFftThreads;
-            if (item < itemEnd) {
-                int b, jj;
-                bt.split(item, log2NR, b, jj);
-                const int d = ((jj >> log2Ns) << (log2Ns + LR)) + (jj & kmask);
-                const int boff = bt.base(b);
+    for (int it = 0; it < IPT; ++it) {
+        const int item = i0 + fftThread() + it * kFftThreads;
+        if (item < itemEnd) {
+            int b, jj;
+            bt.split(item, log2NR, b, jj);
+            const int d = ((jj >> log2Ns) << (log2Ns + LR)) + (jj & kmask);
+            const int boff = bt.base(b);
 #pragma unroll
-                for (int r = 0; r < R; ++r) buf[bt.at(d + (r << log2Ns)) + boff] = v[it][r];
-            }
+            for (int r = 0; r < R; ++r) buf[bt.at(d + (r << log2Ns)) + boff] = v[it][r];
         }
-        __syncthreads();
-    };
-    // ROUNDS: transforms of more than kFftRegs * kFftThreads points together
-    // run in rounds of whole transforms (kCap is a multiple of NR; rows
-    // only).  Without, the caller holds at most that many points.
+    }
+    __syncthreads();
+}
+
+// One Stockham pass of radix R over transforms of length 2^log2N whose
+// sub-transforms so far have length 2^log2Ns.  Bit r of ZM marks input r of
+// every butterfly as known zero (not read).  ROUNDS: transforms of more than
+// kFftRegs * kFftThreads points together run in rounds of whole transforms
+// (kCap is a multiple of NR; rows only).  Without, the caller holds at most
+// that many points.
+template <typename T, int R, int SIGN, typename Bt, unsigned ZM = 0, bool ROUNDS = true>
+__device__ __forceinline__ void stockhamPass(cx<T>* buf, int log2N, int log2Ns, const Bt& bt,
+                                             const cx<T>* __restrict__ tw) {
+    const int items = batchCount(bt) << (log2N - log2Radix<R>());
     if constexpr (ROUNDS) {
-        constexpr int kCap = kFftThreads * IPT;
-        for (int i0 = 0; i0 < items; i0 += kCap) round(i0, items);
+        constexpr int kCap = kFftThreads * (kFftRegs / R);
+        for (int i0 = 0; i0 < items; i0 += kCap)
+            stockhamRound<T, R, SIGN, Bt, ZM>(buf, log2N, log2Ns, bt, tw, i0, items);
     } else {
-        round(0, items);
+        stockhamRound<T, R, SIGN, Bt, ZM>(buf, log2N, log2Ns, bt, tw, 0, items);
     }
 }
 
@@ -590,11 +597,11 @@ __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, c
 }
 
 // Radix-2^LR passes from sub-transform length 2^L2S up to 2^L2N.
-template <typename T, int SIGN, int L2N, int L2S, int LR, typename Bt>
+template <typename T, int SIGN, int L2N, int L2S, int LR, bool ROUNDS, typename Bt>
 __device__ __forceinline__ void radixPasses(cx<T>* buf, const Bt& bt, const cx<T>* __restrict__ tw) {
     if constexpr (L2S < L2N) {
-        stockhamPass<T, 1 << LR, SIGN>(buf, L2N, L2S, bt, tw);
-        radixPasses<T, SIGN, L2N, L2S + LR, LR>(buf, bt, tw);
+        stockhamPass<T, 1 << LR, SIGN, Bt, 0, ROUNDS>(buf, L2N, L2S, bt, tw);
+        radixPasses<T, SIGN, L2N, L2S + LR, LR, ROUNDS>(buf, bt, tw);
     }
 }
 
@@ -645,8 +652,12 @@ template <typename T, int SIGN, int L2N = 0, int LR = 4, typename Bt>
 __device__ __forceinline__ void fftLds(cx<T>* buf, int N, int log2N, const Bt& bt, const cx<T>* __restrict__ tw) {
     if constexpr (L2N > 0) {
         constexpr int lead = L2N % LR;
-        if constexpr (lead != 0) stockhamPass<T, 1 << lead, SIGN>(buf, L2N, 0, bt, tw);
-        radixPasses<T, SIGN, L2N, lead, LR>(buf, bt, tw);
+        // rounds only where a caller holds more than kFftRegs * kFftThreads
+        // points: 8 row transforms of 1024 (at most 8 transforms of 512,
+        // or column tiles of at most 4096 points, elsewhere)
+        constexpr bool kRounds = L2N >= 10;
+        if constexpr (lead != 0) stockhamPass<T, 1 << lead, SIGN, Bt, 0, kRounds>(buf, L2N, 0, bt, tw);
+        radixPasses<T, SIGN, L2N, lead, LR, kRounds>(buf, bt, tw);
         return;
     }
     if (log2N < 0)

@@ -507,6 +507,10 @@ struct KernelImpl {
         const size_t fftPoints = static_cast<size_t>(kFftRegs) * kFftThreads;
         if (cfg.Nx > fftPoints || static_cast<size_t>(KT) * cfg.Ny > fftPoints)
             throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx, Ny <= 4096)."};
+        // (the FFT passes' FP32-reciprocal index division is exact while
+        // item + N < 2^15, items below 8 Nx; the LDS bound below is tighter)
+        if (9 * cfg.Nx >= 32768)
+            throw Failure{WV_UNSUPPORTED_OPERATION, "Horizontal grid too large for the in-LDS FFTs (Nx <= 3640)."};
         int maxLds = 0;
         hipCheck(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
         maxLdsPerBlock = static_cast<size_t>(maxLds);

@@ -1141,8 +1141,8 @@ __global__ __launch_bounds__(kFftThreads) void k_xfused(const cx<T>* __restrict_
     __syncthreads();
     if constexpr (PRUNE) {
         const BatchRows<PAIRS> bt{RB * PAIRS, NxP, RS};
-        stockhamPass<T, 16, +1, BatchRows<PAIRS>, 0x3C0u>(buf, 8, 0, bt, tw);
-        stockhamPass<T, 16, +1>(buf, 8, 4, bt, tw);
+        stockhamPass<T, 16, +1, BatchRows<PAIRS>, 0x3C0u, false>(buf, 8, 0, bt, tw);
+        stockhamPass<T, 16, +1, BatchRows<PAIRS>, 0, false>(buf, 8, 4, bt, tw);
     } else {
         fftRows<T, +1, L2N, PAIRS>(buf, Nx, log2Nx, RB * PAIRS, NxP, RS, tw);
     }
