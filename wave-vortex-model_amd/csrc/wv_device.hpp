@@ -81,9 +81,10 @@ struct Mfma<double> {
 // ----------------------------------------------------------------- FFT ----
 // In-place batched complex FFT in LDS, all threads of the block participate.
 // Two batch layouts:
-//   BatchRows<NB1>: transform b is contiguous (unit stride, one pad slot per 16
-//     entries so that radix-16 passes are free of bank conflicts) and starts at
-//     (b % NB1)*sb + (b / NB1)*sb2; consecutive threads walk one transform.
+//   BatchRows<NB1, PAD>: transform b is contiguous (unit stride; PAD: one pad
+//     slot per 16 entries so that radix-16 passes are free of bank conflicts)
+//     and starts at (b % NB1)*sb + (b / NB1)*sb2; consecutive threads walk one
+//     transform.
 //   BatchCols: nb = 2^log2nb interleaved transforms, element n of transform b
 //     at n*nb + b; consecutive threads walk the transforms.
 // X_k = sum_n x_n exp(SIGN*2*pi*i*n*k/N).  Twiddles: for power-of-two N the
@@ -93,12 +94,13 @@ struct Mfma<double> {
 // Power-of-two N runs Stockham autosort passes of radix 16 (then 8, 4 or 2
 // for the remaining factor), each butterfly an in-register DFT, with all index
 // arithmetic in shifts and masks; other N with prime factors up to 13 run
-// mixed-radix passes (powers of two, then 3, 5, 7, 11, 13) with runtime
-// index arithmetic; the rest a direct DFT.  A pass holds at
-// most 16 complex values per thread, so the caller must launch at least
-// nb*N/16 threads.  Every workgroup that runs these transforms has
-// kFftThreads threads.  The caller synchronizes before the call; the function
-// returns after a final barrier.
+// mixed-radix passes (odd and composite radices first, then the powers of
+// two) with runtime index arithmetic; the rest a direct DFT.  A pass holds
+// about 16 complex values per thread: kFftThreads threads take at most
+// kFftRegs * kFftThreads points per round (rows of more points run in rounds
+// of whole transforms where the caller asks for them, ROUNDS).  Every
+// workgroup that runs these transforms has kFftThreads threads.  The caller
+// synchronizes before the call; the function returns after a final barrier.
 constexpr int kFftRegs = 16;
 constexpr int kFftThreads = 256;
 
