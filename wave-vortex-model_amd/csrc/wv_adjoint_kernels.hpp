@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_fieldsadj(const cx<T>* __restrict__ col
 
 // out_a = alpha x_a + beta y_a for the three coefficient arrays
 // (x, y, out may alias elementwise).
-static __global__ __launch_bounds__(256) void k_axpby3(const cx<double>* x0, const cx<double>* x1, const cx<double>* x2,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_axpby3(const cx<double>* x0, const cx<double>* x1, const cx<double>* x2,
                                                 const cx<double>* y0, const cx<double>* y1, const cx<double>* y2,
                                                 double alpha, double beta, int M, cx<double>* o0, cx<double>* o1,
                                                 cx<double>* o2) {

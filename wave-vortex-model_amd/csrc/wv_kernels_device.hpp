@@ -3,7 +3,8 @@
 // them: wv_kernels.hip (the host code, the kernels compiled per
 // power-of-two length) and wv_kernels_rt.hip (the runtime-length FFT
 // kernels, compiled in parallel with it).  Non-template kernels have
-// internal linkage: each unit that launches one holds its own copy.
+// internal linkage (maybe-unused: the runtime-length unit launches none of
+// them): each unit that launches one holds its own copy.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -1299,7 +1300,7 @@ __global__ __launch_bounds__(256) void k_xinv(const cx<T>* __restrict__ G, doubl
 // Nz - 1 zero); imaginary parts cleared on the self-conjugate rows
 // (:1095-1100).  LDS: the 8 columns over z.
 constexpr double kScalarPi = 3.14159265358979323846;
-static __global__ __launch_bounds__(1024) void k_scalarDeriv(cx<double>* __restrict__ C, int Nx, int Ny, int Nz, int nxh,
+[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_scalarDeriv(cx<double>* __restrict__ C, int Nx, int Ny, int Nz, int nxh,
                                                      int nxhP, double Lx, double Ly, const double* __restrict__ D,
                                                      double scale) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -2332,7 +2333,7 @@ __global__ __launch_bounds__(256) void k_projfwd(const cx<T>* __restrict__ Q, co
 }
 
 // Deterministic two-pass FP64 energy reduction.
-static __global__ __launch_bounds__(256) void k_energy_partial(const cx<double>* __restrict__ Ap,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_energy_partial(const cx<double>* __restrict__ Ap,
                                                         const cx<double>* __restrict__ Am,
                                                         const cx<double>* __restrict__ A0,
                                                         const double* __restrict__ ApmTE,
@@ -2353,7 +2354,7 @@ static __global__ __launch_bounds__(256) void k_energy_partial(const cx<double>*
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
-static __global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__ partial, int n,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__ partial, int n,
                                                       double* __restrict__ out) {
     __shared__ double s[256];
     double acc = 0.0;
