@@ -279,6 +279,29 @@ def c3_leg(wvamd, steps=10, runs=3):
             "stage_ms_per_launch_mixed": {r["name"]: round(r["total_ms"] / max(r["launches"], 1), 4) for r in rec}}
 
 
+def runtime_length_leg(wvamd, n=240, steps=10, runs=3):
+    """FP64 and mixed RK4 steps/s at a horizontal length that is not a power
+    of two (240 = 15 x 16: runtime-length in-LDS FFTs), same vertical grid,
+    flow and CFL step as the headline workload, beside the headline rate per
+    grid point."""
+    import numpy as np
+    cfg = wvamd.make_config(n, n, 129, 85, Lx=200e3 * n / 256, Ly=200e3 * n / 256, Lz=1300.0, N0=5.2e-3,
+                            latitude=33.0)
+    out = {"workload": f"wave-vortex nonlinear flux + fixed RK4, {n}x{n}x129 nonhydrostatic constant N",
+           "steps": steps, "runs": runs}
+    for prec in ("fp64", "mixed-split16"):
+        k = wvamd.WVTransformConstantStratificationKernel(cfg, prec, 0)
+        state, dt = initial_state(k)
+        secs, _ = timed_steps(k, state, dt, steps, 1, runs)
+        sps = steps / float(np.median(secs))
+        key = "fp64" if prec == "fp64" else "mixed"
+        out[f"{key}_steps_per_s"] = round(sps, 2)
+        out[f"{key}_Mpoint_steps_per_s"] = round(sps * n * n * 129 / 1e6, 1)
+        del k
+    out["mixed_over_fp64"] = round(out["mixed_steps_per_s"] / out["fp64_steps_per_s"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,7 +314,7 @@ def main():
     ap.add_argument("--profile", choices=["mixed", "mixed-fp32", "fp64", "both"], default=None,
                     help="run only `steps` RK4 steps of the workload plus a 512 MiB copy kernel "
                          "(the byte-counter calibration), for rocprofv3")
-    ap.add_argument("--no-c3", action="store_true", help="skip the arbitrary-stratification (512x512x65) leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the arbitrary-stratification (512x512x65) and runtime-length (240x240x129) legs")
     ap.add_argument("--profile-c3", choices=["mixed", "fp64"], default=None,
                     help="run only `steps` RK4 steps of the 512x512x65 arbitrary-stratification workload")
     ap.add_argument("--profile-scalar", action="store_true",
@@ -392,6 +415,7 @@ def main():
     if not args.no_c3:
         del k32
         c3 = c3_leg(wvamd)
+    rt = None if args.no_c3 else runtime_length_leg(wvamd)
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -420,7 +444,7 @@ def main():
         "host_memory_flux_per_s": {"mixed": round(host32, 2), "fp64": round(host64, 2),
                                    "note": "PCIe-inclusive: state copied in and flux copied out per call"},
         "stage_ms_per_launch_mixed": stages, "kernels_mixed": kernels, "scalar_advection": scalar_leg,
-        "arbitrary_stratification_512x512x65": c3, "cpu_baseline": cpu,
+        "arbitrary_stratification_512x512x65": c3, "runtime_length_240x240x129": rt, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
