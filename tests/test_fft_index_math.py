@@ -50,7 +50,7 @@ def _pfa(v, p1, p2, sign):
     return out
 
 
-@pytest.mark.parametrize("p1,p2", [(3, 2), (5, 2), (3, 4), (7, 2), (3, 5)])
+@pytest.mark.parametrize("p1,p2", [(3, 2), (5, 2), (3, 4), (7, 2), (3, 5), (9, 2)])
 def test_prime_factor_composite_radices(p1, p2):
     rng = np.random.default_rng(p1 * p2)
     x = rng.standard_normal(p1 * p2) + 1j * rng.standard_normal(p1 * p2)
@@ -75,13 +75,16 @@ def _radices(n, nb):
         out.append(15)
         e[3] -= 1
         e[5] -= 1
-    # the power of two's remainder beyond its radix-16 passes with one odd factor
-    for rem, p, r in ((1, 3, 6), (2, 3, 12), (1, 5, 10), (1, 7, 14)):
-        if p2 % 4 == rem and e[p] and single(r):
+    # the power of two's remainder beyond its radix-16 passes with odd factors
+    for rem, p, cnt, r in ((1, 3, 2, 18), (1, 3, 1, 6), (2, 3, 1, 12), (1, 5, 1, 10), (1, 7, 1, 14)):
+        if p2 % 4 == rem and e[p] >= cnt and single(r):
             out.append(r)
-            e[p] -= 1
+            e[p] -= cnt
             p2 -= rem
             break
+    while e[3] >= 2 and single(9):
+        out.append(9)
+        e[3] -= 2
     for p in (3, 5, 7, 11, 13):
         out += [p] * e[p]
     out += [16] * (p2 // 4)
@@ -108,8 +111,28 @@ def _stockham(x, radices, sign):
     return buf
 
 
+def test_radix9_cooley_tukey():
+    """dftCt9: n = 3 n1 + n2, k = k1 + 3 k2, DFT3 along n1, twiddles
+    W9^(n2 k1), DFT3 along n2."""
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(9) + 1j * rng.standard_normal(9)
+    for sign in (-1, 1):
+        y = np.zeros((3, 3), complex)
+        for n2 in range(3):
+            t = _dft(np.array([x[n2], x[3 + n2], x[6 + n2]]), sign)
+            for k1 in range(3):
+                y[k1, n2] = t[k1] * np.exp(sign * 2j * np.pi * n2 * k1 / 9)
+        out = np.zeros(9, complex)
+        for k1 in range(3):
+            t = _dft(y[k1], sign)
+            for k2 in range(3):
+                out[k1 + 3 * k2] = t[k2]
+        assert np.abs(out - _dft(x, sign)).max() < 1e-12
+
+
 @pytest.mark.parametrize("n,nb", [(240, 4), (240, 16), (96, 8), (192, 4), (200, 8), (288, 8), (60, 2),
-                                  (120, 8), (45, 4), (39, 8), (26, 2), (22, 2), (768, 8)])
+                                  (120, 8), (45, 4), (39, 8), (26, 2), (22, 2), (768, 8), (144, 8), (18, 4),
+                                  (36, 8), (576, 4), (90, 8)])
 def test_mixed_radix_schedule_matches_numpy(n, nb):
     radices = _radices(n, nb)
     assert np.prod(radices) == n

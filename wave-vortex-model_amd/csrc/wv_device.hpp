@@ -417,8 +417,8 @@ __device__ __forceinline__ void dftOddPrime(cx<T>* v) {
 }
 
 // Composite radices of the mixed-radix passes: R = P1 P2 with coprime
-// factors (6 = 3 2, 10 = 5 2, 12 = 3 4, 14 = 7 2, 15 = 3 5), P1 the odd one
-// (3 for 15).
+// factors (6 = 3 2, 10 = 5 2, 12 = 3 4, 14 = 7 2, 15 = 3 5, 18 = 9 2), P1
+// the odd one (3 for 15); 9 = 3 3 by Cooley-Tukey (dftCt9).
 template <int R>
 constexpr int pfaFirst() { return R == 15 ? 3 : (R & 1) ? R : pfaFirst<R / 2>(); }
 template <int R>
@@ -464,12 +464,47 @@ __device__ __forceinline__ void dftPfa(cx<T>* v) {
     }
 }
 
+// In-register DFT of 9 points, natural order in and out, as 3 x 3
+// Cooley-Tukey: with n = 3 n1 + n2 and k = k1 + 3 k2, the DFTs of 3 points
+// along n1, the twiddles W9^(n2 k1) (W9 = exp(SIGN 2 pi i / 9)), then the DFTs
+// of 3 points along n2.
+template <typename T, int SIGN>
+__device__ __forceinline__ void dftCt9(cx<T>* v) {
+    // cos and sin of 2 pi m / 9 for m = n2 k1 = 1, 2, 4
+    constexpr double c9[5] = {1.0, 0.766044443118978, 0.17364817766693041, 0.0, -0.9396926207859083};
+    constexpr double s9[5] = {0.0, 0.6427876096865393, 0.984807753012208, 0.0, 0.3420201433256689};
+    cx<T> y[3][3];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+        cx<T> t[3] = {v[n2], v[3 + n2], v[6 + n2]};
+        dftOddPrime<T, 3, SIGN>(t);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) {
+            const int m = n2 * k1;
+            if (m == 0) {
+                y[k1][n2] = t[k1];
+            } else {
+                const cx<T> w{T(c9[m]), T(SIGN * s9[m])};
+                y[k1][n2] = cmul(t[k1], w);
+            }
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        dftOddPrime<T, 3, SIGN>(y[k1]);
+#pragma unroll
+        for (int k2 = 0; k2 < 3; ++k2) v[k1 + 3 * k2] = y[k1][k2];
+    }
+}
+
 template <typename T, int R, int SIGN>
 __device__ __forceinline__ void dftSmall(cx<T>* v) {
     if constexpr ((R & (R - 1)) == 0)
         dftReg<T, R, SIGN>(v);
     else if constexpr (isPrimeRadix<R>())
         dftOddPrime<T, R, SIGN>(v);
+    else if constexpr (R == 9)
+        dftCt9<T, SIGN>(v);
     else
         dftPfa<T, pfaFirst<R>(), R / pfaFirst<R>(), SIGN>(v);
 }
@@ -546,8 +581,8 @@ __device__ __forceinline__ void passAny(cx<T>* buf, int N, int Ns, const Bt& bt,
     stockhamPassAny<T, R, SIGN, Bt, (kFftRegs + R - 1) / R, ROUNDS>(buf, N, Ns, bt, tw);
 }
 
-// Mixed-radix FFT of a runtime length N (passes of 15, 6, 12, 10 or 14 where
-// they apply, of 3, 5, 7, 11, 13, then radix-16 passes for the power of two
+// Mixed-radix FFT of a runtime length N (passes of 15, 18, 6, 12, 10, 14 or
+// 9 where they apply, of 3, 5, 7, 11, 13, then radix-16 passes for the power of two
 // and 8/4/2 for its remainder) with the full twiddle table; false (nothing
 // done) when N has a prime factor above 13.
 template <typename T, int SIGN, int MAXP = 13, bool ROUNDS = true, typename Bt>
@@ -563,15 +598,18 @@ __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, c
     // odd radices first: the power-of-two passes then scatter with an odd
     // stride Ns (no LDS bank conflicts in an unpadded layout).  Where every
     // butterfly of a pass has its own thread, pairs of factors run as one
-    // composite pass (prime-factor DFTs in registers, one pass fewer): 3 x 5,
-    // then the power of two's remainder beyond its radix-16 passes with an
-    // odd factor (6, 12, 10, 14).
+    // composite pass (DFTs in registers, one pass fewer): 3 x 5, then the
+    // power of two's remainder beyond its radix-16 passes with an odd factor
+    // (18 = 9 x 2, 6, 12, 10, 14), then 9 = 3 x 3.
     int Ns = 1;
     const int nb = batchCount(bt);
     auto single = [&](int R) { return nb * (N / R) <= kFftThreads; };
     for (; e3 > 0 && e5 > 0 && single(15); --e3, --e5, Ns *= 15)
         stockhamPassAny<T, 15, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
-    if ((p2 & 3) == 1 && e3 > 0 && single(6)) {
+    if ((p2 & 3) == 1 && e3 > 1 && single(18)) {
+        stockhamPassAny<T, 18, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
+        e3 -= 2, --p2, Ns *= 18;
+    } else if ((p2 & 3) == 1 && e3 > 0 && single(6)) {
         stockhamPassAny<T, 6, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
         --e3, --p2, Ns *= 6;
     } else if ((p2 & 3) == 2 && e3 > 0 && single(12)) {
@@ -584,6 +622,7 @@ __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, c
         stockhamPassAny<T, 14, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
         --e7, --p2, Ns *= 14;
     }
+    for (; e3 > 1 && single(9); e3 -= 2, Ns *= 9) stockhamPassAny<T, 9, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
     for (; e3 > 0; --e3, Ns *= 3) passAny<T, 3, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
     for (; e5 > 0; --e5, Ns *= 5) passAny<T, 5, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
     for (; e7 > 0; --e7, Ns *= 7) passAny<T, 7, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
