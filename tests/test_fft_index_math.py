@@ -50,7 +50,7 @@ def _pfa(v, p1, p2, sign):
     return out
 
 
-@pytest.mark.parametrize("p1,p2", [(3, 2), (5, 2), (3, 4), (7, 2), (3, 5), (9, 2)])
+@pytest.mark.parametrize("p1,p2", [(3, 2), (5, 2), (3, 4), (7, 2), (3, 5), (9, 2), (5, 4), (3, 8)])
 def test_prime_factor_composite_radices(p1, p2):
     rng = np.random.default_rng(p1 * p2)
     x = rng.standard_normal(p1 * p2) + 1j * rng.standard_normal(p1 * p2)
@@ -76,7 +76,8 @@ def _radices(n, nb):
         e[3] -= 1
         e[5] -= 1
     # the power of two's remainder beyond its radix-16 passes with odd factors
-    for rem, p, cnt, r in ((1, 3, 2, 18), (1, 3, 1, 6), (2, 3, 1, 12), (1, 5, 1, 10), (1, 7, 1, 14)):
+    for rem, p, cnt, r in ((1, 3, 2, 18), (1, 3, 1, 6), (2, 3, 1, 12), (1, 5, 1, 10), (1, 7, 1, 14),
+                           (2, 5, 1, 20), (3, 3, 1, 24)):
         if p2 % 4 == rem and e[p] >= cnt and single(r):
             out.append(r)
             e[p] -= cnt
@@ -132,7 +133,7 @@ def test_radix9_cooley_tukey():
 
 @pytest.mark.parametrize("n,nb", [(240, 4), (240, 16), (96, 8), (192, 4), (200, 8), (288, 8), (60, 2),
                                   (120, 8), (45, 4), (39, 8), (26, 2), (22, 2), (768, 8), (144, 8), (18, 4),
-                                  (36, 8), (576, 4), (90, 8)])
+                                  (36, 8), (576, 4), (90, 8), (320, 8), (384, 8), (20, 4), (24, 8)])
 def test_mixed_radix_schedule_matches_numpy(n, nb):
     radices = _radices(n, nb)
     assert np.prod(radices) == n
