@@ -76,8 +76,7 @@ def _radices(n, nb):
         e[3] -= 1
         e[5] -= 1
     # the power of two's remainder beyond its radix-16 passes with odd factors
-    for rem, p, cnt, r in ((1, 3, 2, 18), (1, 3, 1, 6), (2, 3, 1, 12), (1, 5, 1, 10), (1, 7, 1, 14),
-                           (2, 5, 1, 20), (3, 3, 1, 24)):
+    for rem, p, cnt, r in ((1, 3, 2, 18), (1, 3, 1, 6), (2, 3, 1, 12), (1, 5, 1, 10), (1, 7, 1, 14)):
         if p2 % 4 == rem and e[p] >= cnt and single(r):
             out.append(r)
             e[p] -= cnt

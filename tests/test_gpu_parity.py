@@ -191,8 +191,8 @@ def test_fg_all_derivatives_match_reference_golden(torch_gpu, path):
 # Horizontal lengths that are not powers of two: the in-LDS FFTs run
 # mixed-radix passes (composite 15, 18, 6, 12, 10, 14 and 9 where they apply,
 # 3, 5, 7, 11, 13, then 16 and 8/4/2) for lengths whose prime factors are at
-# most 13 (144 = 9 x 16, 288 = 18 x 16, 384 = 24 x 16, 320 = 20 x 16; 18, 20
-# and 24 in one pass), and a direct DFT for the rest (17 x 19).
+# most 13 (144 = 9 x 16, 288 = 18 x 16, 18 in one pass; 384 = 3 x 16 x 8,
+# 320 = 5 x 16 x 4), and a direct DFT for the rest (17 x 19).
 ODD_SHAPES = [(96, 80, 17, False), (45, 36, 9, False), (26, 22, 9, True), (48, 39, 9, False), (17, 19, 9, False),
               (144, 18, 9, False), (288, 36, 9, True), (384, 20, 9, False), (320, 24, 9, True)]
 

@@ -417,9 +417,10 @@ __device__ __forceinline__ void dftOddPrime(cx<T>* v) {
 }
 
 // Composite radices of the mixed-radix passes: R = P1 P2 with coprime
-// factors (6 = 3 2, 10 = 5 2, 12 = 3 4, 14 = 7 2, 15 = 3 5, 18 = 9 2,
-// 20 = 5 4, 24 = 3 8), P1 the odd one (3 for 15); 9 = 3 3 by Cooley-Tukey
-// (dftCt9).
+// factors (6 = 3 2, 10 = 5 2, 12 = 3 4, 14 = 7 2, 15 = 3 5, 18 = 9 2), P1
+// the odd one (3 for 15); 9 = 3 3 by Cooley-Tukey (dftCt9).  (20 = 5 4 and
+// 24 = 3 8 took 320 and 384 to two passes, 4 % faster, but the larger
+// kernels ran every other runtime length 2-4 % slower: not kept.)
 template <int R>
 constexpr int pfaFirst() { return R == 15 ? 3 : (R & 1) ? R : pfaFirst<R / 2>(); }
 template <int R>
@@ -582,8 +583,8 @@ __device__ __forceinline__ void passAny(cx<T>* buf, int N, int Ns, const Bt& bt,
     stockhamPassAny<T, R, SIGN, Bt, (kFftRegs + R - 1) / R, ROUNDS>(buf, N, Ns, bt, tw);
 }
 
-// Mixed-radix FFT of a runtime length N (passes of 15, 18, 6, 12, 10, 14,
-// 20, 24 or 9 where they apply, of 3, 5, 7, 11, 13, then radix-16 passes for the power of two
+// Mixed-radix FFT of a runtime length N (passes of 15, 18, 6, 12, 10, 14 or
+// 9 where they apply, of 3, 5, 7, 11, 13, then radix-16 passes for the power of two
 // and 8/4/2 for its remainder) with the full twiddle table; false (nothing
 // done) when N has a prime factor above 13.
 template <typename T, int SIGN, int MAXP = 13, bool ROUNDS = true, typename Bt>
@@ -601,7 +602,7 @@ __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, c
     // butterfly of a pass has its own thread, pairs of factors run as one
     // composite pass (DFTs in registers, one pass fewer): 3 x 5, then the
     // power of two's remainder beyond its radix-16 passes with odd factors
-    // (18 = 9 x 2, 6, 12, 10, 14, 20, 24), then 9 = 3 x 3.
+    // (18 = 9 x 2, 6, 12, 10, 14), then 9 = 3 x 3.
     int Ns = 1;
     const int nb = batchCount(bt);
     auto single = [&](int R) { return nb * (N / R) <= kFftThreads; };
@@ -622,12 +623,6 @@ __device__ __forceinline__ bool mixedRadixFft(cx<T>* buf, int N, const Bt& bt, c
     } else if ((p2 & 3) == 1 && e7 > 0 && single(14)) {
         stockhamPassAny<T, 14, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
         --e7, --p2, Ns *= 14;
-    } else if ((p2 & 3) == 2 && e5 > 0 && single(20)) {
-        stockhamPassAny<T, 20, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
-        --e5, p2 -= 2, Ns *= 20;
-    } else if ((p2 & 3) == 3 && e3 > 0 && single(24)) {
-        stockhamPassAny<T, 24, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
-        --e3, p2 -= 3, Ns *= 24;
     }
     for (; e3 > 1 && single(9); e3 -= 2, Ns *= 9) stockhamPassAny<T, 9, SIGN, Bt, 1, ROUNDS>(buf, N, Ns, bt, tw);
     for (; e3 > 0; --e3, Ns *= 3) passAny<T, 3, SIGN, ROUNDS>(buf, N, Ns, bt, tw);
